@@ -1,0 +1,175 @@
+"""ctypes view of the oracle's C restatement (oracle/libcpu_ref.so).
+
+TEST INFRASTRUCTURE ONLY: the checker and the timed CPU baseline.  Used by
+tests/, the smoke check and bench.py's cpu_baseline leg; never by the product.
+Arrays are numpy uint64 in the blst limb layouts: fr = 4 u64 Montgomery,
+G1 affine = 12 u64 (x || y), G1 Jacobian = 18 u64 (X || Y || Z).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcpu_ref.so")
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH) or \
+            os.path.getmtime(LIB_PATH) < os.path.getmtime(os.path.join(HERE, "cpu_ref.c")):
+        subprocess.run(["make", "-C", HERE, "-B" if force else "all"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        S = ctypes.c_size_t
+        U = ctypes.c_uint
+        U64 = ctypes.c_uint64
+        sig = {
+            "cpu_ref_set_threads": ([ctypes.c_int], None),
+            "cpu_ref_get_threads": ([], ctypes.c_int),
+            "cpu_ref_booth_index": ([ctypes.c_int, ctypes.c_int, P], ctypes.c_int32),
+            "cpu_ref_window_size": ([S], ctypes.c_int),
+            "cpu_ref_msm_best": ([P, P, S, P], None),
+            "cpu_ref_msm_parallel": ([P, P, S, P], None),
+            "cpu_ref_msm_specific": ([P, P, S, P], None),
+            "cpu_ref_g1_to_affine": ([P, P], None),
+            "cpu_ref_compute_twiddles": ([P, U, P], None),
+            "cpu_ref_best_fft_with_twiddles": ([P, P, U], None),
+            "cpu_ref_fft_coeff_to_extended": ([P, P, U, S], None),
+            "cpu_ref_root_of_unity": ([U, P], None),
+            "cpu_ref_fr_inv": ([P, P], None),
+            "cpu_ref_fr_mul": ([P, P, P], None),
+            "cpu_ref_ntt": ([P, U], None),
+            "cpu_ref_intt": ([P, U], None),
+            "cpu_ref_coeff_to_extended": ([P, U, U, P], None),
+            "cpu_ref_extended_to_coeff": ([P, U], None),
+            "cpu_ref_divide_by_vanishing_poly": ([P, U, U], None),
+            "cpu_ref_fr_from_seed": ([U64, U64, U64, S, P], None),
+            "cpu_ref_g1_generator": ([P], None),
+            "cpu_ref_srs_bases": ([P, S, P], None),
+            "cpu_ref_poly_eval": ([P, S, P, P], None),
+            "cpu_ref_g1_mul_generator": ([P, P], None),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def set_threads(t: int):
+    lib().cpu_ref_set_threads(int(t))
+
+
+def fr_from_seed(seed: int, stream: int, n: int, offset: int = 0) -> np.ndarray:
+    out = np.zeros((n, 4), dtype=np.uint64)
+    lib().cpu_ref_fr_from_seed(seed, stream, offset, n, _p(out))
+    return out
+
+
+def srs_bases(tau_mont: np.ndarray, n: int) -> np.ndarray:
+    out = np.zeros((n, 12), dtype=np.uint64)
+    lib().cpu_ref_srs_bases(_p(np.ascontiguousarray(tau_mont, dtype=np.uint64)), n, _p(out))
+    return out
+
+
+def msm_best(scalars: np.ndarray, bases: np.ndarray) -> np.ndarray:
+    out = np.zeros(18, dtype=np.uint64)
+    lib().cpu_ref_msm_best(_p(scalars), _p(bases), scalars.shape[0], _p(out))
+    return out
+
+
+def msm_parallel(scalars: np.ndarray, bases: np.ndarray) -> np.ndarray:
+    out = np.zeros(18, dtype=np.uint64)
+    lib().cpu_ref_msm_parallel(_p(scalars), _p(bases), scalars.shape[0], _p(out))
+    return out
+
+
+def msm_specific(scalars: np.ndarray, bases: np.ndarray) -> np.ndarray:
+    out = np.zeros(18, dtype=np.uint64)
+    lib().cpu_ref_msm_specific(_p(scalars), _p(bases), scalars.shape[0], _p(out))
+    return out
+
+
+def g1_to_affine(jac: np.ndarray) -> np.ndarray:
+    out = np.zeros(12, dtype=np.uint64)
+    lib().cpu_ref_g1_to_affine(_p(np.ascontiguousarray(jac, dtype=np.uint64)), _p(out))
+    return out
+
+
+def g1_mul_generator(s_mont: np.ndarray) -> np.ndarray:
+    out = np.zeros(18, dtype=np.uint64)
+    lib().cpu_ref_g1_mul_generator(_p(np.ascontiguousarray(s_mont, dtype=np.uint64)), _p(out))
+    return g1_to_affine(out)
+
+
+def poly_eval(coeffs: np.ndarray, x_mont: np.ndarray) -> np.ndarray:
+    out = np.zeros(4, dtype=np.uint64)
+    lib().cpu_ref_poly_eval(_p(coeffs), coeffs.shape[0], _p(np.ascontiguousarray(x_mont, dtype=np.uint64)), _p(out))
+    return out
+
+
+def ntt(a: np.ndarray, log_n: int) -> np.ndarray:
+    b = np.ascontiguousarray(a, dtype=np.uint64).copy()
+    lib().cpu_ref_ntt(_p(b), log_n)
+    return b
+
+
+def intt(a: np.ndarray, log_n: int) -> np.ndarray:
+    b = np.ascontiguousarray(a, dtype=np.uint64).copy()
+    lib().cpu_ref_intt(_p(b), log_n)
+    return b
+
+
+def coeff_to_extended(a: np.ndarray, k: int, ext_k: int) -> np.ndarray:
+    out = np.zeros((1 << ext_k, 4), dtype=np.uint64)
+    lib().cpu_ref_coeff_to_extended(_p(np.ascontiguousarray(a, dtype=np.uint64)), k, ext_k, _p(out))
+    return out
+
+
+def extended_to_coeff(a: np.ndarray, ext_k: int) -> np.ndarray:
+    b = np.ascontiguousarray(a, dtype=np.uint64).copy()
+    lib().cpu_ref_extended_to_coeff(_p(b), ext_k)
+    return b
+
+
+def divide_by_vanishing_poly(a: np.ndarray, k: int, ext_k: int) -> np.ndarray:
+    b = np.ascontiguousarray(a, dtype=np.uint64).copy()
+    lib().cpu_ref_divide_by_vanishing_poly(_p(b), k, ext_k)
+    return b
+
+
+# ---- conversions between numpy limb arrays and Python ints (canonical)
+def fr_rows_to_ints(a: np.ndarray):
+    from . import bls12_381 as O
+    out = []
+    for row in np.asarray(a, dtype=np.uint64).reshape(-1, 4):
+        out.append(O.fr_from_mont(O.from_limbs(int(v) for v in row)))
+    return out
+
+
+def ints_to_fr_rows(vals) -> np.ndarray:
+    from . import bls12_381 as O
+    out = np.zeros((len(vals), 4), dtype=np.uint64)
+    for i, v in enumerate(vals):
+        out[i] = O.to_limbs(O.fr_to_mont(v), 4)
+    return out

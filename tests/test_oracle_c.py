@@ -1,0 +1,127 @@
+"""Pins the oracle's C restatement (the CPU baseline) against the Python oracle,
+brute force and closed forms (CPU only)."""
+import random
+
+import numpy as np
+import pytest
+
+from oracle import bls12_381 as O
+from oracle import cpu_ref as C
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _lib():
+    C.build()
+    C.set_threads(8)
+
+
+def aff_rows_to_points(rows):
+    return [O.g1_affine_from_mont_limbs(r) for r in np.asarray(rows).reshape(-1, 12)]
+
+
+def test_seed_stream_matches_python():
+    a = C.fr_from_seed(1234, 7, 50, offset=3)
+    assert C.fr_rows_to_ints(a) == [O.fr_from_seed(1234, 7, 3 + i) for i in range(50)]
+
+
+def test_booth_index_matches_python():
+    rng = random.Random(5)
+    for _ in range(200):
+        s = rng.randrange(O.R_MOD)
+        rep = O.fr_to_bytes_le(s)
+        buf = np.frombuffer(rep, dtype=np.uint8).copy()
+        c = rng.randrange(1, 17)
+        w = rng.randrange(O.FR_BITS // c + 1)
+        assert C.lib().cpu_ref_booth_index(w, c, buf.ctypes.data) == O.get_booth_index(w, c, rep)
+
+
+def test_window_size_rule():
+    for n in [1, 3, 4, 31, 32, 100, 8103, 8104, 1 << 16, 1 << 20]:
+        assert C.lib().cpu_ref_window_size(n) == O.msm_window_size(n)
+
+
+def test_srs_bases_match_python():
+    tau = O.fr_from_seed(99, 1, 0)
+    rows = C.srs_bases(C.ints_to_fr_rows([tau])[0], 20)
+    pts = aff_rows_to_points(rows)
+    for i in [0, 1, 2, 7, 19]:
+        assert pts[i] == O.srs_point(tau, i)
+
+
+def test_msm_parallel_small_vs_naive():
+    rng = random.Random(8)
+    n = 40
+    pts = [O.g1_mul(O.G1_GEN, rng.randrange(1, O.R_MOD)) for _ in range(n)]
+    sc = [rng.randrange(O.R_MOD) for _ in range(n)]
+    sc[3] = 0
+    bases = np.array([O.g1_affine_mont_limbs(P) for P in pts], dtype=np.uint64)
+    scal = C.ints_to_fr_rows(sc)
+    ref = O.msm_naive(sc, pts)
+    for fn in (C.msm_parallel, C.msm_best, C.msm_specific):
+        got = O.g1_jacobian_from_mont_limbs(fn(scal, bases))
+        assert got == ref
+
+
+def test_msm_best_batch_affine_path_closed_form():
+    # n >= 8104 so that c >= 10 and msm_best takes the batch-affine scheduler path
+    n = 9000
+    tau = O.fr_from_seed(5, 1, 0)
+    bases = C.srs_bases(C.ints_to_fr_rows([tau])[0], n)
+    scal = C.fr_from_seed(5, 2, n)
+    expect = O.msm_srs_closed_form(C.fr_rows_to_ints(scal), tau)
+    assert O.g1_jacobian_from_mont_limbs(C.msm_best(scal, bases)) == expect
+    assert O.g1_jacobian_from_mont_limbs(C.msm_parallel(scal, bases)) == expect
+
+
+def test_msm_best_exceptional_cases():
+    # few distinct bases and their negatives force doubling and P + (-P) inside
+    # batch_add (msm.rs:88-105) and the Jacobian fallback buckets
+    rng = random.Random(21)
+    n = 8200
+    distinct = [O.g1_mul(O.G1_GEN, rng.randrange(1, O.R_MOD)) for _ in range(3)]
+    choice = [rng.randrange(6) for _ in range(n)]
+    pts = [distinct[c % 3] if c < 3 else O.g1_neg(distinct[c % 3]) for c in choice]
+    pts[10] = None                      # identity base (dropped by msm_best)
+    sc = [rng.randrange(1 << 20) for _ in range(n)]   # small scalars: few windows, dense buckets
+    sc[11] = 0
+    sc[12] = O.R_MOD - 1
+    totals = [0, 0, 0]
+    for s, c, P in zip(sc, choice, pts):
+        if P is None:
+            continue
+        totals[c % 3] += s if c < 3 else -s
+    expect = O.g1_sum([O.g1_mul(distinct[i], totals[i] % O.R_MOD) for i in range(3)])
+    bases = np.array([O.g1_affine_mont_limbs(P) for P in pts], dtype=np.uint64)
+    scal = C.ints_to_fr_rows(sc)
+    for fn in (C.msm_best, C.msm_parallel, C.msm_specific):
+        assert O.g1_jacobian_from_mont_limbs(fn(scal, bases)) == expect
+
+
+@pytest.mark.parametrize("log_n", list(range(1, 11)))
+def test_ntt_matches_python(log_n):
+    vals = O.fr_vector_from_seed(3, log_n, 1 << log_n)
+    a = C.ints_to_fr_rows(vals)
+    out = C.ntt(a, log_n)
+    assert C.fr_rows_to_ints(out) == O.dft(vals, O.root_of_unity(log_n))
+    assert C.fr_rows_to_ints(C.intt(out, log_n)) == vals
+
+
+def test_ntt_2_16_horner_samples():
+    log_n = 16
+    a = C.fr_from_seed(17, 1, 1 << log_n)
+    vals = C.fr_rows_to_ints(a)
+    out = C.fr_rows_to_ints(C.ntt(a, log_n))
+    w = O.root_of_unity(log_n)
+    for j in [0, 1, 12345, (1 << 16) - 1]:
+        assert out[j] == O.poly_eval(vals, pow(w, j, O.R_MOD))
+
+
+@pytest.mark.parametrize("k,ext_k", [(1, 3), (3, 5), (6, 8), (8, 10)])
+def test_coset_extension_matches_python(k, ext_k):
+    vals = O.fr_vector_from_seed(4, k, 1 << k)
+    ext = C.coeff_to_extended(C.ints_to_fr_rows(vals), k, ext_k)
+    assert C.fr_rows_to_ints(ext) == O.coeff_to_extended(vals, ext_k)
+    back = C.extended_to_coeff(ext, ext_k)
+    assert C.fr_rows_to_ints(back) == vals + [0] * ((1 << ext_k) - (1 << k))
+    div = C.divide_by_vanishing_poly(ext, k, ext_k)
+    assert C.fr_rows_to_ints(div) == O.divide_by_vanishing_poly(O.coeff_to_extended(vals, ext_k), k, ext_k)
