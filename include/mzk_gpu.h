@@ -1,0 +1,141 @@
+/*
+ * mzk_gpu.h -- C ABI of the MI355X (gfx950) prover hot path: BLS12-381 G1
+ * multi-scalar multiplication behind KZG commitments and scalar-field NTTs behind
+ * EvaluationDomain.
+ *
+ * Conventions (all entry points):
+ *   - return MZK_OK (0) or a non-zero status; never abort; mzk_last_error() holds
+ *     a thread-local message for the last failure on the calling thread;
+ *   - fr elements are 4 x u64 little-endian Montgomery limbs (R = 2^256), the
+ *     memory image of the reference's `Fq` / blst_fr (curves/src/bls12_381/fq.rs:27-29);
+ *   - G1 affine points are 12 x u64 (x[6] || y[6], Montgomery, R = 2^384; identity
+ *     all zero), the image of `G1Affine` / blst_p1_affine (g1.rs:32-34);
+ *   - G1 projective results are 18 x u64 Jacobian (X || Y || Z, identity Z = 0),
+ *     the image of `G1Projective` / blst_p1 (g1.rs:493-495);
+ *   - `_dev` entry points take device pointers (HBM-resident data) and run on the
+ *     context's stream; host-pointer entry points copy in, compute and copy out;
+ *   - calls on one context are serialised by the context; independent threads may
+ *     use independent contexts concurrently.
+ */
+#ifndef MZK_GPU_H
+#define MZK_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MZK_OK = 0,
+  MZK_ERR_INVALID = 1,     /* bad argument: length mismatch, size out of range, null pointer */
+  MZK_ERR_HIP = 2,         /* HIP runtime failure */
+  MZK_ERR_NOMEM = 3,       /* device allocation failed or a budget was exceeded */
+  MZK_ERR_UNSUPPORTED = 4  /* valid request outside what this build implements */
+};
+
+/* Message for the last failure on this thread ("" if none). */
+const char *mzk_last_error(void);
+/* Library ABI version (major * 100 + minor). */
+int mzk_version(void);
+
+/* ------------------------------------------------------------------ context */
+typedef struct mzk_ctx mzk_ctx;
+/* Binds a context to HIP device `device`; owns one non-blocking stream and the
+ * scratch memory its calls need. */
+int mzk_ctx_create(int device, mzk_ctx **out);
+void mzk_ctx_destroy(mzk_ctx *ctx);
+/* The context's hipStream_t, for callers that record events or order work. */
+void *mzk_ctx_stream(mzk_ctx *ctx);
+int mzk_ctx_synchronize(mzk_ctx *ctx);
+
+/* ------------------------------------------------------------------ MSM
+ * Replaces curves/src/msm.rs:460 `msm_best(coeffs: &[C::Scalar], bases: &[C]) -> C::Curve`
+ * (panics on length mismatch there; here both slices share the length n).
+ * Host pointers; out = sum_i coeffs[i] * bases[i] as Jacobian.  n = 0 gives the identity. */
+int mzk_msm_best(mzk_ctx *ctx, const uint64_t *coeffs, const uint64_t *bases, size_t n,
+                 uint64_t out[18]);
+/* Replaces proofs/src/poly/kzg/msm.rs:174 `msm_specific(coeffs, bases) -> C::Curve`:
+ * zero scalars contribute nothing; all-zero (or empty) input gives the identity. */
+int mzk_msm_specific(mzk_ctx *ctx, const uint64_t *coeffs, const uint64_t *bases, size_t n,
+                     uint64_t out[18]);
+/* Same result with device-resident scalars and bases. */
+int mzk_msm_dev(mzk_ctx *ctx, const void *d_coeffs, const void *d_bases, size_t n,
+                uint64_t out[18]);
+
+/* ------------------------------------------------------------------ resident SRS
+ * Replaces the bases side of proofs/src/poly/kzg/params.rs:22-37 (`ParamsKZG::g`) and
+ * `bases::<B>()` (params.rs:83-93): the bases stay in HBM across commits. */
+typedef struct mzk_srs mzk_srs;
+/* Upload n affine bases (host pointer). */
+int mzk_srs_create(mzk_ctx *ctx, const uint64_t *bases, size_t n, mzk_srs **out);
+/* g_i = [tau^i] G1 for i < n, computed on the device (params.rs:154-172, unsafe_setup). */
+int mzk_srs_generate(mzk_ctx *ctx, const uint64_t tau[4], size_t n, mzk_srs **out);
+size_t mzk_srs_len(const mzk_srs *srs);
+/* Device pointer to the n x 12 u64 affine bases. */
+const void *mzk_srs_bases_dev(const mzk_srs *srs);
+/* Copies bases[offset, offset+count) to host memory. */
+int mzk_srs_download(mzk_srs *srs, size_t offset, size_t count, uint64_t *out);
+void mzk_srs_destroy(mzk_srs *srs);
+/* KZG commit against resident bases (kzg/mod.rs:82-108 commit_many -> msm_specific):
+ * sum_i coeffs[i] * g[offset + i], coeffs device-resident. */
+int mzk_msm_srs_dev(mzk_ctx *ctx, const mzk_srs *srs, size_t offset, const void *d_coeffs,
+                    size_t n, uint64_t out[18]);
+
+/* ------------------------------------------------------------------ NTT
+ * Replaces curves/src/fft.rs:49 `best_fft(a: &mut [G], omega, log_n)` for G = Fq:
+ * in-place A_j = sum_i a_i omega^(ij), natural order, host pointer, n = 2^log_n.
+ * omega must have order dividing 2^log_n (as in the reference, it is not checked). */
+int mzk_best_fft(mzk_ctx *ctx, uint64_t *a, const uint64_t omega[4], uint32_t log_n);
+
+/* Replaces proofs/src/poly/domain.rs:34-167 `EvaluationDomain::new(j, k)`:
+ * k, extended_k = min{e >= k : 2^e >= 2^k (j-1)}, omega tables resident on device. */
+typedef struct mzk_domain mzk_domain;
+int mzk_domain_create(mzk_ctx *ctx, uint32_t j, uint32_t k, mzk_domain **out);
+void mzk_domain_destroy(mzk_domain *d);
+int mzk_domain_info(const mzk_domain *d, uint32_t *k, uint32_t *extended_k);
+/* domain.rs:249-264 lagrange_to_coeff: in place on 2^k device elements. */
+int mzk_lagrange_to_coeff_dev(mzk_domain *d, void *d_a);
+/* domain.rs:268-277 coeff_to_lagrange: in place on 2^k device elements. */
+int mzk_coeff_to_lagrange_dev(mzk_domain *d, void *d_a);
+/* domain.rs:285-305 coeff_to_extended: 2^k coefficients in, 2^extended_k evaluations
+ * on the zeta coset out (d_in and d_out must not overlap). */
+int mzk_coeff_to_extended_dev(mzk_domain *d, const void *d_in, void *d_out);
+/* domain.rs:312-328 extended_to_coeff: in place on 2^extended_k device elements. */
+int mzk_extended_to_coeff_dev(mzk_domain *d, void *d_a);
+/* domain.rs:361-380 divide_by_vanishing_poly: in place on 2^extended_k elements. */
+int mzk_divide_by_vanishing_poly_dev(mzk_domain *d, void *d_a);
+/* Host-pointer forms of the four transforms. */
+int mzk_lagrange_to_coeff(mzk_domain *d, uint64_t *a);
+int mzk_coeff_to_extended(mzk_domain *d, const uint64_t *in, uint64_t *out);
+int mzk_extended_to_coeff(mzk_domain *d, uint64_t *a);
+
+/* ------------------------------------------------------------------ synthetic inputs
+ * Element i of the seeded stream (seed, stream): eight splitmix64 words
+ * w_j = splitmix64(key + 8 (offset + i) + j), key = splitmix64(seed ^ stream * 0xD1B54A32D192ED03),
+ * reduced as Fq::from_uniform_bytes (fq.rs:712-736).  Writes n Montgomery elements. */
+int mzk_fr_from_seed_dev(mzk_ctx *ctx, uint64_t seed, uint64_t stream, uint64_t offset, size_t n,
+                         void *d_out);
+
+/* ------------------------------------------------------------------ diagnostics
+ * Element-wise field / curve operations for parity tests (device pointers, n elements):
+ *   field 0 = fr (8 u32 / element), 1 = fp (12 u32 / element);
+ *   op 0 = mul, 1 = add, 2 = sub, 3 = square, 4 = from_mont. */
+int mzk_diag_field_op(mzk_ctx *ctx, int field, int op, const void *d_a, const void *d_b, void *d_out,
+                      size_t n);
+/* G1: op 0 = affine + affine (through XYZZ), 1 = 2 * affine, 2 = XYZZ + XYZZ of the
+ * two affine inputs each doubled first; outputs Jacobian (18 u64 / element). */
+int mzk_diag_g1_op(mzk_ctx *ctx, int op, const void *d_a, const void *d_b, void *d_out, size_t n);
+/* Micro-benchmarks (time in ms from HIP events):
+ *   which 0: independent 32x32+64 multiply-adds, result = mads issued;
+ *   which 1: fp Montgomery products, many independent chains (throughput);
+ *   which 2: fp Montgomery products, one wave, one dependent chain (latency);
+ *   which 3: fr Montgomery products, many chains (throughput). */
+int mzk_diag_bench(mzk_ctx *ctx, int which, int blocks, int threads, int iters, double *ms,
+                   double *ops);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MZK_GPU_H */

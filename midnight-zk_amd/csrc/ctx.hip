@@ -1,0 +1,152 @@
+// Context, error reporting and the seeded input generator.
+#include <stdarg.h>
+#include <string.h>
+
+#include "ctx.hpp"
+#include "field.hpp"
+
+namespace mzk {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+int Scratch::reserve(size_t bytes) {
+  if (bytes <= cap) return MZK_OK;
+  if (ptr) {
+    MZK_HIP(hipFree(ptr));
+    ptr = nullptr;
+    cap = 0;
+  }
+  size_t want = bytes + bytes / 8;
+  hipError_t e = hipMalloc(&ptr, want);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    set_error("hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    return MZK_ERR_NOMEM;
+  }
+  cap = want;
+  return MZK_OK;
+}
+
+void Scratch::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+}
+
+// ---------------------------------------------------------------- seeded stream
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+uint64_t splitmix64_host(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// Fq::from_uniform_bytes (fq.rs:712-736): a0 * R2 + a1 * R3 as Montgomery products.
+__global__ void k_fr_from_seed(uint64_t key, uint64_t offset, size_t n, uint32_t* __restrict__ out) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  constexpr uint32_t R2[8] = {0xf3f29c6du, 0xc999e990u, 0x87925c23u, 0x2b6cedcbu,
+                              0x7254398fu, 0x05d31496u, 0x9f59ff11u, 0x0748d9d9u};
+  constexpr uint32_t R3[8] = {0x439b73afu, 0xc62c1807u, 0x8cf06990u, 0x1b3e0d18u,
+                              0xc7b5f418u, 0x73d13c71u, 0xc8db33e9u, 0x6e2a5bb9u};
+  uint64_t idx = offset + i;
+  fr_t a0, a1, r2, r3;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    uint64_t w0 = splitmix64(key + idx * 8 + j);
+    uint64_t w1 = splitmix64(key + idx * 8 + 4 + j);
+    a0.v[2 * j] = (uint32_t)w0;
+    a0.v[2 * j + 1] = (uint32_t)(w0 >> 32);
+    a1.v[2 * j] = (uint32_t)w1;
+    a1.v[2 * j + 1] = (uint32_t)(w1 >> 32);
+  }
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    r2.v[j] = R2[j];
+    r3.v[j] = R3[j];
+  }
+  fr_t r = add(mul(a0, r2), mul(a1, r3));
+  store<FrP>(out + 8 * i, r);
+}
+
+}  // namespace mzk
+
+using namespace mzk;
+
+extern "C" {
+
+const char* mzk_last_error(void) { return g_err; }
+
+int mzk_version(void) { return 100; }
+
+int mzk_ctx_create(int device, mzk_ctx** out) {
+  MZK_REQUIRE(out != nullptr, "mzk_ctx_create: null out");
+  int ndev = 0;
+  MZK_HIP(hipGetDeviceCount(&ndev));
+  MZK_REQUIRE(device >= 0 && device < ndev, "mzk_ctx_create: device %d out of range (%d devices)",
+              device, ndev);
+  MZK_HIP(hipSetDevice(device));
+  mzk_ctx* c = new mzk_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    set_error("hipStreamCreateWithFlags: %s", hipGetErrorString(e));
+    return MZK_ERR_HIP;
+  }
+  hipDeviceProp_t prop;
+  MZK_HIP(hipGetDeviceProperties(&prop, device));
+  c->num_cus = prop.multiProcessorCount;
+  c->lds_per_block = prop.sharedMemPerBlock;
+  *out = c;
+  return MZK_OK;
+}
+
+void mzk_ctx_destroy(mzk_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& s : c->scratch) s.release();
+  if (c->host_staging) (void)hipHostFree(c->host_staging);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+void* mzk_ctx_stream(mzk_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int mzk_ctx_synchronize(mzk_ctx* c) {
+  MZK_REQUIRE(c != nullptr, "null context");
+  MZK_HIP(hipSetDevice(c->device));
+  MZK_HIP(hipStreamSynchronize(c->stream));
+  return MZK_OK;
+}
+
+int mzk_fr_from_seed_dev(mzk_ctx* c, uint64_t seed, uint64_t stream, uint64_t offset, size_t n,
+                         void* d_out) {
+  MZK_REQUIRE(c != nullptr, "null context");
+  if (n == 0) return MZK_OK;
+  MZK_REQUIRE(d_out != nullptr, "mzk_fr_from_seed_dev: null output");
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  uint64_t key = splitmix64_host(seed ^ (stream * 0xD1B54A32D192ED03ull));
+  hipLaunchKernelGGL(k_fr_from_seed, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, key, offset, n,
+                     (uint32_t*)d_out);
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
+}
+
+}  // extern "C"

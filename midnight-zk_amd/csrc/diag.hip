@@ -1,0 +1,163 @@
+// Element-wise field / curve kernels for parity tests, and the micro-benchmarks
+// that measure the chip's 32x32+64 multiply-add rate (the roof for MSM and NTT).
+#include "ctx.hpp"
+#include "g1.hpp"
+
+namespace mzk {
+
+template <class P>
+__global__ void k_field_op(int op, const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
+                           uint32_t* __restrict__ out, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  constexpr int N = P::N;
+  Field<P> x = load<P>(a + N * i);
+  Field<P> y = load<P>(b + N * i);
+  Field<P> r;
+  switch (op) {
+    case 0: r = mul(x, y); break;
+    case 1: r = add(x, y); break;
+    case 2: r = sub(x, y); break;
+    case 3: r = sqr(x); break;
+    default: r = from_mont(x); break;
+  }
+  store<P>(out + N * i, r);
+}
+
+__global__ void k_g1_op(int op, const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
+                        uint32_t* __restrict__ out, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  g1_aff_t p = load_affine(a + 24 * i);
+  g1_aff_t q = load_affine(b + 24 * i);
+  g1_xyzz_t r;
+  if (op == 0) {
+    r = g1_add_affine(g1_xyzz_t::from_affine(p), q);
+  } else if (op == 1) {
+    r = g1_add_affine(g1_xyzz_t::from_affine(p), p);
+  } else {
+    g1_xyzz_t p2 = g1_dbl(g1_xyzz_t::from_affine(p));
+    g1_xyzz_t q2 = g1_dbl(g1_xyzz_t::from_affine(q));
+    r = g1_add(p2, q2);
+  }
+  fp_t X, Y, Z;
+  g1_to_jacobian(r, X, Y, Z);
+  store<FpP>(out + 36 * i, X);
+  store<FpP>(out + 36 * i + 12, Y);
+  store<FpP>(out + 36 * i + 24, Z);
+}
+
+// 8 independent multiply-add chains per thread.
+__global__ void k_bench_mad(int iters, const uint32_t* __restrict__ seed, uint64_t* __restrict__ sink) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t x = seed[0] ^ t;
+  uint32_t y = seed[1] + t;
+  uint64_t acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) acc[k] = (uint64_t)seed[2 + k] + t;
+  for (int it = 0; it < iters; it++) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc[k] = (uint64_t)((uint32_t)acc[k] ^ x) * y + (acc[k] >> 32);
+  }
+  uint64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) s ^= acc[k];
+  if (s == 0x123456789abcdefull) sink[t] = s;
+}
+
+template <class P>
+__global__ void k_bench_mul(int iters, const uint32_t* __restrict__ src, uint32_t* __restrict__ sink) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  Field<P> x = load<P>(src);
+  Field<P> y = load<P>(src + 16);
+  x.v[0] ^= t;
+  for (int it = 0; it < iters; it++) x = mul(x, y);
+  if (x.v[0] == 0x12345678u && x.v[1] == t) store<P>(sink, x);
+}
+
+}  // namespace mzk
+
+using namespace mzk;
+
+extern "C" {
+
+int mzk_diag_field_op(mzk_ctx* c, int field, int op, const void* a, const void* b, void* out, size_t n) {
+  MZK_REQUIRE(c != nullptr, "null context");
+  MZK_REQUIRE(field == 0 || field == 1, "field must be 0 (fr) or 1 (fp)");
+  MZK_REQUIRE(op >= 0 && op <= 4, "op out of range");
+  if (n == 0) return MZK_OK;
+  MZK_REQUIRE(a && b && out, "null pointer");
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  if (field == 0)
+    hipLaunchKernelGGL(k_field_op<FrP>, dim3(grid_for(n, 128)), dim3(128), 0, c->stream, op,
+                       (const uint32_t*)a, (const uint32_t*)b, (uint32_t*)out, n);
+  else
+    hipLaunchKernelGGL(k_field_op<FpP>, dim3(grid_for(n, 128)), dim3(128), 0, c->stream, op,
+                       (const uint32_t*)a, (const uint32_t*)b, (uint32_t*)out, n);
+  MZK_HIP(hipGetLastError());
+  MZK_HIP(hipStreamSynchronize(c->stream));
+  return MZK_OK;
+}
+
+int mzk_diag_g1_op(mzk_ctx* c, int op, const void* a, const void* b, void* out, size_t n) {
+  MZK_REQUIRE(c != nullptr, "null context");
+  MZK_REQUIRE(op >= 0 && op <= 2, "op out of range");
+  if (n == 0) return MZK_OK;
+  MZK_REQUIRE(a && b && out, "null pointer");
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_g1_op, dim3(grid_for(n, 64)), dim3(64), 0, c->stream, op, (const uint32_t*)a,
+                     (const uint32_t*)b, (uint32_t*)out, n);
+  MZK_HIP(hipGetLastError());
+  MZK_HIP(hipStreamSynchronize(c->stream));
+  return MZK_OK;
+}
+
+int mzk_diag_bench(mzk_ctx* c, int which, int blocks, int threads, int iters, double* ms, double* ops) {
+  MZK_REQUIRE(c != nullptr && ms && ops, "null argument");
+  MZK_REQUIRE(which >= 0 && which <= 3, "which out of range");
+  MZK_REQUIRE(blocks > 0 && threads > 0 && threads <= 1024 && iters > 0, "bad launch shape");
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  size_t sink_bytes = (size_t)blocks * threads * 8;
+  MZK_TRY(c->scratch[SCR_IN0].reserve(sink_bytes + 4096));
+  uint32_t* buf = (uint32_t*)c->scratch[SCR_IN0].ptr;
+  // src: two fp-sized operands of small fixed values (any values exercise the same code)
+  uint32_t host[32];
+  for (int i = 0; i < 32; i++) host[i] = 0x01020304u * (i + 1) & 0x0fffffffu;
+  MZK_HIP(hipMemcpyAsync(buf, host, sizeof host, hipMemcpyHostToDevice, c->stream));
+  uint32_t* sink = buf + 64;
+  hipEvent_t e0, e1;
+  MZK_HIP(hipEventCreate(&e0));
+  MZK_HIP(hipEventCreate(&e1));
+  MZK_HIP(hipEventRecord(e0, c->stream));
+  double per_thread = 0;
+  switch (which) {
+    case 0:
+      hipLaunchKernelGGL(k_bench_mad, dim3(blocks), dim3(threads), 0, c->stream, iters, buf, (uint64_t*)sink);
+      per_thread = 8.0 * iters;
+      break;
+    case 1:
+    case 2:
+      hipLaunchKernelGGL(k_bench_mul<FpP>, dim3(blocks), dim3(threads), 0, c->stream, iters, buf, sink);
+      per_thread = iters;
+      break;
+    default:
+      hipLaunchKernelGGL(k_bench_mul<FrP>, dim3(blocks), dim3(threads), 0, c->stream, iters, buf, sink);
+      per_thread = iters;
+      break;
+  }
+  MZK_HIP(hipGetLastError());
+  MZK_HIP(hipEventRecord(e1, c->stream));
+  MZK_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  MZK_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = t;
+  *ops = per_thread * (double)blocks * threads;
+  return MZK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,315 @@
+"""Python host mirror of the reference's MSM / NTT interface over the C ABI.
+
+Loads the in-tree gfx950 library libmzk_gpu.so (built by build.py) through ctypes
+and exposes the reference's names:
+
+  msm_best(coeffs, bases)            curves/src/msm.rs:460
+  msm_specific(coeffs, bases)        proofs/src/poly/kzg/msm.rs:174
+  best_fft(a, omega, log_n)          curves/src/fft.rs:49
+  EvaluationDomain(j, k)             proofs/src/poly/domain.rs:54
+     .lagrange_to_coeff / .coeff_to_lagrange / .coeff_to_extended /
+     .extended_to_coeff / .divide_by_vanishing_poly
+  ParamsKZG-style resident bases     proofs/src/poly/kzg/params.rs:22-37,154
+
+Host arrays are numpy uint64 in the blst limb layouts (fr = 4 limbs, G1 affine =
+12, G1 Jacobian = 18).  Device arrays are torch tensors (torch provides device
+memory and nothing else).  Errors raise MzkError, mirroring the reference's
+panics on bad lengths (msm.rs:461, fft.rs:62).  There is no CPU fallback: if the
+library or a GPU is missing, calls fail loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmzk_gpu.so")
+
+
+class MzkError(RuntimeError):
+    pass
+
+
+_lib = None
+
+# (name, argtypes, restype); every int-returning function is a status code
+_P = ctypes.c_void_p
+_S = ctypes.c_size_t
+_U32 = ctypes.c_uint32
+_U64 = ctypes.c_uint64
+_I = ctypes.c_int
+_D = ctypes.POINTER(ctypes.c_double)
+SIGNATURES = {
+    "mzk_last_error": ([], ctypes.c_char_p),
+    "mzk_version": ([], _I),
+    "mzk_ctx_create": ([_I, ctypes.POINTER(_P)], _I),
+    "mzk_ctx_destroy": ([_P], None),
+    "mzk_ctx_stream": ([_P], _P),
+    "mzk_ctx_synchronize": ([_P], _I),
+    "mzk_msm_best": ([_P, _P, _P, _S, _P], _I),
+    "mzk_msm_specific": ([_P, _P, _P, _S, _P], _I),
+    "mzk_msm_dev": ([_P, _P, _P, _S, _P], _I),
+    "mzk_srs_create": ([_P, _P, _S, ctypes.POINTER(_P)], _I),
+    "mzk_srs_generate": ([_P, _P, _S, ctypes.POINTER(_P)], _I),
+    "mzk_srs_len": ([_P], _S),
+    "mzk_srs_bases_dev": ([_P], _P),
+    "mzk_srs_download": ([_P, _S, _S, _P], _I),
+    "mzk_srs_destroy": ([_P], None),
+    "mzk_msm_srs_dev": ([_P, _P, _S, _P, _S, _P], _I),
+    "mzk_best_fft": ([_P, _P, _P, _U32], _I),
+    "mzk_domain_create": ([_P, _U32, _U32, ctypes.POINTER(_P)], _I),
+    "mzk_domain_destroy": ([_P], None),
+    "mzk_domain_info": ([_P, ctypes.POINTER(_U32), ctypes.POINTER(_U32)], _I),
+    "mzk_lagrange_to_coeff_dev": ([_P, _P], _I),
+    "mzk_coeff_to_lagrange_dev": ([_P, _P], _I),
+    "mzk_coeff_to_extended_dev": ([_P, _P, _P], _I),
+    "mzk_extended_to_coeff_dev": ([_P, _P], _I),
+    "mzk_divide_by_vanishing_poly_dev": ([_P, _P], _I),
+    "mzk_lagrange_to_coeff": ([_P, _P], _I),
+    "mzk_coeff_to_extended": ([_P, _P, _P], _I),
+    "mzk_extended_to_coeff": ([_P, _P], _I),
+    "mzk_fr_from_seed_dev": ([_P, _U64, _U64, _U64, _S, _P], _I),
+    "mzk_diag_field_op": ([_P, _I, _I, _P, _P, _P, _S], _I),
+    "mzk_diag_g1_op": ([_P, _I, _P, _P, _P, _S], _I),
+    "mzk_diag_bench": ([_P, _I, _I, _I, _I, _D, _D], _I),
+}
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libmzk_gpu.so and binds every declared symbol (raises if any is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise MzkError(f"{path} is not built; run midnight-zk_amd/build.py")
+    L = ctypes.CDLL(path)
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(L, name)  # AttributeError if the export is missing
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = load_library().mzk_last_error().decode(errors="replace")
+        raise MzkError(f"mzk status {rc}: {msg}")
+
+
+def _np_ptr(a: np.ndarray):
+    if a.dtype != np.uint64 or not a.flags["C_CONTIGUOUS"]:
+        raise MzkError("expected a C-contiguous uint64 array")
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _t_ptr(t) -> int:
+    if not t.is_cuda or not t.is_contiguous():
+        raise MzkError("expected a contiguous device tensor")
+    return t.data_ptr()
+
+
+class Context:
+    """One device context (stream + scratch).  Mirrors no reference type: the
+    reference runs on the rayon pool; this is the device it runs on."""
+
+    def __init__(self, device: int = 0):
+        L = load_library()
+        h = ctypes.c_void_p()
+        _check(L.mzk_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            load_library().mzk_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        return load_library().mzk_ctx_stream(self.h) or 0
+
+    def synchronize(self):
+        _check(load_library().mzk_ctx_synchronize(self.h))
+
+    # ------------------------------------------------------------------ MSM
+    def msm_best(self, coeffs: np.ndarray, bases: np.ndarray) -> np.ndarray:
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.uint64).reshape(-1, 4)
+        bases = np.ascontiguousarray(bases, dtype=np.uint64).reshape(-1, 12)
+        if coeffs.shape[0] != bases.shape[0]:
+            raise MzkError("msm_best: coeffs and bases differ in length")
+        out = np.zeros(18, dtype=np.uint64)
+        _check(load_library().mzk_msm_best(self.h, _np_ptr(coeffs), _np_ptr(bases), coeffs.shape[0], _np_ptr(out)))
+        return out
+
+    def msm_specific(self, coeffs: np.ndarray, bases: np.ndarray) -> np.ndarray:
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.uint64).reshape(-1, 4)
+        bases = np.ascontiguousarray(bases, dtype=np.uint64).reshape(-1, 12)
+        if coeffs.shape[0] != bases.shape[0]:
+            raise MzkError("msm_specific: coeffs and bases differ in length")
+        out = np.zeros(18, dtype=np.uint64)
+        _check(load_library().mzk_msm_specific(self.h, _np_ptr(coeffs), _np_ptr(bases), coeffs.shape[0], _np_ptr(out)))
+        return out
+
+    def msm_dev(self, d_coeffs, d_bases, n: int) -> np.ndarray:
+        out = np.zeros(18, dtype=np.uint64)
+        _check(load_library().mzk_msm_dev(self.h, _t_ptr(d_coeffs), _t_ptr(d_bases), n, _np_ptr(out)))
+        return out
+
+    # ------------------------------------------------------------------ NTT
+    def best_fft(self, a: np.ndarray, omega: np.ndarray, log_n: int) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=np.uint64).reshape(-1, 4).copy()
+        if a.shape[0] != (1 << log_n):
+            raise MzkError("best_fft: len(a) != 2^log_n")
+        om = np.ascontiguousarray(omega, dtype=np.uint64).reshape(4)
+        _check(load_library().mzk_best_fft(self.h, _np_ptr(a), _np_ptr(om), log_n))
+        return a
+
+    def domain(self, j: int, k: int) -> "EvaluationDomain":
+        return EvaluationDomain(self, j, k)
+
+    def srs_generate(self, tau_mont: np.ndarray, n: int) -> "ResidentSrs":
+        return ResidentSrs.generate(self, tau_mont, n)
+
+    def srs_upload(self, bases: np.ndarray) -> "ResidentSrs":
+        return ResidentSrs.upload(self, bases)
+
+    # ------------------------------------------------------------------ inputs
+    def fr_from_seed_dev(self, seed: int, stream: int, n: int, out, offset: int = 0):
+        _check(load_library().mzk_fr_from_seed_dev(self.h, seed, stream, offset, n, _t_ptr(out)))
+        return out
+
+
+class EvaluationDomain:
+    """proofs/src/poly/domain.rs:34-167; transforms on device tensors of shape
+    (len, 4) uint64 (viewed as int64 by torch)."""
+
+    def __init__(self, ctx: Context, j: int, k: int):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(load_library().mzk_domain_create(ctx.h, j, k, ctypes.byref(h)))
+        self.h = h
+        kk, ek = _U32(), _U32()
+        _check(load_library().mzk_domain_info(self.h, ctypes.byref(kk), ctypes.byref(ek)))
+        self.k = kk.value
+        self.extended_k = ek.value
+
+    def close(self):
+        if self.h:
+            load_library().mzk_domain_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def lagrange_to_coeff(self, d_a):
+        _check(load_library().mzk_lagrange_to_coeff_dev(self.h, _t_ptr(d_a)))
+        return d_a
+
+    def coeff_to_lagrange(self, d_a):
+        _check(load_library().mzk_coeff_to_lagrange_dev(self.h, _t_ptr(d_a)))
+        return d_a
+
+    def coeff_to_extended(self, d_in, d_out):
+        _check(load_library().mzk_coeff_to_extended_dev(self.h, _t_ptr(d_in), _t_ptr(d_out)))
+        return d_out
+
+    def extended_to_coeff(self, d_a):
+        _check(load_library().mzk_extended_to_coeff_dev(self.h, _t_ptr(d_a)))
+        return d_a
+
+    def divide_by_vanishing_poly(self, d_a):
+        _check(load_library().mzk_divide_by_vanishing_poly_dev(self.h, _t_ptr(d_a)))
+        return d_a
+
+    # host-pointer forms (copy in / out)
+    def lagrange_to_coeff_host(self, a: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=np.uint64).reshape(-1, 4).copy()
+        _check(load_library().mzk_lagrange_to_coeff(self.h, _np_ptr(a)))
+        return a
+
+    def coeff_to_extended_host(self, a: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=np.uint64).reshape(-1, 4)
+        out = np.zeros((1 << self.extended_k, 4), dtype=np.uint64)
+        _check(load_library().mzk_coeff_to_extended(self.h, _np_ptr(a), _np_ptr(out)))
+        return out
+
+    def extended_to_coeff_host(self, a: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=np.uint64).reshape(-1, 4).copy()
+        _check(load_library().mzk_extended_to_coeff(self.h, _np_ptr(a)))
+        return a
+
+
+class ResidentSrs:
+    """Affine bases resident in HBM (proofs/src/poly/kzg/params.rs:22-37)."""
+
+    def __init__(self, ctx: Context, h):
+        self.ctx = ctx
+        self.h = h
+        self.n = load_library().mzk_srs_len(self.h)
+
+    @classmethod
+    def generate(cls, ctx: Context, tau_mont: np.ndarray, n: int) -> "ResidentSrs":
+        h = ctypes.c_void_p()
+        tau = np.ascontiguousarray(tau_mont, dtype=np.uint64).reshape(4)
+        _check(load_library().mzk_srs_generate(ctx.h, _np_ptr(tau), n, ctypes.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def upload(cls, ctx: Context, bases: np.ndarray) -> "ResidentSrs":
+        h = ctypes.c_void_p()
+        b = np.ascontiguousarray(bases, dtype=np.uint64).reshape(-1, 12)
+        _check(load_library().mzk_srs_create(ctx.h, _np_ptr(b), b.shape[0], ctypes.byref(h)))
+        return cls(ctx, h)
+
+    def download(self, offset: int = 0, count: int | None = None) -> np.ndarray:
+        count = self.n - offset if count is None else count
+        out = np.zeros((count, 12), dtype=np.uint64)
+        _check(load_library().mzk_srs_download(self.h, offset, count, _np_ptr(out)))
+        return out
+
+    def bases_ptr(self) -> int:
+        return load_library().mzk_srs_bases_dev(self.h)
+
+    def commit(self, d_coeffs, n: int | None = None, offset: int = 0) -> np.ndarray:
+        """KZG commit of device coefficients against bases[offset:offset+n]."""
+        n = d_coeffs.shape[0] if n is None else n
+        out = np.zeros(18, dtype=np.uint64)
+        _check(load_library().mzk_msm_srs_dev(self.ctx.h, self.h, offset, _t_ptr(d_coeffs), n, _np_ptr(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            load_library().mzk_srs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def diag_field_op(ctx: Context, field: int, op: int, d_a, d_b, d_out, n: int):
+    _check(load_library().mzk_diag_field_op(ctx.h, field, op, _t_ptr(d_a), _t_ptr(d_b), _t_ptr(d_out), n))
+
+
+def diag_g1_op(ctx: Context, op: int, d_a, d_b, d_out, n: int):
+    _check(load_library().mzk_diag_g1_op(ctx.h, op, _t_ptr(d_a), _t_ptr(d_b), _t_ptr(d_out), n))
+
+
+def diag_bench(ctx: Context, which: int, blocks: int, threads: int, iters: int):
+    ms, ops = ctypes.c_double(), ctypes.c_double()
+    _check(load_library().mzk_diag_bench(ctx.h, which, blocks, threads, iters, ctypes.byref(ms), ctypes.byref(ops)))
+    return ms.value, ops.value
