@@ -86,7 +86,7 @@ int mzk_msm_srs_dev(mzk_ctx *ctx, const mzk_srs *srs, size_t offset, const void 
 /* ------------------------------------------------------------------ NTT
  * Replaces curves/src/fft.rs:49 `best_fft(a: &mut [G], omega, log_n)` for G = Fq:
  * in-place A_j = sum_i a_i omega^(ij), natural order, host pointer, n = 2^log_n.
- * omega must have order dividing 2^log_n (as in the reference, it is not checked). */
+ * omega must be a primitive 2^log_n-th root of unity (not checked, as in the reference). */
 int mzk_best_fft(mzk_ctx *ctx, uint64_t *a, const uint64_t omega[4], uint32_t log_n);
 
 /* Replaces proofs/src/poly/domain.rs:34-167 `EvaluationDomain::new(j, k)`:
@@ -118,6 +118,17 @@ int mzk_extended_to_coeff(mzk_domain *d, uint64_t *a);
 int mzk_fr_from_seed_dev(mzk_ctx *ctx, uint64_t seed, uint64_t stream, uint64_t offset, size_t n,
                          void *d_out);
 
+/* ------------------------------------------------------------------ kernel timing
+ * When enabled, every kernel the context launches is bracketed by HIP events on
+ * its stream.  mzk_prof_read waits for them and writes one line per kernel name,
+ * "<name> <total ms> <launches>\n", then clears the totals.  Enabling or disabling
+ * also clears them.  Timing adds event overhead: leave it off for timed runs. */
+int mzk_prof_enable(mzk_ctx *ctx, int on);
+int mzk_prof_read(mzk_ctx *ctx, char *buf, size_t len);
+/* Plan of the last MSM run on the context: out = {window bits c, windows W, bucket
+ * sets, references per accumulation thread K, non-zero window digits M, n}. */
+int mzk_msm_last_plan(mzk_ctx *ctx, uint64_t out[6]);
+
 /* ------------------------------------------------------------------ diagnostics
  * Element-wise field / curve operations for parity tests (device pointers, n elements):
  *   field 0 = fr (8 u32 / element), 1 = fp (12 u32 / element);
@@ -131,7 +142,9 @@ int mzk_diag_g1_op(mzk_ctx *ctx, int op, const void *d_a, const void *d_b, void 
  *   which 0: independent 32x32+64 multiply-adds, result = mads issued;
  *   which 1: fp Montgomery products, many independent chains (throughput);
  *   which 2: fp Montgomery products, one wave, one dependent chain (latency);
- *   which 3: fr Montgomery products, many chains (throughput). */
+ *   which 3: fr Montgomery products, many chains (throughput);
+ *   which 4: dependent 32x32+64 multiply-add chains with no other instruction
+ *            (v_mad_u64_u32 issue rate: the INT32 multiply roof of the chip). */
 int mzk_diag_bench(mzk_ctx *ctx, int which, int blocks, int threads, int iters, double *ms,
                    double *ops);
 

@@ -40,6 +40,33 @@ void Scratch::release() {
   cap = 0;
 }
 
+void KernelProfiler::begin(hipStream_t st, const char* name) {
+  Pending p;
+  p.name = name;
+  if (hipEventCreate(&p.start) != hipSuccess || hipEventCreate(&p.stop) != hipSuccess) return;
+  (void)hipEventRecord(p.start, st);
+  pending.push_back(p);
+}
+
+void KernelProfiler::end(hipStream_t st) {
+  if (!pending.empty()) (void)hipEventRecord(pending.back().stop, st);
+}
+
+int KernelProfiler::collect() {
+  for (auto& p : pending) {
+    MZK_HIP(hipEventSynchronize(p.stop));
+    float ms = 0;
+    MZK_HIP(hipEventElapsedTime(&ms, p.start, p.stop));
+    auto& t = totals[p.name];
+    t.first += ms;
+    t.second += 1;
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
+  }
+  pending.clear();
+  return MZK_OK;
+}
+
 // ---------------------------------------------------------------- seeded stream
 __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
   x += 0x9E3779B97F4A7C15ull;
@@ -132,6 +159,33 @@ int mzk_ctx_synchronize(mzk_ctx* c) {
   MZK_REQUIRE(c != nullptr, "null context");
   MZK_HIP(hipSetDevice(c->device));
   MZK_HIP(hipStreamSynchronize(c->stream));
+  return MZK_OK;
+}
+
+int mzk_prof_enable(mzk_ctx* c, int on) {
+  MZK_REQUIRE(c != nullptr, "null context");
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  MZK_TRY(c->prof.collect());
+  c->prof.totals.clear();
+  c->prof.on = on != 0;
+  return MZK_OK;
+}
+
+int mzk_prof_read(mzk_ctx* c, char* buf, size_t len) {
+  MZK_REQUIRE(c != nullptr && buf != nullptr && len > 0, "mzk_prof_read: null argument");
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  MZK_TRY(c->prof.collect());
+  std::string out;
+  char line[256];
+  for (auto& kv : c->prof.totals) {
+    snprintf(line, sizeof line, "%s %.6f %ld\n", kv.first.c_str(), kv.second.first, kv.second.second);
+    out += line;
+  }
+  c->prof.totals.clear();
+  MZK_REQUIRE(out.size() < len, "mzk_prof_read: buffer too small (%zu bytes needed)", out.size() + 1);
+  memcpy(buf, out.c_str(), out.size() + 1);
   return MZK_OK;
 }
 

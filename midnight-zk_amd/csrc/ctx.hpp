@@ -1,6 +1,9 @@
 // The context object behind the opaque mzk_ctx handle.
 #pragma once
+#include <map>
 #include <mutex>
+#include <string>
+#include <vector>
 
 #include "common.hpp"
 
@@ -27,7 +30,29 @@ enum ScratchSlot {
 
 uint64_t splitmix64_host(uint64_t x);
 
+// Per-kernel timing with HIP events on the context stream (off unless enabled).
+struct KernelProfiler {
+  bool on = false;
+  struct Pending {
+    const char* name;
+    hipEvent_t start, stop;
+  };
+  std::vector<Pending> pending;
+  std::map<std::string, std::pair<double, long>> totals;   // name -> (ms, launches)
+  void begin(hipStream_t st, const char* name);
+  void end(hipStream_t st);
+  int collect();   // waits for pending events and folds them into totals
+};
+
 }  // namespace mzk
+
+// Launches a kernel on the context stream, bracketed by profiler events when on.
+#define MZK_LAUNCH(ctx, name, kernel, grid, block, lds, ...)                        \
+  do {                                                                             \
+    if ((ctx)->prof.on) (ctx)->prof.begin((ctx)->stream, name);                     \
+    hipLaunchKernelGGL(kernel, grid, block, lds, (ctx)->stream, __VA_ARGS__);       \
+    if ((ctx)->prof.on) (ctx)->prof.end((ctx)->stream);                             \
+  } while (0)
 
 struct mzk_ctx {
   int device = 0;
@@ -37,4 +62,6 @@ struct mzk_ctx {
   size_t lds_per_block = 0;
   mzk::Scratch scratch[mzk::SCR_COUNT];
   void* host_staging = nullptr;
+  mzk::KernelProfiler prof;
+  uint64_t last_msm[6] = {0, 0, 0, 0, 0, 0};   // c, W, sets, K, M, n of the last MSM
 };

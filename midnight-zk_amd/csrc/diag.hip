@@ -65,6 +65,24 @@ __global__ void k_bench_mad(int iters, const uint32_t* __restrict__ seed, uint64
   if (s == 0x123456789abcdefull) sink[t] = s;
 }
 
+// 8 independent dependent chains acc = lo(acc) * y + acc: one v_mad_u64_u32 per
+// step and nothing else, so the time is the multiply-add issue rate.
+__global__ void k_bench_mad_pure(int iters, const uint32_t* __restrict__ seed, uint64_t* __restrict__ sink) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t y = seed[1] | 1u;
+  uint64_t acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) acc[k] = (uint64_t)seed[2 + k] + t;
+  for (int it = 0; it < iters; it++) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc[k] = (uint64_t)(uint32_t)acc[k] * y + acc[k];
+  }
+  uint64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) s ^= acc[k];
+  if (s == 0x123456789abcdefull) sink[t] = s;
+}
+
 template <class P>
 __global__ void k_bench_mul(int iters, const uint32_t* __restrict__ src, uint32_t* __restrict__ sink) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -116,7 +134,7 @@ int mzk_diag_g1_op(mzk_ctx* c, int op, const void* a, const void* b, void* out, 
 
 int mzk_diag_bench(mzk_ctx* c, int which, int blocks, int threads, int iters, double* ms, double* ops) {
   MZK_REQUIRE(c != nullptr && ms && ops, "null argument");
-  MZK_REQUIRE(which >= 0 && which <= 3, "which out of range");
+  MZK_REQUIRE(which >= 0 && which <= 4, "which out of range");
   MZK_REQUIRE(blocks > 0 && threads > 0 && threads <= 1024 && iters > 0, "bad launch shape");
   std::lock_guard<std::mutex> g(c->mu);
   MZK_HIP(hipSetDevice(c->device));
@@ -142,6 +160,10 @@ int mzk_diag_bench(mzk_ctx* c, int which, int blocks, int threads, int iters, do
     case 2:
       hipLaunchKernelGGL(k_bench_mul<FpP>, dim3(blocks), dim3(threads), 0, c->stream, iters, buf, sink);
       per_thread = iters;
+      break;
+    case 4:
+      hipLaunchKernelGGL(k_bench_mad_pure, dim3(blocks), dim3(threads), 0, c->stream, iters, buf, (uint64_t*)sink);
+      per_thread = 8.0 * iters;
       break;
     default:
       hipLaunchKernelGGL(k_bench_mul<FrP>, dim3(blocks), dim3(threads), 0, c->stream, iters, buf, sink);

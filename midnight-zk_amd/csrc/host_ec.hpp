@@ -190,5 +190,127 @@ inline Affine xyzz_to_affine(const Xyzz& p) {
   return a;
 }
 
+
+// ---------------------------------------------------------------- host fr (r, 4 x u64, R = 2^256)
+struct Fr {
+  uint64_t l[4];
+};
+
+static const uint64_t kR[4] = {0xffffffff00000001ull, 0x53bda402fffe5bfeull, 0x3339d80809a1d805ull,
+                               0x73eda753299d7d48ull};
+static const uint64_t kRInv = 0xfffffffeffffffffull;
+static const uint64_t kFrOne[4] = {0x00000001fffffffeull, 0x5884b7fa00034802ull, 0x998c4fefecbc4ff5ull,
+                                   0x1824b159acc5056full};
+static const uint64_t kFrR2[4] = {0xc999e990f3f29c6dull, 0x2b6cedcb87925c23ull, 0x05d314967254398full,
+                                  0x0748d9d99f59ff11ull};
+// 2^32-th root of unity 7^((r-1)/2^32) (fq.rs:70-77) and zeta = 7^((r-1)/3) (fq.rs:700-710), Montgomery
+static const uint64_t kFrRoot[4] = {0xb9b58d8c5f0e466aull, 0x5b1b4c801819d7ecull, 0x0af53ae352a31e64ull,
+                                    0x5bf3adda19e9b27bull};
+static const uint64_t kFrZeta[4] = {0x92d9090b093011d2ull, 0xfc9cbd719d6aa073ull, 0xc1f14ef0cd65a1a6ull,
+                                    0x017f6d35e72fcdebull};
+
+inline Fr fr_zero() { Fr r; memset(r.l, 0, sizeof r.l); return r; }
+inline Fr fr_one() { Fr r; memcpy(r.l, kFrOne, sizeof r.l); return r; }
+inline Fr fr_const(const uint64_t* v) { Fr r; memcpy(r.l, v, sizeof r.l); return r; }
+inline bool fr_eq(const Fr& a, const Fr& b) { return memcmp(a.l, b.l, sizeof a.l) == 0; }
+
+inline bool fr_ge(const uint64_t* a) {
+  for (int i = 3; i >= 0; i--) {
+    if (a[i] != kR[i]) return a[i] > kR[i];
+  }
+  return true;
+}
+inline void fr_sub_mod(uint64_t* a) {
+  uint64_t br = 0;
+  for (int i = 0; i < 4; i++) {
+    u128 d = (u128)a[i] - kR[i] - br;
+    a[i] = (uint64_t)d;
+    br = (uint64_t)(d >> 64) & 1;
+  }
+}
+
+inline Fr fr_mul(const Fr& a, const Fr& b) {
+  uint64_t t[6] = {0};
+  for (int i = 0; i < 4; i++) {
+    u128 c = 0;
+    for (int j = 0; j < 4; j++) {
+      c = (u128)a.l[j] * b.l[i] + t[j] + (uint64_t)(c >> 64);
+      t[j] = (uint64_t)c;
+    }
+    u128 s = (u128)t[4] + (uint64_t)(c >> 64);
+    t[4] = (uint64_t)s;
+    t[5] = (uint64_t)(s >> 64);
+    uint64_t m = t[0] * kRInv;
+    c = (u128)m * kR[0] + t[0];
+    for (int j = 1; j < 4; j++) {
+      c = (u128)m * kR[j] + t[j] + (uint64_t)(c >> 64);
+      t[j - 1] = (uint64_t)c;
+    }
+    s = (u128)t[4] + (uint64_t)(c >> 64);
+    t[3] = (uint64_t)s;
+    t[4] = t[5] + (uint64_t)(s >> 64);
+  }
+  Fr r;
+  memcpy(r.l, t, sizeof r.l);
+  if (t[4] || fr_ge(r.l)) fr_sub_mod(r.l);
+  return r;
+}
+
+inline Fr fr_sub(const Fr& a, const Fr& b) {
+  Fr r;
+  uint64_t br = 0;
+  for (int i = 0; i < 4; i++) {
+    u128 d = (u128)a.l[i] - b.l[i] - br;
+    r.l[i] = (uint64_t)d;
+    br = (uint64_t)(d >> 64) & 1;
+  }
+  if (br) {
+    uint64_t c = 0;
+    for (int i = 0; i < 4; i++) {
+      u128 s = (u128)r.l[i] + kR[i] + c;
+      r.l[i] = (uint64_t)s;
+      c = (uint64_t)(s >> 64);
+    }
+  }
+  return r;
+}
+
+inline Fr fr_pow(Fr a, uint64_t e) {
+  Fr r = fr_one();
+  while (e) {
+    if (e & 1) r = fr_mul(r, a);
+    a = fr_mul(a, a);
+    e >>= 1;
+  }
+  return r;
+}
+
+// a^(r-2)
+inline Fr fr_inv(const Fr& a) {
+  uint64_t e[4];
+  memcpy(e, kR, sizeof e);
+  e[0] -= 2;
+  Fr r = fr_one();
+  for (int i = 255; i >= 0; i--) {
+    r = fr_mul(r, r);
+    if ((e[i / 64] >> (i % 64)) & 1) r = fr_mul(r, a);
+  }
+  return r;
+}
+
+inline Fr fr_from_u64(uint64_t v) {
+  Fr t;
+  memset(t.l, 0, sizeof t.l);
+  t.l[0] = v;
+  return fr_mul(t, fr_const(kFrR2));
+}
+
+// omega = ROOT^(2^(32 - log_n)) (domain.rs:72-91)
+inline Fr fr_root_of_unity(unsigned log_n) {
+  Fr w = fr_const(kFrRoot);
+  for (unsigned i = log_n; i < 32; i++) w = fr_mul(w, w);
+  return w;
+}
+
 }  // namespace host
 }  // namespace mzk

@@ -313,7 +313,7 @@ static int wsum_sets(mzk_ctx* c, const uint32_t* items, uint32_t sets, uint32_t 
   MZK_REQUIRE(4 * blocks * 48 <= work_words, "msm: reduction workspace too small");
   uint32_t* P = work;
   uint32_t* Q = work + blocks * 48;
-  hipLaunchKernelGGL(k_wsum, dim3((unsigned)blocks), dim3(WS_BLOCK), WS_BLOCK * 48 * 4, c->stream, items,
+  MZK_LAUNCH(c, "k_wsum", k_wsum, dim3((unsigned)blocks), dim3(WS_BLOCK), WS_BLOCK * 48 * 4, items,
                      per_set, gps, P, Q);
   MZK_HIP(hipGetLastError());
   out_ws.assign(sets, host::Xyzz::inf());
@@ -380,13 +380,13 @@ int msm_run(mzk_ctx* c, const MsmPlan& p, const uint32_t* d_scalars, size_t n, c
   uint32_t* work = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
 
   hipStream_t st = c->stream;
-  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(NB, 256)), dim3(256), 0, st, counts, NB, 0u);
-  hipLaunchKernelGGL(k_msm_prep, dim3(grid_for(n, 128)), dim3(128), 0, st, d_scalars, n, p.c, p.W, p.per_window,
+  MZK_LAUNCH(c, "k_fill_u32", k_fill_u32, dim3(grid_for(NB, 256)), dim3(256), 0, counts, NB, 0u);
+  MZK_LAUNCH(c, "k_msm_prep", k_msm_prep, dim3(grid_for(n, 128)), dim3(128), 0, d_scalars, n, p.c, p.W, p.per_window,
                      canon, counts);
-  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, st, counts, off, NB, tsums);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, st, tsums, (int)tiles, off, NB);
-  hipLaunchKernelGGL(k_scan_add, dim3(grid_for(NB, 256)), dim3(256), 0, st, off, NB, tsums, cursor);
-  hipLaunchKernelGGL(k_msm_scatter, dim3(grid_for(n, 128)), dim3(128), 0, st, canon, n, p.c, p.W, p.per_window,
+  MZK_LAUNCH(c, "k_scan_tiles", k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, counts, off, NB, tsums);
+  MZK_LAUNCH(c, "k_scan_sums", k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, tsums, (int)tiles, off, NB);
+  MZK_LAUNCH(c, "k_scan_add", k_scan_add, dim3(grid_for(NB, 256)), dim3(256), 0, off, NB, tsums, cursor);
+  MZK_LAUNCH(c, "k_msm_scatter", k_msm_scatter, dim3(grid_for(n, 128)), dim3(128), 0, canon, n, p.c, p.W, p.per_window,
                      ref_stride, ref_offset, cursor, refs);
   MZK_HIP(hipGetLastError());
   uint32_t M = 0;
@@ -397,9 +397,15 @@ int msm_run(mzk_ctx* c, const MsmPlan& p, const uint32_t* d_scalars, size_t n, c
     return MZK_OK;
   }
   const uint32_t chunks = (M + p.K - 1) / p.K;
-  hipLaunchKernelGGL(k_msm_accum, dim3(grid_for(chunks, 64)), dim3(64), 0, st, refs, M, off, (uint32_t)NB, p.K,
+  c->last_msm[0] = (uint64_t)p.c;
+  c->last_msm[1] = (uint64_t)p.W;
+  c->last_msm[2] = (uint64_t)p.sets;
+  c->last_msm[3] = p.K;
+  c->last_msm[4] = M;
+  c->last_msm[5] = n;
+  MZK_LAUNCH(c, "k_msm_accum", k_msm_accum, dim3(grid_for(chunks, 64)), dim3(64), 0, refs, M, off, (uint32_t)NB, p.K,
                      d_bases, partials);
-  hipLaunchKernelGGL(k_bucket_sum, dim3(grid_for(NB, 64)), dim3(64), 0, st, off, (uint32_t)NB, p.K, partials,
+  MZK_LAUNCH(c, "k_bucket_sum", k_bucket_sum, dim3(grid_for(NB, 64)), dim3(64), 0, off, (uint32_t)NB, p.K, partials,
                      sums);
   MZK_HIP(hipGetLastError());
   std::vector<host::Xyzz> ws, tot;
@@ -463,3 +469,10 @@ int mzk_msm_specific(mzk_ctx* c, const uint64_t* coeffs, const uint64_t* bases, 
 }
 
 }  // extern "C"
+
+extern "C" int mzk_msm_last_plan(mzk_ctx* c, uint64_t out[6]) {
+  MZK_REQUIRE(c != nullptr && out != nullptr, "mzk_msm_last_plan: null argument");
+  std::lock_guard<std::mutex> g(c->mu);
+  memcpy(out, c->last_msm, sizeof c->last_msm);
+  return MZK_OK;
+}

@@ -1,0 +1,479 @@
+// Radix-2 NTTs over the BLS12-381 scalar field for gfx950, behind the
+// EvaluationDomain boundary (proofs/src/poly/domain.rs:54-414) and best_fft
+// (curves/src/fft.rs:49-101, 110-186).
+//
+// A transform of size N = N1 * N2 (N1 = 2^ceil(logN/2), N2 = 2^floor(logN/2),
+// both <= 2^11) runs as two passes of the Bailey four-step method:
+//   pass 1: for each n1 < N1, a size-N2 DFT of the column a[n1 + N1 n2], times the
+//           twist omega_N^(n1 k2), stored at tmp[n1 + N1 k2];
+//   pass 2: for each k2 < N2, a size-N1 DFT of the row tmp[k2 N1 + n1], stored at
+//           out[k2 + N2 k1] -- natural order, so no bit-reversal pass exists.
+// Each sub-DFT is staged through LDS (up to 2^11 elements = 64 KiB per workgroup):
+// loaded in bit-reversed order, log2(L) Cooley-Tukey stages, stored naturally.
+// The zeta-coset distribution and zero padding of coeff_to_extended are fused into
+// the pass-1 load; the 1/N scaling and zeta^-(i mod 3) of the inverse transforms
+// into the pass-2 store.  Results are the unique DFT values, hence identical to the
+// reference's bit-reverse + DIT and pruned-DIF schedules.
+#include <vector>
+
+#include "ctx.hpp"
+#include "field.hpp"
+#include "host_ec.hpp"
+#include "ntt.hpp"
+
+namespace mzk {
+
+static constexpr int NTT_MAX_LOG_L = 11;
+static constexpr int NTT_LDS_ELEMS = 1 << NTT_MAX_LOG_L;
+static constexpr int TWIST_LO_BITS = 11;
+
+struct FrWords {
+  uint32_t v[8];
+};
+
+struct PassArgs {
+  const uint32_t* src;
+  uint32_t* dst;
+  const uint32_t* tw;        // omega_L^i, i < L/2
+  const uint32_t* twist_lo;  // omega_N^i, i < 2^TWIST_LO_BITS        (pass 1 only)
+  const uint32_t* twist_hi;  // omega_N^(i 2^TWIST_LO_BITS)            (pass 1 only)
+  uint64_t in_bstride, in_jstride, out_bstride, out_kstride;
+  uint64_t n_real;           // input indices >= n_real read as zero
+  uint32_t log_l;            // sub-DFT size L = 2^log_l
+  uint32_t C;                // sub-DFTs per workgroup
+  int do_twist;
+  int in_coset;              // input index i times [1, zeta, zeta^2][i mod 3]
+  int out_mode;              // 0 none, 1 times scale, 2 times scale*[1, zeta^2, zeta][k mod 3]
+  FrWords zeta, zeta2;
+  FrWords post[3];           // scale * pattern, indexed by output index mod 3
+};
+
+__device__ __forceinline__ fr_t to_fr(const FrWords& w) {
+  fr_t r;
+#pragma unroll
+  for (int i = 0; i < 8; i++) r.v[i] = w.v[i];
+  return r;
+}
+
+__device__ __forceinline__ uint32_t bitrev(uint32_t j, uint32_t bits) {
+  return bits == 0 ? 0u : (__brev(j) >> (32 - bits));
+}
+
+__global__ void __launch_bounds__(256) k_ntt_pass(PassArgs a) {
+  extern __shared__ uint32_t lds[];   // C * L elements of 8 words
+  const uint32_t L = 1u << a.log_l;
+  const uint32_t CL = a.C * L;
+  const uint64_t b0 = (uint64_t)blockIdx.x * a.C;
+  const uint32_t nt = blockDim.x;
+
+  // load (bit-reversed within each sub-DFT)
+  for (uint32_t e = threadIdx.x; e < CL; e += nt) {
+    const uint32_t c = e % a.C, j = e / a.C;
+    const uint64_t gi = (b0 + c) * a.in_bstride + (uint64_t)j * a.in_jstride;
+    fr_t x;
+    if (gi < a.n_real) {
+      x = load<FrP>(a.src + 8 * gi);
+      if (a.in_coset) {
+        const uint32_t m = (uint32_t)(gi % 3);
+        if (m == 1) x = mul(x, to_fr(a.zeta));
+        else if (m == 2) x = mul(x, to_fr(a.zeta2));
+      }
+    } else {
+      x = fr_t::zero();
+    }
+    store<FrP>(lds + 8 * (c * L + bitrev(j, a.log_l)), x);
+  }
+  __syncthreads();
+
+  // log_l radix-2 DIT stages
+  const uint32_t halfL = L >> 1;
+  for (uint32_t s = 0; s < a.log_l; s++) {
+    const uint32_t half = 1u << s;
+    for (uint32_t u = threadIdx.x; u < a.C * halfL; u += nt) {
+      const uint32_t c = u / halfL, v = u % halfL;
+      const uint32_t pos = v & (half - 1);
+      const uint32_t i = ((v >> s) << (s + 1)) + pos;
+      uint32_t* pa = lds + 8 * (c * L + i);
+      uint32_t* pb = pa + 8 * half;
+      fr_t x = load<FrP>(pa);
+      fr_t y = load<FrP>(pb);
+      if (pos != 0) y = mul(y, load<FrP>(a.tw + 8 * (pos << (a.log_l - 1 - s))));
+      store<FrP>(pa, add(x, y));
+      store<FrP>(pb, sub(x, y));
+    }
+    __syncthreads();
+  }
+
+  // store (natural order), fused twist / scaling
+  for (uint32_t e = threadIdx.x; e < CL; e += nt) {
+    const uint32_t c = e % a.C, k = e / a.C;
+    const uint64_t b = b0 + c;
+    fr_t x = load<FrP>(lds + 8 * (c * L + k));
+    if (a.do_twist) {
+      const uint64_t ex = b * k;   // < N
+      if (ex != 0) {
+        fr_t w = mul(load<FrP>(a.twist_lo + 8 * (ex & ((1u << TWIST_LO_BITS) - 1))),
+                     load<FrP>(a.twist_hi + 8 * (ex >> TWIST_LO_BITS)));
+        x = mul(x, w);
+      }
+    }
+    const uint64_t go = b * a.out_bstride + (uint64_t)k * a.out_kstride;
+    if (a.out_mode == 1) {
+      x = mul(x, to_fr(a.post[0]));
+    } else if (a.out_mode == 2) {
+      x = mul(x, to_fr(a.post[go % 3]));
+    }
+    store<FrP>(a.dst + 8 * go, x);
+  }
+}
+
+// out[i] = prod_{j : bit j of i} pows[j]  (pows[j] = g^(2^j)), i.e. g^i.
+struct PowArgs {
+  FrWords pows[32];
+};
+__global__ void k_pow_table(uint32_t* __restrict__ out, uint32_t count, PowArgs p) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  fr_t r = fr_t::one();
+  for (int j = 0; j < 32; j++)
+    if ((i >> j) & 1) r = mul(r, to_fr(p.pows[j]));
+  store<FrP>(out + 8 * (size_t)i, r);
+}
+
+// a[i] *= t[i & mask]
+__global__ void k_mul_periodic(uint32_t* __restrict__ a, size_t n, const uint32_t* __restrict__ t, uint32_t mask) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  store<FrP>(a + 8 * i, mul(load<FrP>(a + 8 * i), load<FrP>(t + 8 * (i & mask))));
+}
+
+// ----------------------------------------------------------------------- host
+static FrWords words(const host::Fr& f) {
+  FrWords w;
+  memcpy(w.v, f.l, 32);
+  return w;
+}
+
+static int pow_table(hipStream_t st, uint32_t* out, uint32_t count, const host::Fr& g) {
+  PowArgs p;
+  host::Fr x = g;
+  for (int j = 0; j < 32; j++) {
+    p.pows[j] = words(x);
+    x = host::fr_mul(x, x);
+  }
+  hipLaunchKernelGGL(k_pow_table, dim3(grid_for(count, 256)), dim3(256), 0, st, out, count, p);
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
+}
+
+NttPlan::~NttPlan() {
+  if (mem) (void)hipFree(mem);
+}
+
+int NttPlan::init(hipStream_t st, uint32_t log_n_, const host::Fr& omega) {
+  log_n = log_n_;
+  l1 = (log_n + 1) / 2;
+  l2 = log_n / 2;
+  const uint32_t N1 = 1u << l1, N2 = 1u << l2;
+  const uint32_t lo_cnt = 1u << std::min<uint32_t>(TWIST_LO_BITS, log_n);
+  const uint32_t hi_cnt = (uint32_t)std::max<uint64_t>(1, ((uint64_t)1 << log_n) >> TWIST_LO_BITS);
+  const size_t words_needed = 8 * ((size_t)std::max(N1 / 2, 1u) + std::max(N2 / 2, 1u) + lo_cnt + hi_cnt);
+  MZK_HIP(hipMalloc(&mem, words_needed * 4));
+  uint32_t* p = (uint32_t*)mem;
+  tw1 = p;
+  p += 8 * std::max(N1 / 2, 1u);
+  tw2 = p;
+  p += 8 * std::max(N2 / 2, 1u);
+  twist_lo = p;
+  p += 8 * lo_cnt;
+  twist_hi = p;
+  // omega_N1 = omega^N2, omega_N2 = omega^N1
+  host::Fr w1 = omega, w2 = omega;
+  for (uint32_t i = 0; i < l2; i++) w1 = host::fr_mul(w1, w1);
+  for (uint32_t i = 0; i < l1; i++) w2 = host::fr_mul(w2, w2);
+  MZK_TRY(pow_table(st, tw1, std::max(N1 / 2, 1u), w1));
+  MZK_TRY(pow_table(st, tw2, std::max(N2 / 2, 1u), w2));
+  MZK_TRY(pow_table(st, twist_lo, lo_cnt, omega));
+  host::Fr whi = omega;
+  for (int i = 0; i < TWIST_LO_BITS; i++) whi = host::fr_mul(whi, whi);
+  MZK_TRY(pow_table(st, twist_hi, hi_cnt, whi));
+  return MZK_OK;
+}
+
+static uint32_t choose_c(uint32_t L, uint64_t nbatch) {
+  uint32_t cmax = NTT_LDS_ELEMS / L;
+  uint64_t want = nbatch / 1024;   // keep >= ~1024 workgroups when the size allows
+  uint32_t c = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cmax, want));
+  while (nbatch % c) c >>= 1;
+  return std::max(c, 1u);
+}
+
+static int launch_pass(mzk_ctx* c, const char* name, PassArgs a, uint64_t nbatch) {
+  const uint32_t L = 1u << a.log_l;
+  const uint32_t threads = std::min<uint32_t>(256, std::max<uint32_t>(64, a.C * L / 2));
+  const size_t lds = (size_t)a.C * L * 32;
+  MZK_LAUNCH(c, name, k_ntt_pass, dim3((unsigned)(nbatch / a.C)), dim3(threads), lds, a);
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
+}
+
+int ntt_run(mzk_ctx* c, const NttPlan& plan, const uint32_t* src, uint32_t* dst, uint64_t n_real, int in_coset,
+            int out_mode, const host::Fr& scale) {
+  const uint32_t log_n = plan.log_n;
+  const uint64_t N = 1ull << log_n;
+  hipStream_t st = c->stream;
+  host::Fr z = host::fr_const(host::kFrZeta);
+  host::Fr z2 = host::fr_mul(z, z);
+  PassArgs a;
+  memset(&a, 0, sizeof a);
+  a.zeta = words(z);
+  a.zeta2 = words(z2);
+  a.post[0] = words(scale);
+  a.post[1] = words(host::fr_mul(scale, z2));
+  a.post[2] = words(host::fr_mul(scale, z));
+  a.n_real = n_real;
+  a.in_coset = in_coset;
+  if (log_n <= (uint32_t)NTT_MAX_LOG_L) {
+    a.src = src;
+    a.dst = dst;
+    a.tw = plan.twist_lo;   // omega_N^i, i < N/2 is the full sub-DFT table here
+    a.in_bstride = 0;
+    a.in_jstride = 1;
+    a.out_bstride = 0;
+    a.out_kstride = 1;
+    a.log_l = log_n;
+    a.C = 1;
+    a.out_mode = out_mode;
+    return launch_pass(c, "k_ntt_pass_single", a, 1);
+  }
+  const uint64_t N1 = 1ull << plan.l1, N2 = 1ull << plan.l2;
+  MZK_TRY(c->scratch[SCR_NTT].reserve(N * 32));
+  uint32_t* tmp = (uint32_t*)c->scratch[SCR_NTT].ptr;
+  // pass 1: columns (size N2), twist, out-of-place into tmp
+  a.src = src;
+  a.dst = tmp;
+  a.tw = plan.tw2;
+  a.twist_lo = plan.twist_lo;
+  a.twist_hi = plan.twist_hi;
+  a.do_twist = 1;
+  a.in_bstride = 1;
+  a.in_jstride = N1;
+  a.out_bstride = 1;
+  a.out_kstride = N1;
+  a.log_l = plan.l2;
+  a.C = choose_c((uint32_t)N2, N1);
+  a.out_mode = 0;
+  MZK_TRY(launch_pass(c, "k_ntt_pass1", a, N1));
+  // pass 2: rows (size N1), natural-order output
+  PassArgs b = a;
+  b.src = tmp;
+  b.dst = dst;
+  b.tw = plan.tw1;
+  b.do_twist = 0;
+  b.in_coset = 0;
+  b.n_real = N;
+  b.in_bstride = N1;
+  b.in_jstride = 1;
+  b.out_bstride = 1;
+  b.out_kstride = N2;
+  b.log_l = plan.l1;
+  b.C = choose_c((uint32_t)N1, N2);
+  b.out_mode = out_mode;
+  return launch_pass(c, "k_ntt_pass2", b, N2);
+}
+
+}  // namespace mzk
+
+// ----------------------------------------------------------------------- domain
+struct mzk_domain {
+  mzk_ctx* ctx = nullptr;
+  uint32_t k = 0, ext_k = 0;
+  mzk::NttPlan fwd, inv, ext_fwd, ext_inv;
+  mzk::host::Fr n_inv, ext_n_inv;
+  uint32_t* t_evals = nullptr;   // 2^(ext_k - k) values of 1 / ((zeta w_ext^i)^n - 1)
+  uint32_t t_count = 0;
+};
+
+using namespace mzk;
+
+static int domain_check(mzk_domain* d) {
+  MZK_REQUIRE(d != nullptr && d->ctx != nullptr, "null domain");
+  return MZK_OK;
+}
+
+extern "C" {
+
+int mzk_domain_create(mzk_ctx* c, uint32_t j, uint32_t k, mzk_domain** out) {
+  MZK_REQUIRE(c != nullptr && out != nullptr, "mzk_domain_create: null argument");
+  MZK_REQUIRE(j >= 1, "mzk_domain_create: j must be >= 1");
+  MZK_REQUIRE(k >= 1 && k <= 32, "mzk_domain_create: k out of range");
+  // extended_k (domain.rs:64-67)
+  const uint64_t n = 1ull << k;
+  const uint64_t qd = (uint64_t)(j > 1 ? j - 1 : 0);
+  uint32_t ek = k;
+  while (ek < 40 && (1ull << ek) < n * qd) ek++;
+  MZK_REQUIRE(ek <= 32, "mzk_domain_create: extended_k %u exceeds S = 32", ek);
+  if (ek > 2 * NTT_MAX_LOG_L) {
+    set_error("mzk_domain_create: extended_k %u above the supported 2^%d", ek, 2 * NTT_MAX_LOG_L);
+    return MZK_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  mzk_domain* d = new mzk_domain();
+  d->ctx = c;
+  d->k = k;
+  d->ext_k = ek;
+  host::Fr w_ext = host::fr_root_of_unity(ek);
+  host::Fr w = w_ext;
+  for (uint32_t i = k; i < ek; i++) w = host::fr_mul(w, w);
+  host::Fr w_inv = host::fr_inv(w), w_ext_inv = host::fr_inv(w_ext);
+  int rc = d->fwd.init(c->stream, k, w);
+  if (rc == MZK_OK) rc = d->inv.init(c->stream, k, w_inv);
+  if (rc == MZK_OK) rc = d->ext_fwd.init(c->stream, ek, w_ext);
+  if (rc == MZK_OK) rc = d->ext_inv.init(c->stream, ek, w_ext_inv);
+  if (rc != MZK_OK) {
+    delete d;
+    return rc;
+  }
+  d->n_inv = host::fr_inv(host::fr_from_u64(n));
+  d->ext_n_inv = host::fr_inv(host::fr_from_u64(1ull << ek));
+  // t_evaluations (domain.rs:104-140)
+  d->t_count = 1u << (ek - k);
+  std::vector<uint64_t> t(4 * d->t_count);
+  host::Fr zeta = host::fr_const(host::kFrZeta);
+  host::Fr orig = host::fr_pow(zeta, n), step = host::fr_pow(w_ext, n), cur = orig;
+  for (uint32_t i = 0; i < d->t_count; i++) {
+    host::Fr v = host::fr_inv(host::fr_sub(cur, host::fr_one()));
+    memcpy(&t[4 * i], v.l, 32);
+    cur = host::fr_mul(cur, step);
+  }
+  hipError_t e = hipMalloc(&d->t_evals, 32 * (size_t)d->t_count);
+  if (e != hipSuccess) {
+    delete d;
+    set_error("hipMalloc: %s", hipGetErrorString(e));
+    return MZK_ERR_NOMEM;
+  }
+  MZK_HIP(hipMemcpyAsync(d->t_evals, t.data(), 32 * (size_t)d->t_count, hipMemcpyHostToDevice, c->stream));
+  MZK_HIP(hipStreamSynchronize(c->stream));
+  *out = d;
+  return MZK_OK;
+}
+
+void mzk_domain_destroy(mzk_domain* d) {
+  if (!d) return;
+  if (d->ctx) {
+    (void)hipSetDevice(d->ctx->device);
+    (void)hipStreamSynchronize(d->ctx->stream);
+  }
+  if (d->t_evals) (void)hipFree(d->t_evals);
+  delete d;
+}
+
+int mzk_domain_info(const mzk_domain* d, uint32_t* k, uint32_t* ek) {
+  MZK_REQUIRE(d != nullptr, "null domain");
+  if (k) *k = d->k;
+  if (ek) *ek = d->ext_k;
+  return MZK_OK;
+}
+
+// Transform bodies; callers hold the context lock.
+static int l2c_impl(mzk_domain* d, const void* in, void* out) {
+  return ntt_run(d->ctx, d->inv, (const uint32_t*)in, (uint32_t*)out, 1ull << d->k, 0, 1, d->n_inv);
+}
+static int c2l_impl(mzk_domain* d, const void* in, void* out) {
+  return ntt_run(d->ctx, d->fwd, (const uint32_t*)in, (uint32_t*)out, 1ull << d->k, 0, 0, host::fr_one());
+}
+static int c2e_impl(mzk_domain* d, const void* in, void* out) {
+  return ntt_run(d->ctx, d->ext_fwd, (const uint32_t*)in, (uint32_t*)out, 1ull << d->k, 1, 0, host::fr_one());
+}
+static int e2c_impl(mzk_domain* d, const void* in, void* out) {
+  return ntt_run(d->ctx, d->ext_inv, (const uint32_t*)in, (uint32_t*)out, 1ull << d->ext_k, 0, 2, d->ext_n_inv);
+}
+
+typedef int (*transform_fn)(mzk_domain*, const void*, void*);
+
+static int run_dev(mzk_domain* d, const void* in, void* out, transform_fn fn) {
+  MZK_TRY(domain_check(d));
+  MZK_REQUIRE(in != nullptr && out != nullptr, "null data");
+  std::lock_guard<std::mutex> g(d->ctx->mu);
+  MZK_HIP(hipSetDevice(d->ctx->device));
+  return fn(d, in, out);
+}
+
+// Host-pointer form: stage through context scratch, run, copy back, synchronise,
+// all under the context lock.
+static int run_host(mzk_domain* d, const uint64_t* in, uint64_t* out, size_t in_elems, size_t out_elems,
+                    transform_fn fn) {
+  MZK_TRY(domain_check(d));
+  MZK_REQUIRE(in != nullptr && out != nullptr, "null data");
+  mzk_ctx* c = d->ctx;
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  MZK_TRY(c->scratch[SCR_IN0].reserve(in_elems * 32));
+  MZK_TRY(c->scratch[SCR_IN1].reserve(out_elems * 32));
+  MZK_HIP(hipMemcpyAsync(c->scratch[SCR_IN0].ptr, in, in_elems * 32, hipMemcpyHostToDevice, c->stream));
+  MZK_TRY(fn(d, c->scratch[SCR_IN0].ptr, c->scratch[SCR_IN1].ptr));
+  MZK_HIP(hipMemcpyAsync(out, c->scratch[SCR_IN1].ptr, out_elems * 32, hipMemcpyDeviceToHost, c->stream));
+  MZK_HIP(hipStreamSynchronize(c->stream));
+  return MZK_OK;
+}
+
+int mzk_lagrange_to_coeff_dev(mzk_domain* d, void* a) { return run_dev(d, a, a, l2c_impl); }
+int mzk_coeff_to_lagrange_dev(mzk_domain* d, void* a) { return run_dev(d, a, a, c2l_impl); }
+int mzk_coeff_to_extended_dev(mzk_domain* d, const void* in, void* out) {
+  MZK_REQUIRE(in != out, "coeff_to_extended: input and output must differ");
+  return run_dev(d, in, out, c2e_impl);
+}
+int mzk_extended_to_coeff_dev(mzk_domain* d, void* a) { return run_dev(d, a, a, e2c_impl); }
+
+int mzk_divide_by_vanishing_poly_dev(mzk_domain* d, void* a) {
+  MZK_TRY(domain_check(d));
+  MZK_REQUIRE(a != nullptr, "null data");
+  std::lock_guard<std::mutex> g(d->ctx->mu);
+  MZK_HIP(hipSetDevice(d->ctx->device));
+  const size_t N = 1ull << d->ext_k;
+  mzk_ctx* c = d->ctx;
+  MZK_LAUNCH(c, "k_mul_periodic", k_mul_periodic, dim3(grid_for(N, 256)), dim3(256), 0, (uint32_t*)a, N,
+             d->t_evals, d->t_count - 1);
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
+}
+
+int mzk_lagrange_to_coeff(mzk_domain* d, uint64_t* a) {
+  MZK_TRY(domain_check(d));
+  return run_host(d, a, a, 1ull << d->k, 1ull << d->k, l2c_impl);
+}
+int mzk_extended_to_coeff(mzk_domain* d, uint64_t* a) {
+  MZK_TRY(domain_check(d));
+  return run_host(d, a, a, 1ull << d->ext_k, 1ull << d->ext_k, e2c_impl);
+}
+int mzk_coeff_to_extended(mzk_domain* d, const uint64_t* in, uint64_t* out) {
+  MZK_TRY(domain_check(d));
+  return run_host(d, in, out, 1ull << d->k, 1ull << d->ext_k, c2e_impl);
+}
+
+// best_fft (fft.rs:49): DFT with a caller-supplied omega, in place on host memory.
+int mzk_best_fft(mzk_ctx* c, uint64_t* a, const uint64_t omega[4], uint32_t log_n) {
+  MZK_REQUIRE(c != nullptr && a != nullptr && omega != nullptr, "mzk_best_fft: null argument");
+  if (log_n > (uint32_t)(2 * NTT_MAX_LOG_L)) {
+    set_error("mzk_best_fft: log_n %u above the supported 2^%d", log_n, 2 * NTT_MAX_LOG_L);
+    return MZK_ERR_UNSUPPORTED;
+  }
+  const size_t N = 1ull << log_n;
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  if (log_n == 0) return MZK_OK;   // a DFT of length 1 is the identity
+  NttPlan plan;
+  host::Fr om;
+  memcpy(om.l, omega, 32);
+  MZK_TRY(plan.init(c->stream, log_n, om));
+  MZK_TRY(c->scratch[SCR_IN0].reserve(N * 32));
+  MZK_HIP(hipMemcpyAsync(c->scratch[SCR_IN0].ptr, a, N * 32, hipMemcpyHostToDevice, c->stream));
+  MZK_TRY(ntt_run(c, plan, (const uint32_t*)c->scratch[SCR_IN0].ptr, (uint32_t*)c->scratch[SCR_IN0].ptr, N, 0, 0,
+                  host::fr_one()));
+  MZK_HIP(hipMemcpyAsync(a, c->scratch[SCR_IN0].ptr, N * 32, hipMemcpyDeviceToHost, c->stream));
+  MZK_HIP(hipStreamSynchronize(c->stream));
+  return MZK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,290 @@
+// Resident SRS bases (proofs/src/poly/kzg/params.rs:22-37) and their device-side
+// generation g_i = [tau^i] G1 (ParamsKZG::unsafe_setup, params.rs:154-172).
+//
+// Generation: tau^i by a 2^j power table, a fixed-base comb over 8-bit digits of
+// the canonical scalar (32 mixed additions per point against a 32 x 256 table of
+// d * 2^(8w) G built once on the host), then batch normalisation to affine with
+// one Fermat inversion per 64 points (Montgomery's trick).
+#include <vector>
+
+#include "ctx.hpp"
+#include "g1.hpp"
+#include "host_ec.hpp"
+#include "msm.hpp"
+#include "srs.hpp"
+
+namespace mzk {
+
+struct FrPows {
+  uint32_t v[32][8];
+};
+
+// out[i] = tau^(start + i) as Montgomery, from pows[j] = tau^(2^j)
+__global__ void k_tau_powers(uint32_t* __restrict__ out, size_t n, FrPows p) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  fr_t r = fr_t::one();
+  for (int j = 0; j < 32; j++) {
+    if ((i >> j) & 1) {
+      fr_t q;
+#pragma unroll
+      for (int t = 0; t < 8; t++) q.v[t] = p.v[j][t];
+      r = mul(r, q);
+    }
+  }
+  store<FrP>(out + 8 * i, r);
+}
+
+// XYZZ [s_i] G by the comb table (32 windows of 8 bits).
+__global__ void __launch_bounds__(64) k_fixed_base(const uint32_t* __restrict__ scalars, size_t n,
+                                                   const uint32_t* __restrict__ comb, uint32_t* __restrict__ out) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  fr_t s = from_mont(load<FrP>(scalars + 8 * i));
+  g1_xyzz_t acc = g1_xyzz_t::inf();
+  for (int w = 0; w < 32; w++) {
+    uint32_t d = (s.v[w >> 2] >> (8 * (w & 3))) & 0xffu;
+    if (d) acc = g1_add_affine(acc, load_affine(comb + 24 * (size_t)(w * 256 + d)));
+  }
+  store_xyzz(out + 48 * i, acc);
+}
+
+// a^(p-2) on the device (one dependent chain; used once per 64 points).
+__device__ fp_t fp_inv_dev(const fp_t& a) {
+  // exponent p - 2, scanned from the top bit, 4-bit fixed window
+  constexpr uint32_t E[12] = {0xffffaaa9u, 0xb9feffffu, 0xb153ffffu, 0x1eabfffeu, 0xf6b0f624u, 0x6730d2a0u,
+                              0xf38512bfu, 0x64774b84u, 0x434bacd7u, 0x4b1ba7b6u, 0x397fe69au, 0x1a0111eau};
+  fp_t tab[16];
+  tab[0] = fp_t::one();
+  tab[1] = a;
+  for (int i = 2; i < 16; i++) tab[i] = mul(tab[i - 1], a);
+  fp_t r = fp_t::one();
+  for (int nib = 95; nib >= 0; nib--) {
+    r = sqr(sqr(sqr(sqr(r))));
+    const uint32_t d = (E[nib >> 3] >> (4 * (nib & 7))) & 15u;
+    fp_t m = tab[0];
+    for (int t = 1; t < 16; t++)
+      if ((uint32_t)t == d) m = tab[t];
+    if (d) r = mul(r, m);
+  }
+  return r;
+}
+
+static constexpr int NORM_CHUNK = 64;
+
+// XYZZ -> affine for n points, NORM_CHUNK per thread: with u = ZZ * ZZZ,
+// 1/ZZ = ZZZ / u and 1/ZZZ = ZZ / u; one inversion per chunk.
+__global__ void __launch_bounds__(64) k_batch_normalize(const uint32_t* __restrict__ in, size_t n,
+                                                        uint32_t* __restrict__ out, uint32_t* __restrict__ prefix) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t i0 = t * NORM_CHUNK;
+  if (i0 >= n) return;
+  size_t i1 = i0 + NORM_CHUNK < n ? i0 + NORM_CHUNK : n;
+  fp_t acc = fp_t::one();
+  for (size_t i = i0; i < i1; i++) {
+    store<FpP>(prefix + 12 * i, acc);
+    const uint32_t* p = in + 48 * i;
+    fp_t zz = load<FpP>(p + 24);
+    if (!zz.is_zero()) acc = mul(acc, mul(zz, load<FpP>(p + 36)));
+  }
+  fp_t inv = fp_inv_dev(acc);
+  for (size_t i = i1; i-- > i0;) {
+    const uint32_t* p = in + 48 * i;
+    fp_t zz = load<FpP>(p + 24);
+    uint32_t* o = out + 24 * i;
+    if (zz.is_zero()) {
+      store<FpP>(o, fp_t::zero());
+      store<FpP>(o + 12, fp_t::zero());
+      continue;
+    }
+    fp_t zzz = load<FpP>(p + 36);
+    fp_t uinv = mul(inv, load<FpP>(prefix + 12 * i));
+    inv = mul(inv, mul(zz, zzz));
+    store<FpP>(o, mul(load<FpP>(p), mul(zzz, uinv)));
+    store<FpP>(o + 12, mul(load<FpP>(p + 12), mul(zz, uinv)));
+  }
+}
+
+// ------------------------------------------------------------------- host tables
+static const uint64_t kG1x[6] = {0x5cb38790fd530c16ull, 0x7817fc679976fff5ull, 0x154f95c7143ba1c1ull,
+                                 0xf0ae6acdf3d0e747ull, 0xedce6ecc21dbf440ull, 0x120177419e0bfb75ull};
+static const uint64_t kG1y[6] = {0xbaac93d50ce72271ull, 0x8c22631a7918fd8eull, 0xdd595f13570725ceull,
+                                 0x51ac582950405194ull, 0x0e1c8c3fad0059c0ull, 0x0bbc3efc5008a26aull};
+
+// comb[w][d] = d * 2^(8w) G (affine, blst layout), d in [0, 256), identity at d = 0.
+static std::vector<uint64_t> comb_table() {
+  using namespace host;
+  std::vector<Xyzz> pts(32 * 256);
+  Affine g;
+  memcpy(g.x.l, kG1x, 48);
+  memcpy(g.y.l, kG1y, 48);
+  Xyzz base = xyzz_from_affine(g);
+  for (int w = 0; w < 32; w++) {
+    Xyzz acc = Xyzz::inf();
+    pts[w * 256] = acc;
+    for (int d = 1; d < 256; d++) {
+      acc = xyzz_add(acc, base);
+      pts[w * 256 + d] = acc;
+    }
+    for (int k = 0; k < 8; k++) base = xyzz_dbl(base);
+  }
+  // batch-normalise: u = ZZ * ZZZ
+  std::vector<Fp> pre(pts.size());
+  Fp acc = fp_one();
+  for (size_t i = 0; i < pts.size(); i++) {
+    pre[i] = acc;
+    if (!pts[i].is_inf()) acc = fp_mul(acc, fp_mul(pts[i].ZZ, pts[i].ZZZ));
+  }
+  Fp inv = fp_inv(acc);
+  std::vector<uint64_t> out(pts.size() * 12, 0);
+  for (size_t i = pts.size(); i-- > 0;) {
+    if (pts[i].is_inf()) continue;
+    Fp uinv = fp_mul(inv, pre[i]);
+    inv = fp_mul(inv, fp_mul(pts[i].ZZ, pts[i].ZZZ));
+    Fp x = fp_mul(pts[i].X, fp_mul(pts[i].ZZZ, uinv));
+    Fp y = fp_mul(pts[i].Y, fp_mul(pts[i].ZZ, uinv));
+    memcpy(&out[12 * i], x.l, 48);
+    memcpy(&out[12 * i + 6], y.l, 48);
+  }
+  return out;
+}
+
+int batch_normalize(mzk_ctx* c, const uint32_t* d_xyzz, size_t n, uint32_t* d_affine) {
+  MZK_TRY(c->scratch[SCR_MSM_D].reserve(n * 48 + 64));
+  uint32_t* prefix = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
+  size_t threads = (n + NORM_CHUNK - 1) / NORM_CHUNK;
+  MZK_LAUNCH(c, "k_batch_normalize", k_batch_normalize, dim3(grid_for(threads, 64)), dim3(64), 0, d_xyzz, n, d_affine,
+                     prefix);
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
+}
+
+}  // namespace mzk
+
+using namespace mzk;
+
+static int srs_alloc(mzk_ctx* c, size_t n, mzk_srs** out) {
+  mzk_srs* s = new mzk_srs();
+  s->ctx = c;
+  s->n = n;
+  hipError_t e = hipMalloc(&s->bases, n * 96);
+  if (e != hipSuccess) {
+    delete s;
+    set_error("srs: hipMalloc(%zu) failed: %s", n * 96, hipGetErrorString(e));
+    return MZK_ERR_NOMEM;
+  }
+  *out = s;
+  return MZK_OK;
+}
+
+extern "C" {
+
+int mzk_srs_create(mzk_ctx* c, const uint64_t* bases, size_t n, mzk_srs** out) {
+  MZK_REQUIRE(c != nullptr && out != nullptr && (bases != nullptr || n == 0), "mzk_srs_create: null argument");
+  MZK_REQUIRE(n >= 1 && n < (1ull << 31), "mzk_srs_create: n out of range");
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  mzk_srs* s = nullptr;
+  MZK_TRY(srs_alloc(c, n, &s));
+  hipError_t e = hipMemcpyAsync(s->bases, bases, n * 96, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    mzk_srs_destroy(s);
+    set_error("srs upload: %s", hipGetErrorString(e));
+    return MZK_ERR_HIP;
+  }
+  *out = s;
+  return MZK_OK;
+}
+
+int mzk_srs_generate(mzk_ctx* c, const uint64_t tau[4], size_t n, mzk_srs** out) {
+  MZK_REQUIRE(c != nullptr && out != nullptr && tau != nullptr, "mzk_srs_generate: null argument");
+  MZK_REQUIRE(n >= 1 && n < (1ull << 31), "mzk_srs_generate: n out of range");
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  mzk_srs* s = nullptr;
+  MZK_TRY(srs_alloc(c, n, &s));
+  std::vector<uint64_t> comb = comb_table();
+  int rc = MZK_OK;
+  do {
+    if ((rc = c->scratch[SCR_IN0].reserve(comb.size() * 8 + n * 32)) != MZK_OK) break;
+    if ((rc = c->scratch[SCR_MSM_C].reserve(n * 192)) != MZK_OK) break;
+    uint32_t* d_comb = (uint32_t*)c->scratch[SCR_IN0].ptr;
+    uint32_t* d_pows = d_comb + comb.size() * 2;
+    uint32_t* d_xyzz = (uint32_t*)c->scratch[SCR_MSM_C].ptr;
+    if (hipMemcpyAsync(d_comb, comb.data(), comb.size() * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+      rc = MZK_ERR_HIP;
+      set_error("srs: comb upload failed");
+      break;
+    }
+    FrPows p;
+    host::Fr t;
+    memcpy(t.l, tau, 32);
+    for (int j = 0; j < 32; j++) {
+      memcpy(p.v[j], t.l, 32);
+      t = host::fr_mul(t, t);
+    }
+    MZK_LAUNCH(c, "k_tau_powers", k_tau_powers, dim3(grid_for(n, 256)), dim3(256), 0, d_pows, n, p);
+    MZK_LAUNCH(c, "k_fixed_base", k_fixed_base, dim3(grid_for(n, 64)), dim3(64), 0, d_pows, n, d_comb, d_xyzz);
+    if (hipGetLastError() != hipSuccess) {
+      rc = MZK_ERR_HIP;
+      set_error("srs: kernel launch failed");
+      break;
+    }
+    if ((rc = batch_normalize(c, d_xyzz, n, (uint32_t*)s->bases)) != MZK_OK) break;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+      rc = MZK_ERR_HIP;
+      set_error("srs: generation failed");
+      break;
+    }
+  } while (0);
+  if (rc != MZK_OK) {
+    mzk_srs_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return MZK_OK;
+}
+
+size_t mzk_srs_len(const mzk_srs* s) { return s ? s->n : 0; }
+
+const void* mzk_srs_bases_dev(const mzk_srs* s) { return s ? s->bases : nullptr; }
+
+int mzk_srs_download(mzk_srs* s, size_t offset, size_t count, uint64_t* out) {
+  MZK_REQUIRE(s != nullptr && (out != nullptr || count == 0), "mzk_srs_download: null argument");
+  MZK_REQUIRE(offset <= s->n && count <= s->n - offset, "mzk_srs_download: range out of bounds");
+  std::lock_guard<std::mutex> g(s->ctx->mu);
+  MZK_HIP(hipSetDevice(s->ctx->device));
+  MZK_HIP(hipMemcpyAsync(out, (const char*)s->bases + offset * 96, count * 96, hipMemcpyDeviceToHost,
+                         s->ctx->stream));
+  MZK_HIP(hipStreamSynchronize(s->ctx->stream));
+  return MZK_OK;
+}
+
+void mzk_srs_destroy(mzk_srs* s) {
+  if (!s) return;
+  if (s->ctx) {
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+  }
+  if (s->bases) (void)hipFree(s->bases);
+  if (s->shifted) (void)hipFree(s->shifted);
+  delete s;
+}
+
+int mzk_msm_srs_dev(mzk_ctx* c, const mzk_srs* s, size_t offset, const void* d_coeffs, size_t n, uint64_t out[18]) {
+  MZK_REQUIRE(c != nullptr && s != nullptr && out != nullptr, "mzk_msm_srs_dev: null argument");
+  MZK_REQUIRE(offset <= s->n && n <= s->n - offset, "mzk_msm_srs_dev: %zu bases from %zu exceed the SRS (%zu)", n,
+              offset, s->n);
+  if (n == 0) {
+    memset(out, 0, 18 * 8);
+    return MZK_OK;
+  }
+  MZK_REQUIRE(d_coeffs != nullptr, "mzk_msm_srs_dev: null coefficients");
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  MsmPlan p = msm_plan(n, 1, c->num_cus, 0);
+  return msm_run(c, p, (const uint32_t*)d_coeffs, n, (const uint32_t*)s->bases + 24 * offset, 0, 0, out);
+}
+
+}  // extern "C"

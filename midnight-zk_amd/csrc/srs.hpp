@@ -1,0 +1,16 @@
+// The resident-SRS object behind the opaque mzk_srs handle.
+#pragma once
+#include "ctx.hpp"
+
+struct mzk_srs {
+  mzk_ctx* ctx = nullptr;
+  size_t n = 0;
+  void* bases = nullptr;     // n x 24 u32 affine (blst_p1_affine layout)
+  void* shifted = nullptr;   // optional window-shifted copies 2^(c w) g_i
+  int shifted_c = 0;
+};
+
+namespace mzk {
+// XYZZ (n x 48 u32) -> affine (n x 24 u32) on the context stream.
+int batch_normalize(mzk_ctx* c, const uint32_t* d_xyzz, size_t n, uint32_t* d_affine);
+}  // namespace mzk

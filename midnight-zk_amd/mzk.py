@@ -74,6 +74,9 @@ SIGNATURES = {
     "mzk_diag_field_op": ([_P, _I, _I, _P, _P, _P, _S], _I),
     "mzk_diag_g1_op": ([_P, _I, _P, _P, _P, _S], _I),
     "mzk_diag_bench": ([_P, _I, _I, _I, _I, _D, _D], _I),
+    "mzk_prof_enable": ([_P, _I], _I),
+    "mzk_prof_read": ([_P, ctypes.c_char_p, _S], _I),
+    "mzk_msm_last_plan": ([_P, _P], _I),
 }
 
 
@@ -84,6 +87,13 @@ def load_library(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise MzkError(f"{path} is not built; run midnight-zk_amd/build.py")
+    # torch ships its own libamdhip64.so.7.  Loading torch first makes this library
+    # bind to that same HIP runtime; the other order puts two runtimes in the
+    # process and the second one finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(path)
     for name, (args, res) in SIGNATURES.items():
         fn = getattr(L, name)  # AttributeError if the export is missing
@@ -139,6 +149,27 @@ class Context:
 
     def synchronize(self):
         _check(load_library().mzk_ctx_synchronize(self.h))
+
+    def profile(self, on: bool = True):
+        """Per-kernel HIP-event timing on this context's stream (off by default)."""
+        _check(load_library().mzk_prof_enable(self.h, 1 if on else 0))
+
+    def profile_read(self) -> dict:
+        """{kernel name: (total ms, launches)} since the last read; clears the totals."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(load_library().mzk_prof_read(self.h, buf, len(buf)))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, ms, cnt = line.split()
+            out[name] = (float(ms), int(cnt))
+        return out
+
+    def msm_last_plan(self) -> dict:
+        """Plan of the last MSM on this context (window bits, digits, ...)."""
+        out = np.zeros(6, dtype=np.uint64)
+        _check(load_library().mzk_msm_last_plan(self.h, _np_ptr(out)))
+        keys = ("c", "windows", "bucket_sets", "refs_per_thread", "digits", "n")
+        return {k: int(v) for k, v in zip(keys, out)}
 
     # ------------------------------------------------------------------ MSM
     def msm_best(self, coeffs: np.ndarray, bases: np.ndarray) -> np.ndarray:
