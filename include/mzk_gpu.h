@@ -132,7 +132,8 @@ int mzk_msm_last_plan(mzk_ctx *ctx, uint64_t out[6]);
 /* ------------------------------------------------------------------ diagnostics
  * Element-wise field / curve operations for parity tests (device pointers, n elements):
  *   field 0 = fr (8 u32 / element), 1 = fp (12 u32 / element);
- *   op 0 = mul, 1 = add, 2 = sub, 3 = square, 4 = from_mont. */
+ *   op 0 = mul (the product-scanning form every kernel uses), 1 = add, 2 = sub,
+ *   3 = square, 4 = from_mont, 5 = mul by the operand-scanning (CIOS) form. */
 int mzk_diag_field_op(mzk_ctx *ctx, int field, int op, const void *d_a, const void *d_b, void *d_out,
                       size_t n);
 /* G1: op 0 = affine + affine (through XYZZ), 1 = 2 * affine, 2 = XYZZ + XYZZ of the
@@ -140,11 +141,13 @@ int mzk_diag_field_op(mzk_ctx *ctx, int field, int op, const void *d_a, const vo
 int mzk_diag_g1_op(mzk_ctx *ctx, int op, const void *d_a, const void *d_b, void *d_out, size_t n);
 /* Micro-benchmarks (time in ms from HIP events):
  *   which 0: independent 32x32+64 multiply-adds, result = mads issued;
- *   which 1: fp Montgomery products, many independent chains (throughput);
- *   which 2: fp Montgomery products, one wave, one dependent chain (latency);
- *   which 3: fr Montgomery products, many chains (throughput);
+ *   which 1: fp CIOS products, many independent chains (throughput);
+ *   which 2: fp CIOS products, one wave, one dependent chain (latency);
+ *   which 3: fr CIOS products, many chains (throughput);
  *   which 4: dependent 32x32+64 multiply-add chains with no other instruction
- *            (v_mad_u64_u32 issue rate: the INT32 multiply roof of the chip). */
+ *            (v_mad_u64_u32 issue rate: the INT32 multiply roof of the chip);
+ *   which 5: fp product-scanning (default) products, many chains (throughput);
+ *   which 6: fr product-scanning (default) products, many chains (throughput). */
 int mzk_diag_bench(mzk_ctx *ctx, int which, int blocks, int threads, int iters, double *ms,
                    double *ops);
 

@@ -2,6 +2,8 @@
 #include <stdarg.h>
 #include <string.h>
 
+#include <chrono>
+
 #include "ctx.hpp"
 #include "field.hpp"
 
@@ -38,6 +40,10 @@ void Scratch::release() {
   if (ptr) (void)hipFree(ptr);
   ptr = nullptr;
   cap = 0;
+}
+
+double host_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
 void KernelProfiler::begin(hipStream_t st, const char* name) {

@@ -42,7 +42,16 @@ struct KernelProfiler {
   void begin(hipStream_t st, const char* name);
   void end(hipStream_t st);
   int collect();   // waits for pending events and folds them into totals
+  void host(const char* name, double ms) {   // host-side phase (waits, serial tails)
+    if (!on) return;
+    auto& t = totals[name];
+    t.first += ms;
+    t.second += 1;
+  }
 };
+
+// Wall-clock milliseconds since an arbitrary origin (host-phase timing).
+double host_ms();
 
 }  // namespace mzk
 

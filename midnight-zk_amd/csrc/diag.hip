@@ -19,7 +19,8 @@ __global__ void k_field_op(int op, const uint32_t* __restrict__ a, const uint32_
     case 1: r = add(x, y); break;
     case 2: r = sub(x, y); break;
     case 3: r = sqr(x); break;
-    default: r = from_mont(x); break;
+    case 4: r = from_mont(x); break;
+    default: r = mul_cios(x, y); break;
   }
   store<P>(out + N * i, r);
 }
@@ -83,13 +84,13 @@ __global__ void k_bench_mad_pure(int iters, const uint32_t* __restrict__ seed, u
   if (s == 0x123456789abcdefull) sink[t] = s;
 }
 
-template <class P>
+template <class P, bool FIPS>
 __global__ void k_bench_mul(int iters, const uint32_t* __restrict__ src, uint32_t* __restrict__ sink) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   Field<P> x = load<P>(src);
   Field<P> y = load<P>(src + 16);
   x.v[0] ^= t;
-  for (int it = 0; it < iters; it++) x = mul(x, y);
+  for (int it = 0; it < iters; it++) x = FIPS ? mul_fips(x, y) : mul_cios(x, y);
   if (x.v[0] == 0x12345678u && x.v[1] == t) store<P>(sink, x);
 }
 
@@ -102,7 +103,7 @@ extern "C" {
 int mzk_diag_field_op(mzk_ctx* c, int field, int op, const void* a, const void* b, void* out, size_t n) {
   MZK_REQUIRE(c != nullptr, "null context");
   MZK_REQUIRE(field == 0 || field == 1, "field must be 0 (fr) or 1 (fp)");
-  MZK_REQUIRE(op >= 0 && op <= 4, "op out of range");
+  MZK_REQUIRE(op >= 0 && op <= 5, "op out of range");
   if (n == 0) return MZK_OK;
   MZK_REQUIRE(a && b && out, "null pointer");
   std::lock_guard<std::mutex> g(c->mu);
@@ -134,7 +135,7 @@ int mzk_diag_g1_op(mzk_ctx* c, int op, const void* a, const void* b, void* out, 
 
 int mzk_diag_bench(mzk_ctx* c, int which, int blocks, int threads, int iters, double* ms, double* ops) {
   MZK_REQUIRE(c != nullptr && ms && ops, "null argument");
-  MZK_REQUIRE(which >= 0 && which <= 4, "which out of range");
+  MZK_REQUIRE(which >= 0 && which <= 6, "which out of range");
   MZK_REQUIRE(blocks > 0 && threads > 0 && threads <= 1024 && iters > 0, "bad launch shape");
   std::lock_guard<std::mutex> g(c->mu);
   MZK_HIP(hipSetDevice(c->device));
@@ -158,7 +159,15 @@ int mzk_diag_bench(mzk_ctx* c, int which, int blocks, int threads, int iters, do
       break;
     case 1:
     case 2:
-      hipLaunchKernelGGL(k_bench_mul<FpP>, dim3(blocks), dim3(threads), 0, c->stream, iters, buf, sink);
+      hipLaunchKernelGGL((k_bench_mul<FpP, false>), dim3(blocks), dim3(threads), 0, c->stream, iters, buf, sink);
+      per_thread = iters;
+      break;
+    case 5:
+      hipLaunchKernelGGL((k_bench_mul<FpP, true>), dim3(blocks), dim3(threads), 0, c->stream, iters, buf, sink);
+      per_thread = iters;
+      break;
+    case 6:
+      hipLaunchKernelGGL((k_bench_mul<FrP, true>), dim3(blocks), dim3(threads), 0, c->stream, iters, buf, sink);
       per_thread = iters;
       break;
     case 4:
@@ -166,7 +175,7 @@ int mzk_diag_bench(mzk_ctx* c, int which, int blocks, int threads, int iters, do
       per_thread = 8.0 * iters;
       break;
     default:
-      hipLaunchKernelGGL(k_bench_mul<FrP>, dim3(blocks), dim3(threads), 0, c->stream, iters, buf, sink);
+      hipLaunchKernelGGL((k_bench_mul<FrP, false>), dim3(blocks), dim3(threads), 0, c->stream, iters, buf, sink);
       per_thread = iters;
       break;
   }

@@ -159,9 +159,59 @@ int batch_normalize(mzk_ctx* c, const uint32_t* d_xyzz, size_t n, uint32_t* d_af
   return MZK_OK;
 }
 
+// out[i] = 2^c * in[i] (affine in, XYZZ out): one doubling from affine, c - 1 more.
+__global__ void __launch_bounds__(64) k_shift_window(const uint32_t* __restrict__ in, size_t n, int c,
+                                                     uint32_t* __restrict__ out) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  g1_aff_t a = load_affine(in + 24 * i);
+  g1_xyzz_t p = a.is_inf() ? g1_xyzz_t::inf() : g1_dbl_affine(a);
+  for (int k = 1; k < c; k++) p = g1_dbl(p);
+  store_xyzz(out + 48 * i, p);
+}
+
+// Window-shifted copies tab[w n + i] = 2^(c w) g_i for w < W, so that all windows of a
+// commitment can share one bucket set.  Built window by window (n XYZZ temporaries).
+int srs_build_shifted(mzk_ctx* c, mzk_srs* s, int cbits) {
+  const int W = 255 / cbits + 1;
+  const size_t n = s->n;
+  const size_t bytes = (size_t)W * n * 96;
+  hipError_t e = hipMalloc(&s->shifted, bytes);
+  if (e != hipSuccess) {
+    s->shifted = nullptr;
+    set_error("srs: hipMalloc(%zu) for shifted bases failed: %s", bytes, hipGetErrorString(e));
+    return MZK_ERR_NOMEM;
+  }
+  MZK_TRY(c->scratch[SCR_MSM_C].reserve(n * 192));
+  uint32_t* tmp = (uint32_t*)c->scratch[SCR_MSM_C].ptr;
+  uint32_t* tab = (uint32_t*)s->shifted;
+  MZK_HIP(hipMemcpyAsync(tab, s->bases, n * 96, hipMemcpyDeviceToDevice, c->stream));
+  for (int w = 1; w < W; w++) {
+    MZK_LAUNCH(c, "k_shift_window", k_shift_window, dim3(grid_for(n, 64)), dim3(64), 0, tab + 24 * (size_t)(w - 1) * n,
+               n, cbits, tmp);
+    MZK_TRY(batch_normalize(c, tmp, n, tab + 24 * (size_t)w * n));
+  }
+  MZK_HIP(hipGetLastError());
+  MZK_HIP(hipStreamSynchronize(c->stream));
+  s->shifted_c = cbits;
+  return MZK_OK;
+}
+
 }  // namespace mzk
 
 using namespace mzk;
+
+// Commitments of at least this many terms use the window-shifted bases (one shared
+// bucket set); smaller ones use per-window buckets on the plain bases.
+static constexpr size_t SHARED_MIN_TERMS = 1u << 14;
+static constexpr int SHARED_C = 16;
+// Shifted tables cost (255/16 + 1) x 96 B per base; built for SRS of at least this size.
+static constexpr size_t SHIFTED_MIN_SRS = 1u << 14;
+
+static int srs_finish(mzk_ctx* c, mzk_srs* s) {
+  if (s->n < SHIFTED_MIN_SRS || s->n * 16 >= (1ull << 31)) return MZK_OK;
+  return srs_build_shifted(c, s, SHARED_C);
+}
 
 static int srs_alloc(mzk_ctx* c, size_t n, mzk_srs** out) {
   mzk_srs* s = new mzk_srs();
@@ -192,6 +242,11 @@ int mzk_srs_create(mzk_ctx* c, const uint64_t* bases, size_t n, mzk_srs** out) {
     mzk_srs_destroy(s);
     set_error("srs upload: %s", hipGetErrorString(e));
     return MZK_ERR_HIP;
+  }
+  int rc = srs_finish(c, s);
+  if (rc != MZK_OK) {
+    mzk_srs_destroy(s);
+    return rc;
   }
   *out = s;
   return MZK_OK;
@@ -237,6 +292,7 @@ int mzk_srs_generate(mzk_ctx* c, const uint64_t tau[4], size_t n, mzk_srs** out)
       set_error("srs: generation failed");
       break;
     }
+    rc = srs_finish(c, s);
   } while (0);
   if (rc != MZK_OK) {
     mzk_srs_destroy(s);
@@ -283,6 +339,11 @@ int mzk_msm_srs_dev(mzk_ctx* c, const mzk_srs* s, size_t offset, const void* d_c
   MZK_REQUIRE(d_coeffs != nullptr, "mzk_msm_srs_dev: null coefficients");
   std::lock_guard<std::mutex> g(c->mu);
   MZK_HIP(hipSetDevice(c->device));
+  if (s->shifted && n >= SHARED_MIN_TERMS) {
+    MsmPlan p = msm_plan(n, 0, c->num_cus, s->shifted_c);
+    return msm_run(c, p, (const uint32_t*)d_coeffs, n, (const uint32_t*)s->shifted, (uint32_t)s->n, (uint32_t)offset,
+                   out);
+  }
   MsmPlan p = msm_plan(n, 1, c->num_cus, 0);
   return msm_run(c, p, (const uint32_t*)d_coeffs, n, (const uint32_t*)s->bases + 24 * offset, 0, 0, out);
 }

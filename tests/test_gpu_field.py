@@ -29,7 +29,8 @@ def test_field_ops_bit_exact(ctx, field):
     A = to_dev([O.to_limbs(tm(x), nl) for x in xs])
     B = to_dev([O.to_limbs(tm(y), nl) for y in ys])
     out = empty_dev(n, nl)
-    ops = {0: lambda x, y: x * y, 1: lambda x, y: x + y, 2: lambda x, y: x - y, 3: lambda x, y: x * x}
+    ops = {0: lambda x, y: x * y, 1: lambda x, y: x + y, 2: lambda x, y: x - y, 3: lambda x, y: x * x,
+           5: lambda x, y: x * y}
     for op, fn in ops.items():
         mzk.diag_field_op(ctx, field, op, A, B, out, n)
         got = from_dev(out)

@@ -110,6 +110,21 @@ def test_resident_srs_commit_with_offset(ctx, srs16):
     srs.close()
 
 
+def test_resident_srs_shared_bucket_path_with_offset(ctx, srs16):
+    """Commits of >= 2^14 terms use the window-shifted bases and one shared bucket
+    set; check it (with an offset into the SRS) against msm_best."""
+    tau, bases = srs16
+    srs = ctx.srs_upload(bases[:40000])
+    n = 20000
+    sc = C.fr_from_seed(12, 1, n)
+    sc[5] = 0
+    d = to_dev(sc)
+    for off in (0, 777, 20000):
+        assert aff(srs.commit(d, n, offset=off)) == aff(C.msm_best(sc, bases[off:off + n]))
+    assert ctx.msm_last_plan()["bucket_sets"] == 1
+    srs.close()
+
+
 def test_msm_2_20_closed_form(ctx):
     """The metric configuration: n = 2^20, bases [tau^i]G, checked against
     [poly_s(tau)] G (one Horner pass and one scalar multiplication)."""
