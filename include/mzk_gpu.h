@@ -64,6 +64,10 @@ int mzk_msm_specific(mzk_ctx *ctx, const uint64_t *coeffs, const uint64_t *bases
 int mzk_msm_dev(mzk_ctx *ctx, const void *d_coeffs, const void *d_bases, size_t n,
                 uint64_t out[18]);
 
+/* Jacobian -> affine on the host (G1Projective::to_affine; g1.rs:725-739 normalises in
+ * batches with the same result): x = X/Z^2, y = Y/Z^3; the identity maps to all zeros. */
+int mzk_g1_jacobian_to_affine(const uint64_t jac[18], uint64_t aff[12]);
+
 /* ------------------------------------------------------------------ resident SRS
  * Replaces the bases side of proofs/src/poly/kzg/params.rs:22-37 (`ParamsKZG::g`) and
  * `bases::<B>()` (params.rs:83-93): the bases stay in HBM across commits. */
@@ -82,6 +86,18 @@ void mzk_srs_destroy(mzk_srs *srs);
  * sum_i coeffs[i] * g[offset + i], coeffs device-resident. */
 int mzk_msm_srs_dev(mzk_ctx *ctx, const mzk_srs *srs, size_t offset, const void *d_coeffs,
                     size_t n, uint64_t out[18]);
+
+/* Many commitments at once (the independent commit_many / par_iter batches of
+ * prover.rs:619-623, 758-762 and permutation/prover.rs:241-245): one sort, one bucket
+ * accumulation and one reduction over all jobs, so uneven sizes share the device.
+ * outs receives 18 u64 per job; each result equals mzk_msm_srs_dev on that job. */
+typedef struct {
+  const void *d_coeffs; /* n device Montgomery scalars */
+  size_t n;
+  size_t offset;        /* first SRS base used */
+} mzk_msm_job;
+int mzk_msm_srs_batch_dev(mzk_ctx *ctx, const mzk_srs *srs, const mzk_msm_job *jobs, size_t njobs,
+                          uint64_t *outs);
 
 /* ------------------------------------------------------------------ NTT
  * Replaces curves/src/fft.rs:49 `best_fft(a: &mut [G], omega, log_n)` for G = Fq:
@@ -106,6 +122,28 @@ int mzk_coeff_to_extended_dev(mzk_domain *d, const void *d_in, void *d_out);
 int mzk_extended_to_coeff_dev(mzk_domain *d, void *d_a);
 /* domain.rs:361-380 divide_by_vanishing_poly: in place on 2^extended_k elements. */
 int mzk_divide_by_vanishing_poly_dev(mzk_domain *d, void *d_a);
+/* Stand-alone transforms of size 2^log_n (log_n <= 22) on device memory, with the
+ * domain semantics of domain.rs:
+ *   flags 0                          : DFT with omega_n of n_real coefficients zero-padded
+ *                                      to 2^log_n (coeff_to_lagrange when n_real = 2^log_n);
+ *   MZK_NTT_COSET                    : the same on the zeta coset (coeff_to_extended, :285);
+ *   MZK_NTT_INVERSE                  : inverse DFT times 1/2^log_n (lagrange_to_coeff, :249);
+ *   MZK_NTT_INVERSE | MZK_NTT_COSET  : extended_to_coeff (:312).
+ * d_in holds n_real elements (2^log_n for inverses); d_out holds 2^log_n; in place is
+ * allowed.  Twiddle tables are built once per (log_n, direction) and kept on the context. */
+enum { MZK_NTT_INVERSE = 1, MZK_NTT_COSET = 2 };
+typedef struct {
+  const void *d_in;
+  void *d_out;
+  uint64_t n_real;
+  uint32_t log_n;
+  uint32_t flags;
+} mzk_ntt_job;
+int mzk_ntt_dev(mzk_ctx *ctx, const mzk_ntt_job *job);
+/* Many transforms of mixed sizes in two launches (all first passes, then all second
+ * passes): the batched form of the prover's per-column par_iter NTTs
+ * (prover.rs:276-280, 652-659). */
+int mzk_ntt_batch_dev(mzk_ctx *ctx, const mzk_ntt_job *jobs, size_t njobs);
 /* Host-pointer forms of the four transforms. */
 int mzk_lagrange_to_coeff(mzk_domain *d, uint64_t *a);
 int mzk_coeff_to_extended(mzk_domain *d, const uint64_t *in, uint64_t *out);

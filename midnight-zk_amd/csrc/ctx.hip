@@ -153,6 +153,7 @@ void mzk_ctx_destroy(mzk_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  free_ntt_plans(c);
   for (auto& s : c->scratch) s.release();
   if (c->host_staging) (void)hipHostFree(c->host_staging);
   (void)hipStreamDestroy(c->stream);

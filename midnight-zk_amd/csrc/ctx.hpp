@@ -30,6 +30,8 @@ enum ScratchSlot {
 
 uint64_t splitmix64_host(uint64_t x);
 
+struct NttPlan;
+
 // Per-kernel timing with HIP events on the context stream (off unless enabled).
 struct KernelProfiler {
   bool on = false;
@@ -54,6 +56,12 @@ struct KernelProfiler {
 double host_ms();
 
 }  // namespace mzk
+struct mzk_ctx;
+namespace mzk {
+// Frees the context's cached NTT plans (ntt.hip).
+void free_ntt_plans(mzk_ctx* c);
+
+}  // namespace mzk
 
 // Launches a kernel on the context stream, bracketed by profiler events when on.
 #define MZK_LAUNCH(ctx, name, kernel, grid, block, lds, ...)                        \
@@ -73,4 +81,5 @@ struct mzk_ctx {
   void* host_staging = nullptr;
   mzk::KernelProfiler prof;
   uint64_t last_msm[6] = {0, 0, 0, 0, 0, 0};   // c, W, sets, K, M, n of the last MSM
+  std::map<int, mzk::NttPlan*> ntt_plans;        // (log_n, inverse) -> resident twiddle tables
 };

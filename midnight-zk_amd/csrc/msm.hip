@@ -28,6 +28,7 @@
 //  * one shared set (resident SRS with window-shifted copies 2^(cw) g_i built once):
 //    key = |d| - 1, reference = w * N + offset + i, a single set of weight 1.
 #include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "ctx.hpp"
@@ -63,34 +64,53 @@ __device__ __forceinline__ int window_digit(const uint32_t s[8], int w, int c, u
   return (int)d;
 }
 
+// One MSM of a batch: its scalars, its first reference, where its canonical
+// scalars go, and its first bucket set.
+struct JobDesc {
+  const uint32_t* coeffs;
+  uint32_t n;
+  uint32_t ref_offset;
+  uint32_t canon_off;
+  uint32_t set_base;
+};
+
+// A tile of one job's scalars, handled by one sort workgroup.
+struct TileDesc {
+  uint32_t job, lo, hi;
+};
+
 struct SortArgs {
-  uint32_t n, tile, G;   // scalars, scalars per tile, tiles
+  const JobDesc* jobs;
+  const TileDesc* tiles;
+  uint32_t G;            // tiles
   int c, W, per_window;
   uint32_t l2;           // low key bits sorted in k_sort_bins
   uint32_t D1;           // bins of the top key bits
-  uint32_t ref_stride, ref_offset;
+  uint32_t ref_stride;   // table stride between windows (shared-set layout)
 };
 
+// Bucket key of digit d of window w of a job.
+__device__ __forceinline__ uint32_t bucket_key(const SortArgs& a, const JobDesc& j, int w, int d) {
+  const uint32_t B = 1u << (a.c - 1);
+  const uint32_t set = j.set_base + (a.per_window ? (uint32_t)w : 0u);
+  return set * B + (uint32_t)(d > 0 ? d : -d) - 1u;
+}
+
 // 1. digits + per-tile histogram of key >> l2 (LDS atomics)
-__global__ void __launch_bounds__(SORT_BLOCK) k_sort_hist(const uint32_t* __restrict__ scalars, SortArgs a,
-                                                          uint32_t* __restrict__ canon, uint32_t* __restrict__ hist) {
+__global__ void __launch_bounds__(SORT_BLOCK) k_sort_hist(SortArgs a, uint32_t* __restrict__ canon,
+                                                          uint32_t* __restrict__ hist) {
   extern __shared__ uint32_t cnt[];   // D1 counters
   for (uint32_t i = threadIdx.x; i < a.D1; i += SORT_BLOCK) cnt[i] = 0;
   __syncthreads();
-  const uint32_t B = 1u << (a.c - 1);
-  const uint32_t lo = blockIdx.x * a.tile;
-  const uint32_t hi = min(lo + a.tile, a.n);
-  for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_BLOCK) {
-    fr_t s = from_mont(load<FrP>(scalars + 8 * (size_t)i));
-    store<FrP>(canon + 8 * (size_t)i, s);
+  const TileDesc tl = a.tiles[blockIdx.x];
+  const JobDesc j = a.jobs[tl.job];
+  for (uint32_t i = tl.lo + threadIdx.x; i < tl.hi; i += SORT_BLOCK) {
+    fr_t s = from_mont(load<FrP>(j.coeffs + 8 * (size_t)i));
+    store<FrP>(canon + 8 * ((size_t)j.canon_off + i), s);
     uint32_t carry = 0;
     for (int w = 0; w < a.W; w++) {
       int d = window_digit(s.v, w, a.c, carry);
-      if (d != 0) {
-        uint32_t key = (uint32_t)(d > 0 ? d : -d) - 1u;
-        if (a.per_window) key += (uint32_t)w * B;
-        atomicAdd(&cnt[key >> a.l2], 1u);
-      }
+      if (d != 0) atomicAdd(&cnt[bucket_key(a, j, w, d) >> a.l2], 1u);
     }
   }
   __syncthreads();
@@ -105,20 +125,18 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __r
   extern __shared__ uint32_t cur[];   // D1 cursors
   for (uint32_t b = threadIdx.x; b < a.D1; b += SORT_BLOCK) cur[b] = hoff[(size_t)b * a.G + blockIdx.x];
   __syncthreads();
-  const uint32_t B = 1u << (a.c - 1);
   const uint32_t mask2 = (1u << a.l2) - 1u;
-  const uint32_t lo = blockIdx.x * a.tile;
-  const uint32_t hi = min(lo + a.tile, a.n);
-  for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_BLOCK) {
-    fr_t s = load<FrP>(canon + 8 * (size_t)i);
+  const TileDesc tl = a.tiles[blockIdx.x];
+  const JobDesc j = a.jobs[tl.job];
+  for (uint32_t i = tl.lo + threadIdx.x; i < tl.hi; i += SORT_BLOCK) {
+    fr_t s = load<FrP>(canon + 8 * ((size_t)j.canon_off + i));
     uint32_t carry = 0;
     for (int w = 0; w < a.W; w++) {
       int d = window_digit(s.v, w, a.c, carry);
       if (d != 0) {
-        uint32_t key = (uint32_t)(d > 0 ? d : -d) - 1u;
-        uint32_t ref = a.ref_offset + i;
-        if (a.per_window) key += (uint32_t)w * B;
-        else ref += (uint32_t)w * a.ref_stride;
+        const uint32_t key = bucket_key(a, j, w, d);
+        uint32_t ref = j.ref_offset + i;
+        if (!a.per_window) ref += (uint32_t)w * a.ref_stride;
         uint32_t pos = atomicAdd(&cur[key >> a.l2], 1u);
         tmp_key[pos] = key & mask2;
         tmp_ref[pos] = ref | (d < 0 ? NEG_FLAG : 0u);
@@ -396,21 +414,80 @@ MsmPlan msm_plan(size_t n, int per_window, int num_cus, int c_override) {
   return p;
 }
 
-int msm_run(mzk_ctx* c, const MsmPlan& p, const uint32_t* d_scalars, size_t n, const uint32_t* d_bases,
-            uint32_t ref_stride, uint32_t ref_offset, uint64_t out[18]) {
-  if (n == 0) {
-    memset(out, 0, 18 * 8);
+// Runs fn(j) for j < n on up to 16 host threads (serially for small n).
+template <class F>
+static void parallel_jobs(size_t n, F fn) {
+  const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>(std::min<size_t>(16, hw), n / 2);
+  if (nt <= 1) {
+    for (size_t j = 0; j < n; j++) fn(j);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (size_t t = 0; t < nt; t++)
+    th.emplace_back([&, t]() {
+      for (size_t j = t; j < n; j += nt) fn(j);
+    });
+  for (auto& x : th) x.join();
+}
+
+// Host Horner over one job's channel positions.  Set s (of sets_per_job) weighs
+// 2^(c s); inside it, sum_t A_t has weight 1 and U_j (bit j of t) 2^(log2 L + j).
+static void job_tail(const uint32_t* h, int sets_per_job, int c, uint32_t nch, uint32_t lbits, uint32_t tbits,
+                     uint64_t out[18]) {
+  const int top = (sets_per_job - 1) * c + (int)(lbits + tbits);
+  std::vector<int> at(top + 1, -1), at2(top + 1, -1);   // channel per position (two may share)
+  for (int s = 0; s < sets_per_job; s++) {
+    for (uint32_t h2 = 0; h2 < nch; h2++) {
+      const int pos = s * c + (h2 == 0 ? 0 : (int)(lbits + h2 - 1));
+      (at[pos] < 0 ? at[pos] : at2[pos]) = (int)(s * nch + h2);
+    }
+  }
+  host::Xyzz acc = host::Xyzz::inf();
+  for (int pos = top; pos >= 0; pos--) {
+    acc = host::xyzz_dbl(acc);
+    if (at[pos] >= 0) acc = host::xyzz_add(acc, to_host(h + 48 * (size_t)at[pos]));
+    if (at2[pos] >= 0) acc = host::xyzz_add(acc, to_host(h + 48 * (size_t)at2[pos]));
+  }
+  host::xyzz_to_jacobian(acc, out);
+}
+
+int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, const uint32_t* d_bases,
+                 uint32_t ref_stride, uint64_t* outs) {
+  const size_t nj = jobs.size();
+  size_t ntot = 0;
+  for (auto& j : jobs) ntot += j.n;
+  if (ntot == 0) {
+    memset(outs, 0, nj * 18 * 8);
     return MZK_OK;
   }
-  const size_t M_max = (size_t)p.W * n;
+  const size_t M_max = (size_t)p.W * ntot;
   MZK_REQUIRE(M_max < 0x7fffffffull, "msm: %zu window digits exceed the 2^31 index range", M_max);
-  const uint32_t NB = p.NB;
+  const int spj = p.per_window ? p.W : 1;   // bucket sets per job
+  const size_t sets = nj * (size_t)spj;
+  MZK_REQUIRE(sets * p.B < (1ull << 31), "msm: %zu bucket sets too many", sets);
+  const uint32_t NB = (uint32_t)(sets * p.B);
 
-  // sort geometry: D1 bins of the top key bits (LDS histogram), G tiles of scalars
+  // tiles of at most ~2048 scalars, never crossing a job
+  std::vector<JobDesc> jd(nj);
+  std::vector<TileDesc> td;
+  // ~1024-1536 tiles plus at most one partial tile per job keeps the (bin, tile)
+  // histogram within the two-level scan (2048 bins x 2048 tiles)
+  const size_t tile = std::max<size_t>(256, (ntot + 1535) / 1536);
+  MZK_REQUIRE(nj < 512, "msm: at most 511 jobs per batch (%zu given)", nj);
+  size_t canon_off = 0;
+  for (size_t j = 0; j < nj; j++) {
+    jd[j].coeffs = (const uint32_t*)jobs[j].d_coeffs;
+    jd[j].n = (uint32_t)jobs[j].n;
+    jd[j].ref_offset = jobs[j].ref_offset;
+    jd[j].canon_off = (uint32_t)canon_off;
+    jd[j].set_base = (uint32_t)(j * spj);
+    for (size_t lo = 0; lo < jobs[j].n; lo += tile)
+      td.push_back(TileDesc{(uint32_t)j, (uint32_t)lo, (uint32_t)std::min(lo + tile, jobs[j].n)});
+    canon_off += jobs[j].n;
+  }
   SortArgs sa;
-  sa.n = (uint32_t)n;
-  sa.G = (uint32_t)std::min<size_t>(1024, std::max<size_t>(1, (n + 2047) / 2048));
-  sa.tile = (uint32_t)((n + sa.G - 1) / sa.G);
+  sa.G = (uint32_t)td.size();
   sa.c = p.c;
   sa.W = p.W;
   sa.per_window = p.per_window;
@@ -419,37 +496,45 @@ int msm_run(mzk_ctx* c, const MsmPlan& p, const uint32_t* d_scalars, size_t n, c
   sa.l2 = (uint32_t)(nb_bits - l1);
   sa.D1 = (uint32_t)((NB + (1u << sa.l2) - 1) >> sa.l2);
   sa.ref_stride = ref_stride;
-  sa.ref_offset = ref_offset;
   const size_t HN = (size_t)sa.D1 * sa.G;   // histogram entries
   const size_t tiles = (HN + SCAN_TILE - 1) / SCAN_TILE;
   MZK_REQUIRE(tiles <= (size_t)SCAN_TILE, "msm: sort histogram too large (%zu)", HN);
-  const size_t nchunks = (M_max + p.K - 1) / p.K;
+  const uint32_t K = p.K;
+  const size_t nchunks = (M_max + K - 1) / K;
   // reduction geometry: segments of L buckets, T per set; channels A + bits of t.
-  // One shared set has few, large buckets: a thread per bucket keeps chains short.
-  // Many per-window sets have many small buckets: segments of 4 cut the channel work.
-  const uint32_t L = std::min<uint32_t>(p.per_window ? 4 : 1, p.B);
+  // One MSM alone: a thread per bucket keeps chains short.  Many sets: segments of
+  // 4 cut the channel work, which is then parallel across sets.
+  const uint32_t L = std::min<uint32_t>(sets > 1 ? 4 : 1, p.B);
   const uint32_t T = p.B / L;
   const uint32_t tbits = (uint32_t)ceil_log2(T);
   const uint32_t nch = 1 + tbits;
   const uint32_t parts = std::max<uint32_t>(1, T / 2048);
-  const size_t nchan = (size_t)p.sets * nch;
+  const size_t nchan = sets * nch;
   const uint32_t lbits = (uint32_t)ceil_log2(L);
 
-  // scratch A: canonical scalars | hist | hoff (HN+1) | tile sums | off (NB+1)
-  size_t a_words = 8 * n + HN + (HN + 1) + tiles + (NB + 1) + 64;
+  // scratch A: descriptors | canonical scalars | hist | hoff (HN+1) | tile sums | off (NB+1)
+  const size_t desc_words = (nj * sizeof(JobDesc) + td.size() * sizeof(TileDesc) + 15) / 4 + 8;
+  size_t a_words = desc_words + 8 * ntot + HN + (HN + 1) + tiles + ((size_t)NB + 1) + 64;
   MZK_TRY(c->scratch[SCR_MSM_A].reserve(a_words * 4));
-  uint32_t* canon = (uint32_t*)c->scratch[SCR_MSM_A].ptr;
-  uint32_t* hist = canon + 8 * n;
+  uint32_t* base = (uint32_t*)c->scratch[SCR_MSM_A].ptr;
+  JobDesc* d_jobs = (JobDesc*)base;
+  TileDesc* d_tiles = (TileDesc*)((char*)base + ((nj * sizeof(JobDesc) + 15) & ~(size_t)15));
+  uint32_t* canon = base + ((desc_words + 3) & ~(size_t)3);
+  uint32_t* hist = canon + 8 * ntot;
   uint32_t* hoff = hist + HN;
   uint32_t* tsums = hoff + HN + 1;
   uint32_t* off = tsums + tiles;
+  MZK_HIP(hipMemcpyAsync(d_jobs, jd.data(), nj * sizeof(JobDesc), hipMemcpyHostToDevice, c->stream));
+  MZK_HIP(hipMemcpyAsync(d_tiles, td.data(), td.size() * sizeof(TileDesc), hipMemcpyHostToDevice, c->stream));
+  sa.jobs = d_jobs;
+  sa.tiles = d_tiles;
   // scratch B: tmp keys | tmp refs | sorted refs
   MZK_TRY(c->scratch[SCR_MSM_B].reserve(3 * M_max * 4 + 64));
   uint32_t* tmp_key = (uint32_t*)c->scratch[SCR_MSM_B].ptr;
   uint32_t* tmp_ref = tmp_key + M_max;
   uint32_t* refs = tmp_ref + M_max;
   // scratch C: chunk partials (nchunks + NB slots) then segment sums A, R (sets * T each)
-  const size_t nseg = (size_t)p.sets * T;
+  const size_t nseg = sets * T;
   size_t c_words = (nchunks + NB) * 48 + 2 * nseg * 48;
   MZK_TRY(c->scratch[SCR_MSM_C].reserve(c_words * 4));
   uint32_t* partials = (uint32_t*)c->scratch[SCR_MSM_C].ptr;
@@ -461,7 +546,7 @@ int msm_run(mzk_ctx* c, const MsmPlan& p, const uint32_t* d_scalars, size_t n, c
   uint32_t* chan_parts = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
   uint32_t* chan = chan_parts + nchan * parts * 48;
 
-  MZK_LAUNCH(c, "k_sort_hist", k_sort_hist, dim3(sa.G), dim3(SORT_BLOCK), sa.D1 * 4, d_scalars, sa, canon, hist);
+  MZK_LAUNCH(c, "k_sort_hist", k_sort_hist, dim3(sa.G), dim3(SORT_BLOCK), sa.D1 * 4, sa, canon, hist);
   MZK_LAUNCH(c, "k_scan_tiles", k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, hist, hoff, HN, tsums);
   MZK_LAUNCH(c, "k_scan_sums", k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, tsums, (int)tiles, hoff, HN);
   MZK_LAUNCH(c, "k_scan_add", k_scan_add, dim3(grid_for(HN, 256)), dim3(256), 0, hoff, HN, tsums);
@@ -469,9 +554,9 @@ int msm_run(mzk_ctx* c, const MsmPlan& p, const uint32_t* d_scalars, size_t n, c
              tmp_ref);
   MZK_LAUNCH(c, "k_sort_bins", k_sort_bins, dim3(sa.D1), dim3(SORT_BLOCK), (1u << sa.l2) * 4, hoff, sa, NB, tmp_key,
              tmp_ref, refs, off);
-  MZK_LAUNCH(c, "k_msm_accum", k_msm_accum, dim3(grid_for(nchunks, 64)), dim3(64), 0, refs, off, NB, p.K, d_bases,
+  MZK_LAUNCH(c, "k_msm_accum", k_msm_accum, dim3(grid_for(nchunks, 64)), dim3(64), 0, refs, off, NB, K, d_bases,
              partials);
-  MZK_LAUNCH(c, "k_bucket_seg", k_bucket_seg, dim3(grid_for(nseg, 64)), dim3(64), 0, off, NB, p.K, L, partials, segA,
+  MZK_LAUNCH(c, "k_bucket_seg", k_bucket_seg, dim3(grid_for(nseg, 64)), dim3(64), 0, off, NB, K, L, partials, segA,
              segR);
   MZK_LAUNCH(c, "k_chan_sum", k_chan_sum, dim3((unsigned)(nchan * parts)), dim3(CHAN_BLOCK), CHAN_BLOCK * 48 * 4, segA,
              segR, T, nch, parts, chan_parts);
@@ -488,29 +573,26 @@ int msm_run(mzk_ctx* c, const MsmPlan& p, const uint32_t* d_scalars, size_t n, c
   c->prof.host("host_wait", h1 - h0);
   c->last_msm[0] = (uint64_t)p.c;
   c->last_msm[1] = (uint64_t)p.W;
-  c->last_msm[2] = (uint64_t)p.sets;
-  c->last_msm[3] = p.K;
+  c->last_msm[2] = (uint64_t)sets;
+  c->last_msm[3] = K;
   c->last_msm[4] = M;
-  c->last_msm[5] = n;
-  // Set s weighs 2^(c s); inside it, sum_t A_t has weight 1 and U_j (bit j of t)
-  // weight 2^(log2 L + j).  Horner over bit positions from the top.
-  const int top = (p.sets - 1) * p.c + (int)(lbits + tbits);
-  std::vector<int> at(top + 1, -1), at2(top + 1, -1);   // channel index per position (two may share)
-  for (int s = 0; s < p.sets; s++) {
-    for (uint32_t h2 = 0; h2 < nch; h2++) {
-      const int pos = s * p.c + (h2 == 0 ? 0 : (int)(lbits + h2 - 1));
-      (at[pos] < 0 ? at[pos] : at2[pos]) = (int)(s * nch + h2);
-    }
-  }
-  host::Xyzz acc = host::Xyzz::inf();
-  for (int pos = top; pos >= 0; pos--) {
-    acc = host::xyzz_dbl(acc);
-    if (at[pos] >= 0) acc = host::xyzz_add(acc, to_host(h.data() + 48 * (size_t)at[pos]));
-    if (at2[pos] >= 0) acc = host::xyzz_add(acc, to_host(h.data() + 48 * (size_t)at2[pos]));
-  }
-  host::xyzz_to_jacobian(acc, out);
+  c->last_msm[5] = ntot;
+  const size_t per_job = (size_t)spj * nch * 48;
+  parallel_jobs(nj, [&](size_t j) {
+    if (jobs[j].n == 0) memset(outs + 18 * j, 0, 18 * 8);
+    else job_tail(h.data() + j * per_job, spj, p.c, nch, lbits, tbits, outs + 18 * j);
+  });
   c->prof.host("host_horner", host_ms() - h1);
   return MZK_OK;
+}
+
+int msm_run(mzk_ctx* c, const MsmPlan& p, const uint32_t* d_scalars, size_t n, const uint32_t* d_bases,
+            uint32_t ref_stride, uint32_t ref_offset, uint64_t out[18]) {
+  std::vector<MsmJob> jobs(1);
+  jobs[0].d_coeffs = d_scalars;
+  jobs[0].n = n;
+  jobs[0].ref_offset = ref_offset;
+  return msm_run_jobs(c, p, jobs, d_bases, ref_stride, out);
 }
 
 }  // namespace mzk
@@ -560,6 +642,23 @@ int mzk_msm_best(mzk_ctx* c, const uint64_t* coeffs, const uint64_t* bases, size
 // (kzg/msm.rs:176-181) is implicit in the bucket histogram.
 int mzk_msm_specific(mzk_ctx* c, const uint64_t* coeffs, const uint64_t* bases, size_t n, uint64_t out[18]) {
   return msm_host(c, coeffs, bases, n, out);
+}
+
+int mzk_g1_jacobian_to_affine(const uint64_t jac[18], uint64_t aff[12]) {
+  MZK_REQUIRE(jac != nullptr && aff != nullptr, "mzk_g1_jacobian_to_affine: null argument");
+  host::Fp X, Y, Z;
+  memcpy(X.l, jac, 48);
+  memcpy(Y.l, jac + 6, 48);
+  memcpy(Z.l, jac + 12, 48);
+  if (host::fp_is_zero(Z)) {
+    memset(aff, 0, 96);
+    return MZK_OK;
+  }
+  host::Fp zi = host::fp_inv(Z), zi2 = host::fp_mul(zi, zi);
+  host::Fp x = host::fp_mul(X, zi2), y = host::fp_mul(Y, host::fp_mul(zi2, zi));
+  memcpy(aff, x.l, 48);
+  memcpy(aff + 6, y.l, 48);
+  return MZK_OK;
 }
 
 int mzk_msm_last_plan(mzk_ctx* c, uint64_t out[6]) {

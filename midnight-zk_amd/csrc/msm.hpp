@@ -1,6 +1,8 @@
 // MSM planning and the shared device pipeline (msm.hip), used by the per-call
 // entry points and by the resident-SRS commit path.
 #pragma once
+#include <vector>
+
 #include "ctx.hpp"
 
 namespace mzk {
@@ -16,6 +18,18 @@ struct MsmPlan {
 };
 
 MsmPlan msm_plan(size_t n, int per_window, int num_cus, int c_override);
+
+// One MSM of a batch: n device scalars; its point references start at ref_offset.
+struct MsmJob {
+  const void* d_coeffs = nullptr;
+  size_t n = 0;
+  uint32_t ref_offset = 0;
+};
+
+// Runs all jobs through one pipeline (one sort, one accumulation, one reduction):
+// outs[18 j ..] receives job j's Jacobian result.
+int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, const uint32_t* d_bases,
+                 uint32_t ref_stride, uint64_t* outs);
 
 // Runs the pipeline on device scalars (n x 8 u32 Montgomery) and device bases.
 // Reference r of scalar i in window w is ref_offset + i (+ w * ref_stride when

@@ -59,11 +59,30 @@ __device__ __forceinline__ uint32_t bitrev(uint32_t j, uint32_t bits) {
   return bits == 0 ? 0u : (__brev(j) >> (32 - bits));
 }
 
+__device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block, uint32_t* lds);
+
 __global__ void __launch_bounds__(256) k_ntt_pass(PassArgs a) {
   extern __shared__ uint32_t lds[];   // C * L elements of 8 words
+  ntt_pass_body(a, blockIdx.x, lds);
+}
+
+// Many transforms in one launch: workgroups [first[i], first[i+1]) run pass table[i].
+__global__ void __launch_bounds__(256) k_ntt_pass_batch(const PassArgs* __restrict__ table,
+                                                        const uint32_t* __restrict__ first, uint32_t njobs) {
+  extern __shared__ uint32_t lds[];
+  uint32_t lo = 0, hi = njobs;   // first[lo] <= blockIdx.x < first[hi]
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (first[mid] <= blockIdx.x) lo = mid; else hi = mid;
+  }
+  const PassArgs a = table[lo];
+  ntt_pass_body(a, blockIdx.x - first[lo], lds);
+}
+
+__device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block, uint32_t* lds) {
   const uint32_t L = 1u << a.log_l;
   const uint32_t CL = a.C * L;
-  const uint64_t b0 = (uint64_t)blockIdx.x * a.C;
+  const uint64_t b0 = (uint64_t)block * a.C;
   const uint32_t nt = blockDim.x;
 
   // load (bit-reversed within each sub-DFT)
@@ -217,11 +236,12 @@ static int launch_pass(mzk_ctx* c, const char* name, PassArgs a, uint64_t nbatch
   return MZK_OK;
 }
 
-int ntt_run(mzk_ctx* c, const NttPlan& plan, const uint32_t* src, uint32_t* dst, uint64_t n_real, int in_coset,
-            int out_mode, const host::Fr& scale) {
+// The passes of one transform: 1 (N <= 2^11, straight from src to dst) or 2 (through
+// tmp, which must hold N elements).  Returns the pass count; groups[i] = workgroups.
+static int build_passes(const NttPlan& plan, const uint32_t* src, uint32_t* dst, uint32_t* tmp, uint64_t n_real,
+                        int in_coset, int out_mode, const host::Fr& scale, PassArgs pa[2], uint64_t groups[2]) {
   const uint32_t log_n = plan.log_n;
   const uint64_t N = 1ull << log_n;
-  hipStream_t st = c->stream;
   host::Fr z = host::fr_const(host::kFrZeta);
   host::Fr z2 = host::fr_mul(z, z);
   PassArgs a;
@@ -244,11 +264,11 @@ int ntt_run(mzk_ctx* c, const NttPlan& plan, const uint32_t* src, uint32_t* dst,
     a.log_l = log_n;
     a.C = 1;
     a.out_mode = out_mode;
-    return launch_pass(c, "k_ntt_pass_single", a, 1);
+    pa[0] = a;
+    groups[0] = 1;
+    return 1;
   }
   const uint64_t N1 = 1ull << plan.l1, N2 = 1ull << plan.l2;
-  MZK_TRY(c->scratch[SCR_NTT].reserve(N * 32));
-  uint32_t* tmp = (uint32_t*)c->scratch[SCR_NTT].ptr;
   // pass 1: columns (size N2), twist, out-of-place into tmp
   a.src = src;
   a.dst = tmp;
@@ -263,7 +283,8 @@ int ntt_run(mzk_ctx* c, const NttPlan& plan, const uint32_t* src, uint32_t* dst,
   a.log_l = plan.l2;
   a.C = choose_c((uint32_t)N2, N1);
   a.out_mode = 0;
-  MZK_TRY(launch_pass(c, "k_ntt_pass1", a, N1));
+  pa[0] = a;
+  groups[0] = N1 / a.C;
   // pass 2: rows (size N1), natural-order output
   PassArgs b = a;
   b.src = tmp;
@@ -279,7 +300,105 @@ int ntt_run(mzk_ctx* c, const NttPlan& plan, const uint32_t* src, uint32_t* dst,
   b.log_l = plan.l1;
   b.C = choose_c((uint32_t)N1, N2);
   b.out_mode = out_mode;
-  return launch_pass(c, "k_ntt_pass2", b, N2);
+  pa[1] = b;
+  groups[1] = N2 / b.C;
+  return 2;
+}
+
+int ntt_run(mzk_ctx* c, const NttPlan& plan, const uint32_t* src, uint32_t* dst, uint64_t n_real, int in_coset,
+            int out_mode, const host::Fr& scale) {
+  const uint64_t N = 1ull << plan.log_n;
+  uint32_t* tmp = nullptr;
+  if (plan.log_n > (uint32_t)NTT_MAX_LOG_L) {
+    MZK_TRY(c->scratch[SCR_NTT].reserve(N * 32));
+    tmp = (uint32_t*)c->scratch[SCR_NTT].ptr;
+  }
+  PassArgs pa[2];
+  uint64_t groups[2];
+  const int np = build_passes(plan, src, dst, tmp, n_real, in_coset, out_mode, scale, pa, groups);
+  static const char* names[3] = {"k_ntt_pass_single", "k_ntt_pass1", "k_ntt_pass2"};
+  for (int i = 0; i < np; i++) MZK_TRY(launch_pass(c, names[np == 1 ? 0 : 1 + i], pa[i], groups[i] * pa[i].C));
+  return MZK_OK;
+}
+
+// Cached plan for (log_n, inverse) on a context: omega = ROOT^(2^(32 - log_n)) or its inverse.
+const NttPlan* ctx_plan(mzk_ctx* c, uint32_t log_n, int inverse) {
+  const int key = (int)log_n * 2 + (inverse ? 1 : 0);
+  auto it = c->ntt_plans.find(key);
+  if (it != c->ntt_plans.end()) return it->second;
+  host::Fr w = host::fr_root_of_unity(log_n);
+  if (inverse) w = host::fr_inv(w);
+  NttPlan* p = new NttPlan();
+  if (p->init(c->stream, log_n, w) != MZK_OK) {
+    delete p;
+    return nullptr;
+  }
+  c->ntt_plans[key] = p;
+  return p;
+}
+
+void free_ntt_plans(mzk_ctx* c) {
+  for (auto& kv : c->ntt_plans) delete kv.second;
+  c->ntt_plans.clear();
+}
+
+// Runs many transforms with two launches in all: every job's first pass, then
+// every two-pass job's second pass (each workgroup finds its job in a table).
+int ntt_run_batch(mzk_ctx* c, const mzk_ntt_job* jobs, size_t njobs) {
+  size_t tmp_elems = 0;
+  for (size_t j = 0; j < njobs; j++)
+    if (jobs[j].log_n > (uint32_t)NTT_MAX_LOG_L) tmp_elems += 1ull << jobs[j].log_n;
+  MZK_TRY(c->scratch[SCR_NTT].reserve(tmp_elems * 32 + 64));
+  uint32_t* tmp = (uint32_t*)c->scratch[SCR_NTT].ptr;
+  std::vector<PassArgs> t1, t2;
+  std::vector<uint32_t> f1(1, 0), f2(1, 0);
+  for (size_t j = 0; j < njobs; j++) {
+    const mzk_ntt_job& jb = jobs[j];
+    const int inverse = (jb.flags & MZK_NTT_INVERSE) != 0;
+    const int coset = (jb.flags & MZK_NTT_COSET) != 0;
+    const NttPlan* plan = ctx_plan(c, jb.log_n, inverse);
+    MZK_REQUIRE(plan != nullptr, "ntt batch: plan for 2^%u failed", jb.log_n);
+    const uint64_t N = 1ull << jb.log_n;
+    host::Fr scale = inverse ? host::fr_inv(host::fr_from_u64(N)) : host::fr_one();
+    PassArgs pa[2];
+    uint64_t groups[2];
+    const int np = build_passes(*plan, (const uint32_t*)jb.d_in, (uint32_t*)jb.d_out, tmp,
+                                inverse ? N : jb.n_real, inverse ? 0 : coset,
+                                inverse ? (coset ? 2 : 1) : 0, scale, pa, groups);
+    if (np == 2) tmp += 8 * N;
+    t1.push_back(pa[0]);
+    f1.push_back(f1.back() + (uint32_t)groups[0]);
+    if (np == 2) {
+      t2.push_back(pa[1]);
+      f2.push_back(f2.back() + (uint32_t)groups[1]);
+    }
+  }
+  // tables: PassArgs and first-workgroup arrays, copied once into the batch scratch
+  const size_t bytes = (t1.size() + t2.size()) * sizeof(PassArgs) + (f1.size() + f2.size()) * 4 + 64;
+  MZK_TRY(c->scratch[SCR_IN1].reserve(bytes));
+  char* d = (char*)c->scratch[SCR_IN1].ptr;
+  std::vector<char> h(bytes);
+  size_t o = 0;
+  auto put = [&](const void* src, size_t n) {
+    memcpy(h.data() + o, src, n);
+    size_t at = o;
+    o = (o + n + 15) & ~(size_t)15;
+    return d + at;
+  };
+  PassArgs* d_t1 = (PassArgs*)put(t1.data(), t1.size() * sizeof(PassArgs));
+  PassArgs* d_t2 = (PassArgs*)put(t2.data(), t2.size() * sizeof(PassArgs));
+  uint32_t* d_f1 = (uint32_t*)put(f1.data(), f1.size() * 4);
+  uint32_t* d_f2 = (uint32_t*)put(f2.data(), f2.size() * 4);
+  MZK_HIP(hipMemcpyAsync(d, h.data(), o, hipMemcpyHostToDevice, c->stream));
+  const size_t lds = (size_t)NTT_LDS_ELEMS * 32;
+  if (!t1.empty())
+    MZK_LAUNCH(c, "k_ntt_pass_batch1", k_ntt_pass_batch, dim3(f1.back()), dim3(256), lds, d_t1, d_f1,
+               (uint32_t)t1.size());
+  if (!t2.empty())
+    MZK_LAUNCH(c, "k_ntt_pass_batch2", k_ntt_pass_batch, dim3(f2.back()), dim3(256), lds, d_t2, d_f2,
+               (uint32_t)t2.size());
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
 }
 
 }  // namespace mzk
@@ -450,6 +569,44 @@ int mzk_extended_to_coeff(mzk_domain* d, uint64_t* a) {
 int mzk_coeff_to_extended(mzk_domain* d, const uint64_t* in, uint64_t* out) {
   MZK_TRY(domain_check(d));
   return run_host(d, in, out, 1ull << d->k, 1ull << d->ext_k, c2e_impl);
+}
+
+static int check_ntt_job(const mzk_ntt_job& j, size_t i) {
+  MZK_REQUIRE(j.d_in != nullptr && j.d_out != nullptr, "ntt job %zu: null data", i);
+  MZK_REQUIRE(j.log_n >= 1 && j.log_n <= (uint32_t)(2 * NTT_MAX_LOG_L), "ntt job %zu: log_n %u outside [1, %d]", i,
+              j.log_n, 2 * NTT_MAX_LOG_L);
+  MZK_REQUIRE((j.flags & ~(uint32_t)(MZK_NTT_INVERSE | MZK_NTT_COSET)) == 0, "ntt job %zu: unknown flags", i);
+  MZK_REQUIRE(j.n_real <= (1ull << j.log_n), "ntt job %zu: n_real above 2^log_n", i);
+  MZK_REQUIRE(!(j.flags & MZK_NTT_INVERSE) || j.n_real == (1ull << j.log_n),
+              "ntt job %zu: inverse transforms take 2^log_n inputs", i);
+  // the two-pass form reads all input before writing dst only through tmp; a coset or
+  // padded forward transform with d_in == d_out is still correct because pass 1 reads
+  // d_in and writes tmp, and pass 2 writes d_out after pass 1 completes
+  return MZK_OK;
+}
+
+int mzk_ntt_dev(mzk_ctx* c, const mzk_ntt_job* job) {
+  MZK_REQUIRE(c != nullptr && job != nullptr, "mzk_ntt_dev: null argument");
+  MZK_TRY(check_ntt_job(*job, 0));
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  const int inverse = (job->flags & MZK_NTT_INVERSE) != 0, coset = (job->flags & MZK_NTT_COSET) != 0;
+  const NttPlan* plan = ctx_plan(c, job->log_n, inverse);
+  MZK_REQUIRE(plan != nullptr, "mzk_ntt_dev: plan for 2^%u failed", job->log_n);
+  const uint64_t N = 1ull << job->log_n;
+  if (inverse)
+    return ntt_run(c, *plan, (const uint32_t*)job->d_in, (uint32_t*)job->d_out, N, 0, coset ? 2 : 1,
+                   host::fr_inv(host::fr_from_u64(N)));
+  return ntt_run(c, *plan, (const uint32_t*)job->d_in, (uint32_t*)job->d_out, job->n_real, coset, 0, host::fr_one());
+}
+
+int mzk_ntt_batch_dev(mzk_ctx* c, const mzk_ntt_job* jobs, size_t njobs) {
+  MZK_REQUIRE(c != nullptr && (jobs != nullptr || njobs == 0), "mzk_ntt_batch_dev: null argument");
+  for (size_t i = 0; i < njobs; i++) MZK_TRY(check_ntt_job(jobs[i], i));
+  if (njobs == 0) return MZK_OK;
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  return ntt_run_batch(c, jobs, njobs);
 }
 
 // best_fft (fft.rs:49): DFT with a caller-supplied omega, in place on host memory.

@@ -348,4 +348,32 @@ int mzk_msm_srs_dev(mzk_ctx* c, const mzk_srs* s, size_t offset, const void* d_c
   return msm_run(c, p, (const uint32_t*)d_coeffs, n, (const uint32_t*)s->bases + 24 * offset, 0, 0, out);
 }
 
+int mzk_msm_srs_batch_dev(mzk_ctx* c, const mzk_srs* s, const mzk_msm_job* jobs, size_t njobs, uint64_t* outs) {
+  MZK_REQUIRE(c != nullptr && s != nullptr && (jobs != nullptr || njobs == 0) && (outs != nullptr || njobs == 0),
+              "mzk_msm_srs_batch_dev: null argument");
+  if (njobs == 0) return MZK_OK;
+  std::vector<MsmJob> v(njobs);
+  size_t total = 0;
+  for (size_t j = 0; j < njobs; j++) {
+    MZK_REQUIRE(jobs[j].offset <= s->n && jobs[j].n <= s->n - jobs[j].offset,
+                "mzk_msm_srs_batch_dev: job %zu (%zu bases from %zu) exceeds the SRS (%zu)", j, jobs[j].n,
+                jobs[j].offset, s->n);
+    MZK_REQUIRE(jobs[j].d_coeffs != nullptr || jobs[j].n == 0, "mzk_msm_srs_batch_dev: job %zu has no coefficients",
+                j);
+    v[j].d_coeffs = jobs[j].d_coeffs;
+    v[j].n = jobs[j].n;
+    v[j].ref_offset = (uint32_t)jobs[j].offset;
+    total += jobs[j].n;
+  }
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  if (s->shifted) {
+    MsmPlan p = msm_plan(total, 0, c->num_cus, s->shifted_c);
+    return msm_run_jobs(c, p, v, (const uint32_t*)s->shifted, (uint32_t)s->n, outs);
+  }
+  MsmPlan p = msm_plan(total / njobs + 1, 1, c->num_cus, 0);
+  MsmPlan pt = msm_plan(total, 1, c->num_cus, p.c);   // K from the batch's total digits
+  return msm_run_jobs(c, pt, v, (const uint32_t*)s->bases, 0, outs);
+}
+
 }  // extern "C"

@@ -32,6 +32,31 @@ class MzkError(RuntimeError):
     pass
 
 
+class MsmJob(ctypes.Structure):
+    """mzk_msm_job: one commitment of a batch."""
+    _fields_ = [("d_coeffs", ctypes.c_void_p), ("n", ctypes.c_size_t), ("offset", ctypes.c_size_t)]
+
+
+NTT_INVERSE = 1
+NTT_COSET = 2
+
+
+class NttJob(ctypes.Structure):
+    """mzk_ntt_job: one transform of size 2^log_n (flags NTT_INVERSE / NTT_COSET)."""
+    _fields_ = [("d_in", ctypes.c_void_p), ("d_out", ctypes.c_void_p), ("n_real", ctypes.c_uint64),
+                ("log_n", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
+
+    @classmethod
+    def make(cls, d_in, d_out, log_n: int, flags: int = 0, n_real: int | None = None) -> "NttJob":
+        j = cls()
+        j.d_in = _t_ptr(d_in)
+        j.d_out = _t_ptr(d_out)
+        j.log_n = log_n
+        j.flags = flags
+        j.n_real = (1 << log_n) if n_real is None else n_real
+        return j
+
+
 _lib = None
 
 # (name, argtypes, restype); every int-returning function is a status code
@@ -58,6 +83,10 @@ SIGNATURES = {
     "mzk_srs_download": ([_P, _S, _S, _P], _I),
     "mzk_srs_destroy": ([_P], None),
     "mzk_msm_srs_dev": ([_P, _P, _S, _P, _S, _P], _I),
+    "mzk_msm_srs_batch_dev": ([_P, _P, _P, _S, _P], _I),
+    "mzk_g1_jacobian_to_affine": ([_P, _P], _I),
+    "mzk_ntt_dev": ([_P, _P], _I),
+    "mzk_ntt_batch_dev": ([_P, _P, _S], _I),
     "mzk_best_fft": ([_P, _P, _P, _U32], _I),
     "mzk_domain_create": ([_P, _U32, _U32, ctypes.POINTER(_P)], _I),
     "mzk_domain_destroy": ([_P], None),
@@ -204,6 +233,14 @@ class Context:
         _check(load_library().mzk_best_fft(self.h, _np_ptr(a), _np_ptr(om), log_n))
         return a
 
+    def ntt(self, job: "NttJob"):
+        _check(load_library().mzk_ntt_dev(self.h, ctypes.byref(job)))
+
+    def ntt_batch(self, jobs):
+        jobs = list(jobs)
+        arr = (NttJob * max(1, len(jobs)))(*jobs)
+        _check(load_library().mzk_ntt_batch_dev(self.h, ctypes.cast(arr, ctypes.c_void_p), len(jobs)))
+
     def domain(self, j: int, k: int) -> "EvaluationDomain":
         return EvaluationDomain(self, j, k)
 
@@ -313,6 +350,20 @@ class ResidentSrs:
     def bases_ptr(self) -> int:
         return load_library().mzk_srs_bases_dev(self.h)
 
+    def commit_batch(self, jobs) -> np.ndarray:
+        """Many commitments in one pipeline.  jobs: iterable of (device coeffs, n, offset);
+        returns an (len(jobs), 18) array of Jacobian results."""
+        jobs = list(jobs)
+        arr = (MsmJob * max(1, len(jobs)))()
+        for i, (d, n, off) in enumerate(jobs):
+            arr[i].d_coeffs = _t_ptr(d) if n else None
+            arr[i].n = n
+            arr[i].offset = off
+        out = np.zeros((max(1, len(jobs)), 18), dtype=np.uint64)
+        _check(load_library().mzk_msm_srs_batch_dev(self.ctx.h, self.h, ctypes.cast(arr, ctypes.c_void_p),
+                                                     len(jobs), _np_ptr(out)))
+        return out[:len(jobs)]
+
     def commit(self, d_coeffs, n: int | None = None, offset: int = 0) -> np.ndarray:
         """KZG commit of device coefficients against bases[offset:offset+n]."""
         n = d_coeffs.shape[0] if n is None else n
@@ -330,6 +381,14 @@ class ResidentSrs:
             self.close()
         except Exception:
             pass
+
+
+def g1_to_affine(jac: np.ndarray) -> np.ndarray:
+    """G1Projective::to_affine: 18-limb Jacobian -> 12-limb affine (identity = zeros)."""
+    jac = np.ascontiguousarray(jac, dtype=np.uint64).reshape(18)
+    out = np.zeros(12, dtype=np.uint64)
+    _check(load_library().mzk_g1_jacobian_to_affine(_np_ptr(jac), _np_ptr(out)))
+    return out
 
 
 def diag_field_op(ctx: Context, field: int, op: int, d_a, d_b, d_out, n: int):
