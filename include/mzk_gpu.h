@@ -76,6 +76,9 @@ typedef struct mzk_srs mzk_srs;
 int mzk_srs_create(mzk_ctx *ctx, const uint64_t *bases, size_t n, mzk_srs **out);
 /* g_i = [tau^i] G1 for i < n, computed on the device (params.rs:154-172, unsafe_setup). */
 int mzk_srs_generate(mzk_ctx *ctx, const uint64_t tau[4], size_t n, mzk_srs **out);
+/* Bases [s_i] G1 for n device scalars s_i: the Lagrange basis of unsafe_setup
+ * (params.rs:174-186, g_lagrange_i = [L_i(tau)] G1) takes s_i = L_i(tau). */
+int mzk_srs_from_scalars_dev(mzk_ctx *ctx, const void *d_scalars, size_t n, mzk_srs **out);
 size_t mzk_srs_len(const mzk_srs *srs);
 /* Device pointer to the n x 12 u64 affine bases. */
 const void *mzk_srs_bases_dev(const mzk_srs *srs);
@@ -148,6 +151,30 @@ int mzk_ntt_batch_dev(mzk_ctx *ctx, const mzk_ntt_job *jobs, size_t njobs);
 int mzk_lagrange_to_coeff(mzk_domain *d, uint64_t *a);
 int mzk_coeff_to_extended(mzk_domain *d, const uint64_t *in, uint64_t *out);
 int mzk_extended_to_coeff(mzk_domain *d, uint64_t *a);
+
+/* ------------------------------------------------------------------ scalar vectors
+ * Device vectors of n Montgomery Fr elements (4 u64 each); in-place use allowed
+ * unless stated.  Element-wise a (+,-,*) b: op 0 = add, 1 = sub, 2 = mul
+ * (Polynomial Add/Sub/Mul, proofs/src/poly/mod.rs). */
+int mzk_fr_vec_op(mzk_ctx *ctx, int op, const void *d_a, const void *d_b, void *d_out, size_t n);
+/* With a scalar s: op 0: out = a * s; 1: out = a + s; 2: out = out + a * s
+ * (Polynomial Mul<F> / Add<F>, mod.rs; the axpy of poly_inner_product, kzg/mod.rs:125-147). */
+int mzk_fr_vec_scalar(mzk_ctx *ctx, int op, const void *d_a, const uint64_t s[4], void *d_out,
+                      size_t n);
+/* a_i <- a_i^-1 in place, zeros left as zero (ff BatchInvert as used by
+ * permutation/prover.rs:148 and domain.rs:141). */
+int mzk_fr_batch_invert(mzk_ctx *ctx, void *d_a, size_t n);
+/* out_i = a_0 * ... * a_i (inclusive products; the grand product of
+ * permutation/prover.rs:190-197 is the exclusive form, out shifted by one).  n <= 2^24.
+ * d_out must not alias d_a. */
+int mzk_fr_prefix_product(mzk_ctx *ctx, const void *d_a, void *d_out, size_t n);
+/* sum_i a_i x^i to host memory (eval_polynomial, utils/arithmetic.rs:59-87). */
+int mzk_fr_eval(mzk_ctx *ctx, const void *d_a, size_t n, const uint64_t x[4], uint64_t out[4]);
+/* Quotient of a(X) - a(b) by (X - b): n - 1 coefficients into d_q (kate_division,
+ * utils/arithmetic.rs:106-125).  n >= 1; d_q must not alias d_a. */
+int mzk_fr_kate_div(mzk_ctx *ctx, const void *d_a, size_t n, const uint64_t b[4], void *d_q);
+/* out_i = x^i for i < n (powers, kzg/mod.rs:151-167 and the domain's omega tables). */
+int mzk_fr_powers(mzk_ctx *ctx, const uint64_t x[4], size_t n, void *d_out);
 
 /* ------------------------------------------------------------------ synthetic inputs
  * Element i of the seeded stream (seed, stream): eight splitmix64 words

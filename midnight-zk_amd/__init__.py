@@ -5,5 +5,6 @@ The directory name is not an identifier; import it with
 the reference's MSM / NTT interface over the C ABI in libmzk_gpu.so.
 """
 from . import mzk  # noqa: F401
+from . import plonk  # noqa: F401
 from .mzk import (Context, EvaluationDomain, MzkError, ResidentSrs,  # noqa: F401
                   load_library)

@@ -275,6 +275,18 @@ inline Fr fr_sub(const Fr& a, const Fr& b) {
   return r;
 }
 
+inline Fr fr_add(const Fr& a, const Fr& b) {   // r < 2^255: the sum fits four limbs
+  Fr r;
+  uint64_t c = 0;
+  for (int i = 0; i < 4; i++) {
+    u128 s = (u128)a.l[i] + b.l[i] + c;
+    r.l[i] = (uint64_t)s;
+    c = (uint64_t)(s >> 64);
+  }
+  if (fr_ge(r.l)) fr_sub_mod(r.l);
+  return r;
+}
+
 inline Fr fr_pow(Fr a, uint64_t e) {
   Fr r = fr_one();
   while (e) {

@@ -1,0 +1,437 @@
+// Scalar-field vector kernels the prover needs around its MSMs and NTTs:
+// element-wise arithmetic (Polynomial ops, proofs/src/poly/mod.rs), batch inversion
+// (ff BatchInvert, used by the permutation argument), prefix products (the grand
+// product z, permutation/prover.rs:177-215), Horner evaluation (eval_polynomial,
+// proofs/src/utils/arithmetic.rs:59-87), Kate division (arithmetic.rs:106-125) and
+// powers of a point.  Scans and the Kate recurrence are parallel three-phase scans.
+#include <vector>
+
+#include "ctx.hpp"
+#include "field.hpp"
+#include "host_ec.hpp"
+
+namespace mzk {
+
+struct FrArg {
+  uint32_t v[8];
+};
+
+__device__ __forceinline__ fr_t arg_fr(const FrArg& a) {
+  fr_t r;
+#pragma unroll
+  for (int i = 0; i < 8; i++) r.v[i] = a.v[i];
+  return r;
+}
+
+static FrArg to_arg(const uint64_t s[4]) {
+  FrArg a;
+  memcpy(a.v, s, 32);
+  return a;
+}
+
+static FrArg to_arg(const host::Fr& f) {
+  FrArg a;
+  memcpy(a.v, f.l, 32);
+  return a;
+}
+
+__global__ void k_vec_binop(int op, const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
+                            uint32_t* __restrict__ out, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  fr_t x = load<FrP>(a + 8 * i), y = load<FrP>(b + 8 * i);
+  fr_t r = op == 0 ? add(x, y) : op == 1 ? sub(x, y) : mul(x, y);
+  store<FrP>(out + 8 * i, r);
+}
+
+// op 0: out = a * s; 1: out = a + s; 2: out = out + a * s
+__global__ void k_vec_scalar(int op, const uint32_t* __restrict__ a, FrArg s, uint32_t* __restrict__ out, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  fr_t x = load<FrP>(a + 8 * i);
+  fr_t r;
+  if (op == 0) r = mul(x, arg_fr(s));
+  else if (op == 1) r = add(x, arg_fr(s));
+  else r = add(load<FrP>(out + 8 * i), mul(x, arg_fr(s)));
+  store<FrP>(out + 8 * i, r);
+}
+
+// a^(r-2) (one dependent chain, used once per chunk)
+__device__ fr_t fr_inv_dev(const fr_t& a) {
+  constexpr uint32_t E[8] = {0xffffffffu, 0xfffffffeu, 0xfffe5bfeu, 0x53bda402u,
+                             0x09a1d805u, 0x3339d808u, 0x299d7d48u, 0x73eda753u};
+  fr_t r = fr_t::one();
+  for (int bit = 254; bit >= 0; bit--) {
+    r = sqr(r);
+    if ((E[bit >> 5] >> (bit & 31)) & 1u) r = mul(r, a);
+  }
+  return r;
+}
+
+static constexpr int INV_CHUNK = 32;
+
+// In place, INV_CHUNK elements per thread, one inversion per chunk; zeros stay zero.
+__global__ void k_batch_invert(uint32_t* __restrict__ a, size_t n, uint32_t* __restrict__ pre) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t i0 = t * INV_CHUNK;
+  if (i0 >= n) return;
+  const size_t i1 = i0 + INV_CHUNK < n ? i0 + INV_CHUNK : n;
+  fr_t acc = fr_t::one();
+  for (size_t i = i0; i < i1; i++) {
+    store<FrP>(pre + 8 * i, acc);
+    fr_t x = load<FrP>(a + 8 * i);
+    if (!x.is_zero()) acc = mul(acc, x);
+  }
+  fr_t inv = fr_inv_dev(acc);
+  for (size_t i = i1; i-- > i0;) {
+    fr_t x = load<FrP>(a + 8 * i);
+    if (x.is_zero()) continue;
+    store<FrP>(a + 8 * i, mul(inv, load<FrP>(pre + 8 * i)));
+    inv = mul(inv, x);
+  }
+}
+
+// ---------------------------------------------------------------- product scan
+static constexpr int SCAN_T = 256;
+static constexpr int SCAN_E = 16;   // elements per thread
+
+// Exclusive block scan of one field element per thread under multiplication.
+__device__ fr_t block_excl_prod(fr_t v, uint32_t* lds, fr_t& total) {
+  const int t = threadIdx.x;
+  store<FrP>(lds + 8 * t, v);
+  __syncthreads();
+  fr_t x = v;
+  for (int d = 1; d < SCAN_T; d <<= 1) {
+    fr_t y = t >= d ? load<FrP>(lds + 8 * (t - d)) : fr_t::one();
+    __syncthreads();
+    x = mul(x, y);
+    store<FrP>(lds + 8 * t, x);
+    __syncthreads();
+  }
+  total = load<FrP>(lds + 8 * (SCAN_T - 1));
+  fr_t excl = t > 0 ? load<FrP>(lds + 8 * (t - 1)) : fr_t::one();
+  __syncthreads();
+  return excl;
+}
+
+// Phase 1: inclusive products within each tile of SCAN_T * SCAN_E; tile totals.
+__global__ void __launch_bounds__(SCAN_T) k_prod_tiles(const uint32_t* __restrict__ a, uint32_t* __restrict__ out,
+                                                       size_t n, uint32_t* __restrict__ totals) {
+  __shared__ uint32_t lds[8 * SCAN_T];
+  const size_t base = ((size_t)blockIdx.x * SCAN_T + threadIdx.x) * SCAN_E;
+  fr_t p = fr_t::one();
+  for (int k = 0; k < SCAN_E; k++)
+    if (base + k < n) p = mul(p, load<FrP>(a + 8 * (base + k)));
+  fr_t total;
+  fr_t run = block_excl_prod(p, lds, total);
+  for (int k = 0; k < SCAN_E; k++) {
+    if (base + k >= n) break;
+    run = mul(run, load<FrP>(a + 8 * (base + k)));
+    store<FrP>(out + 8 * (base + k), run);
+  }
+  if (threadIdx.x == 0) store<FrP>(totals + 8 * blockIdx.x, total);
+}
+
+// Phase 2: one block turns the tile totals into exclusive prefixes (tiles <= SCAN_T * SCAN_E).
+__global__ void __launch_bounds__(SCAN_T) k_prod_totals(uint32_t* __restrict__ totals, size_t tiles) {
+  __shared__ uint32_t lds[8 * SCAN_T];
+  const size_t base = (size_t)threadIdx.x * SCAN_E;
+  fr_t p = fr_t::one();
+  for (int k = 0; k < SCAN_E; k++)
+    if (base + k < tiles) p = mul(p, load<FrP>(totals + 8 * (base + k)));
+  fr_t total;
+  fr_t run = block_excl_prod(p, lds, total);
+  for (int k = 0; k < SCAN_E; k++) {
+    if (base + k >= tiles) break;
+    fr_t v = load<FrP>(totals + 8 * (base + k));
+    store<FrP>(totals + 8 * (base + k), run);
+    run = mul(run, v);
+  }
+}
+
+// Phase 3: out[i] *= exclusive prefix of its tile.
+__global__ void k_prod_apply(uint32_t* __restrict__ out, size_t n, const uint32_t* __restrict__ totals) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t tile = i / (SCAN_T * SCAN_E);
+  if (tile == 0) return;
+  store<FrP>(out + 8 * i, mul(load<FrP>(out + 8 * i), load<FrP>(totals + 8 * tile)));
+}
+
+// ---------------------------------------------------------------- evaluation
+struct Pows {
+  FrArg p[32];   // x^(2^j)
+};
+
+__device__ __forceinline__ fr_t pow_index(const Pows& P, uint64_t e) {
+  fr_t r = fr_t::one();
+  for (int j = 0; e; j++, e >>= 1)
+    if (e & 1) r = mul(r, arg_fr(P.p[j]));
+  return r;
+}
+
+static constexpr int EVAL_CHUNK = 64;
+
+// Horner per chunk of EVAL_CHUNK coefficients, times x^start; block sums in `partials`.
+__global__ void __launch_bounds__(256) k_eval(const uint32_t* __restrict__ a, size_t n, FrArg x, Pows P,
+                                              uint32_t* __restrict__ partials) {
+  __shared__ uint32_t lds[8 * 256];
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t lo = t * EVAL_CHUNK;
+  fr_t acc = fr_t::zero();
+  if (lo < n) {
+    const size_t hi = lo + EVAL_CHUNK < n ? lo + EVAL_CHUNK : n;
+    const fr_t xx = arg_fr(x);
+    for (size_t i = hi; i-- > lo;) acc = add(mul(acc, xx), load<FrP>(a + 8 * i));
+    acc = mul(acc, pow_index(P, lo));
+  }
+  store<FrP>(lds + 8 * threadIdx.x, acc);
+  __syncthreads();
+  for (int d = 128; d >= 1; d >>= 1) {
+    if ((int)threadIdx.x < d) {
+      acc = add(acc, load<FrP>(lds + 8 * (threadIdx.x + d)));
+      store<FrP>(lds + 8 * threadIdx.x, acc);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) store<FrP>(partials + 8 * blockIdx.x, acc);
+}
+
+// ---------------------------------------------------------------- Kate division
+// q_i = a_(i+1) + b q_(i+1), i < m = n - 1, with q_m = 0 (kate_division).  Chunk c holds
+// q indices [c L, c L + L): q_lo = loc_c + b^L q_hi, composed from the top chunk down.
+static constexpr int KATE_L = 64;
+
+// Phase 1: loc_c (the chunk's bottom value with a zero carry-in).
+__global__ void k_kate_local(const uint32_t* __restrict__ a, size_t m, FrArg b, uint32_t* __restrict__ loc) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t lo = c * KATE_L;
+  if (lo >= m) return;
+  const size_t hi = lo + KATE_L < m ? lo + KATE_L : m;
+  const fr_t bb = arg_fr(b);
+  fr_t q = fr_t::zero();
+  for (size_t i = hi; i-- > lo;) q = add(load<FrP>(a + 8 * (i + 1)), mul(bb, q));
+  store<FrP>(loc + 8 * c, q);
+}
+
+// Phase 2 (one block): carry-in of every chunk, i.e. q at the bottom of the chunk above.
+// The maps f_c(x) = loc_c + g x (g = b^L) compose from the top: carry_c = f_(c+1)(carry_(c+1)).
+// Thread t owns chunks [t S, t S + S) (S = ceil(C / 256)); a suffix scan of the composed
+// per-thread maps gives each thread its incoming value.
+__global__ void __launch_bounds__(256) k_kate_carries(const uint32_t* __restrict__ loc, size_t C, FrArg g_L,
+                                                      uint32_t* __restrict__ carry) {
+  __shared__ uint32_t la[8 * 256], lm[8 * 256];
+  const int t = threadIdx.x;
+  const size_t S = (C + 255) / 256;
+  const size_t c0 = t * S;
+  const size_t c1 = c0 + S < C ? c0 + S : C;
+  const fr_t g = arg_fr(g_L);
+  // thread map F_t = f_c0 o f_(c0+1) o ... o f_(c1-1): F(x) = A + M x
+  fr_t A = fr_t::zero(), M = fr_t::one();
+  for (size_t c = c1; c-- > c0;) {   // apply the lower chunk last: F <- f_c o F
+    A = add(load<FrP>(loc + 8 * c), mul(g, A));
+    M = mul(g, M);
+  }
+  if (c0 >= C) {
+    A = fr_t::zero();
+    M = fr_t::one();
+  }
+  // suffix scan over threads: incoming(t) = F_(t+1) o F_(t+2) o ... (0)
+  store<FrP>(la + 8 * t, A);
+  store<FrP>(lm + 8 * t, M);
+  __syncthreads();
+  fr_t SA = A, SM = M;   // composition of F_t .. F_(t + 2^k - 1)
+  for (int d = 1; d < 256; d <<= 1) {
+    fr_t nA = fr_t::zero(), nM = fr_t::one();
+    if (t + d < 256) {
+      nA = load<FrP>(la + 8 * (t + d));
+      nM = load<FrP>(lm + 8 * (t + d));
+    }
+    __syncthreads();
+    // (S o N)(x) = SA + SM (nA + nM x)
+    SA = add(SA, mul(SM, nA));
+    SM = mul(SM, nM);
+    store<FrP>(la + 8 * t, SA);
+    store<FrP>(lm + 8 * t, SM);
+    __syncthreads();
+  }
+  // value entering thread t from above = (F_(t+1) o ...)(0) = suffix A of t + 1
+  fr_t in = t + 1 < 256 ? load<FrP>(la + 8 * (t + 1)) : fr_t::zero();
+  for (size_t c = c1; c-- > c0;) {
+    store<FrP>(carry + 8 * c, in);
+    in = add(load<FrP>(loc + 8 * c), mul(g, in));
+  }
+}
+
+// Phase 3: recompute each chunk with its true carry-in; write q.
+__global__ void k_kate_final(const uint32_t* __restrict__ a, size_t m, FrArg b, const uint32_t* __restrict__ carry,
+                             uint32_t* __restrict__ q) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t lo = c * KATE_L;
+  if (lo >= m) return;
+  const size_t hi = lo + KATE_L < m ? lo + KATE_L : m;
+  const fr_t bb = arg_fr(b);
+  fr_t v = load<FrP>(carry + 8 * c);
+  for (size_t i = hi; i-- > lo;) {
+    v = add(load<FrP>(a + 8 * (i + 1)), mul(bb, v));
+    store<FrP>(q + 8 * i, v);
+  }
+}
+
+// out[i] = x^i
+__global__ void k_powers(uint32_t* __restrict__ out, size_t n, Pows P) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  store<FrP>(out + 8 * i, pow_index(P, i));
+}
+
+static Pows make_pows(const host::Fr& x) {
+  Pows P;
+  host::Fr t = x;
+  for (int j = 0; j < 32; j++) {
+    P.p[j] = to_arg(t);
+    t = host::fr_mul(t, t);
+  }
+  return P;
+}
+
+}  // namespace mzk
+
+using namespace mzk;
+
+static int vec_ready(mzk_ctx* c) {
+  MZK_REQUIRE(c != nullptr, "null context");
+  MZK_HIP(hipSetDevice(c->device));
+  return MZK_OK;
+}
+
+extern "C" {
+
+int mzk_fr_vec_op(mzk_ctx* c, int op, const void* a, const void* b, void* out, size_t n) {
+  MZK_REQUIRE(c != nullptr, "null context");
+  MZK_REQUIRE(op >= 0 && op <= 2, "mzk_fr_vec_op: op out of range");
+  if (n == 0) return MZK_OK;
+  MZK_REQUIRE(a && b && out, "mzk_fr_vec_op: null data");
+  MZK_TRY(vec_ready(c));
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_LAUNCH(c, "k_vec_binop", k_vec_binop, dim3(grid_for(n, 256)), dim3(256), 0, op, (const uint32_t*)a,
+             (const uint32_t*)b, (uint32_t*)out, n);
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
+}
+
+int mzk_fr_vec_scalar(mzk_ctx* c, int op, const void* a, const uint64_t s[4], void* out, size_t n) {
+  MZK_REQUIRE(c != nullptr, "null context");
+  MZK_REQUIRE(op >= 0 && op <= 2, "mzk_fr_vec_scalar: op out of range");
+  if (n == 0) return MZK_OK;
+  MZK_REQUIRE(a && s && out, "mzk_fr_vec_scalar: null data");
+  MZK_TRY(vec_ready(c));
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_LAUNCH(c, "k_vec_scalar", k_vec_scalar, dim3(grid_for(n, 256)), dim3(256), 0, op, (const uint32_t*)a,
+             to_arg(s), (uint32_t*)out, n);
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
+}
+
+int mzk_fr_batch_invert(mzk_ctx* c, void* a, size_t n) {
+  MZK_REQUIRE(c != nullptr, "null context");
+  if (n == 0) return MZK_OK;
+  MZK_REQUIRE(a != nullptr, "mzk_fr_batch_invert: null data");
+  MZK_TRY(vec_ready(c));
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_TRY(c->scratch[SCR_MSM_D].reserve(n * 32 + 64));
+  const size_t threads = (n + INV_CHUNK - 1) / INV_CHUNK;
+  MZK_LAUNCH(c, "k_batch_invert", k_batch_invert, dim3(grid_for(threads, 64)), dim3(64), 0, (uint32_t*)a, n,
+             (uint32_t*)c->scratch[SCR_MSM_D].ptr);
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
+}
+
+int mzk_fr_prefix_product(mzk_ctx* c, const void* a, void* out, size_t n) {
+  MZK_REQUIRE(c != nullptr, "null context");
+  if (n == 0) return MZK_OK;
+  MZK_REQUIRE(a && out, "mzk_fr_prefix_product: null data");
+  const size_t tile = (size_t)SCAN_T * SCAN_E;
+  const size_t tiles = (n + tile - 1) / tile;
+  MZK_REQUIRE(tiles <= tile, "mzk_fr_prefix_product: n above %zu", tile * tile);
+  MZK_TRY(vec_ready(c));
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_TRY(c->scratch[SCR_MSM_D].reserve(tiles * 32 + 64));
+  uint32_t* totals = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
+  MZK_LAUNCH(c, "k_prod_tiles", k_prod_tiles, dim3((unsigned)tiles), dim3(SCAN_T), 0, (const uint32_t*)a,
+             (uint32_t*)out, n, totals);
+  MZK_LAUNCH(c, "k_prod_totals", k_prod_totals, dim3(1), dim3(SCAN_T), 0, totals, tiles);
+  MZK_LAUNCH(c, "k_prod_apply", k_prod_apply, dim3(grid_for(n, 256)), dim3(256), 0, (uint32_t*)out, n, totals);
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
+}
+
+int mzk_fr_eval(mzk_ctx* c, const void* a, size_t n, const uint64_t x[4], uint64_t out[4]) {
+  MZK_REQUIRE(c != nullptr && x && out, "mzk_fr_eval: null argument");
+  if (n == 0) {
+    memset(out, 0, 32);
+    return MZK_OK;
+  }
+  MZK_REQUIRE(a != nullptr, "mzk_fr_eval: null data");
+  MZK_TRY(vec_ready(c));
+  std::lock_guard<std::mutex> g(c->mu);
+  const size_t threads = (n + EVAL_CHUNK - 1) / EVAL_CHUNK;
+  const unsigned blocks = grid_for(threads, 256);
+  MZK_TRY(c->scratch[SCR_MSM_D].reserve((size_t)blocks * 32 + 64));
+  uint32_t* partials = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
+  host::Fr hx;
+  memcpy(hx.l, x, 32);
+  MZK_LAUNCH(c, "k_eval", k_eval, dim3(blocks), dim3(256), 0, (const uint32_t*)a, n, to_arg(x), make_pows(hx),
+             partials);
+  MZK_HIP(hipGetLastError());
+  std::vector<uint64_t> h(4 * (size_t)blocks);
+  MZK_HIP(hipMemcpyAsync(h.data(), partials, 32 * (size_t)blocks, hipMemcpyDeviceToHost, c->stream));
+  MZK_HIP(hipStreamSynchronize(c->stream));
+  host::Fr acc = host::fr_zero();
+  for (unsigned b = 0; b < blocks; b++) {
+    host::Fr v;
+    memcpy(v.l, &h[4 * b], 32);
+    acc = host::fr_add(acc, v);
+  }
+  memcpy(out, acc.l, 32);
+  return MZK_OK;
+}
+
+int mzk_fr_kate_div(mzk_ctx* c, const void* a, size_t n, const uint64_t b[4], void* q) {
+  MZK_REQUIRE(c != nullptr && b != nullptr, "mzk_fr_kate_div: null argument");
+  MZK_REQUIRE(n >= 1, "mzk_fr_kate_div: empty polynomial");
+  if (n == 1) return MZK_OK;   // the quotient has no coefficients
+  MZK_REQUIRE(a && q && a != q, "mzk_fr_kate_div: null or aliased data");
+  const size_t m = n - 1;
+  const size_t C = (m + KATE_L - 1) / KATE_L;
+  MZK_TRY(vec_ready(c));
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_TRY(c->scratch[SCR_MSM_D].reserve(2 * C * 32 + 64));
+  uint32_t* loc = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
+  uint32_t* carry = loc + 8 * C;
+  host::Fr hb;
+  memcpy(hb.l, b, 32);
+  const host::Fr gL = host::fr_pow(hb, KATE_L);
+  MZK_LAUNCH(c, "k_kate_local", k_kate_local, dim3(grid_for(C, 64)), dim3(64), 0, (const uint32_t*)a, m, to_arg(b),
+             loc);
+  MZK_LAUNCH(c, "k_kate_carries", k_kate_carries, dim3(1), dim3(256), 0, loc, C, to_arg(gL), carry);
+  MZK_LAUNCH(c, "k_kate_final", k_kate_final, dim3(grid_for(C, 64)), dim3(64), 0, (const uint32_t*)a, m, to_arg(b),
+             carry, (uint32_t*)q);
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
+}
+
+int mzk_fr_powers(mzk_ctx* c, const uint64_t x[4], size_t n, void* out) {
+  MZK_REQUIRE(c != nullptr && x != nullptr, "mzk_fr_powers: null argument");
+  if (n == 0) return MZK_OK;
+  MZK_REQUIRE(out != nullptr, "mzk_fr_powers: null data");
+  MZK_TRY(vec_ready(c));
+  std::lock_guard<std::mutex> g(c->mu);
+  host::Fr hx;
+  memcpy(hx.l, x, 32);
+  MZK_LAUNCH(c, "k_powers", k_powers, dim3(grid_for(n, 256)), dim3(256), 0, (uint32_t*)out, n, make_pows(hx));
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
+}
+
+}  // extern "C"

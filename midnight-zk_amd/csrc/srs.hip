@@ -252,9 +252,9 @@ int mzk_srs_create(mzk_ctx* c, const uint64_t* bases, size_t n, mzk_srs** out) {
   return MZK_OK;
 }
 
-int mzk_srs_generate(mzk_ctx* c, const uint64_t tau[4], size_t n, mzk_srs** out) {
-  MZK_REQUIRE(c != nullptr && out != nullptr && tau != nullptr, "mzk_srs_generate: null argument");
-  MZK_REQUIRE(n >= 1 && n < (1ull << 31), "mzk_srs_generate: n out of range");
+// Affine [s_i]G for i < n into a fresh SRS: the scalars are the powers of tau when
+// d_scalars is null, else the n device scalars given (Montgomery).
+static int srs_fixed_base(mzk_ctx* c, const uint64_t* tau, const void* d_scalars, size_t n, mzk_srs** out) {
   std::lock_guard<std::mutex> g(c->mu);
   MZK_HIP(hipSetDevice(c->device));
   mzk_srs* s = nullptr;
@@ -272,15 +272,19 @@ int mzk_srs_generate(mzk_ctx* c, const uint64_t tau[4], size_t n, mzk_srs** out)
       set_error("srs: comb upload failed");
       break;
     }
-    FrPows p;
-    host::Fr t;
-    memcpy(t.l, tau, 32);
-    for (int j = 0; j < 32; j++) {
-      memcpy(p.v[j], t.l, 32);
-      t = host::fr_mul(t, t);
+    const uint32_t* d_sc = (const uint32_t*)d_scalars;
+    if (d_sc == nullptr) {
+      FrPows p;
+      host::Fr t;
+      memcpy(t.l, tau, 32);
+      for (int j = 0; j < 32; j++) {
+        memcpy(p.v[j], t.l, 32);
+        t = host::fr_mul(t, t);
+      }
+      MZK_LAUNCH(c, "k_tau_powers", k_tau_powers, dim3(grid_for(n, 256)), dim3(256), 0, d_pows, n, p);
+      d_sc = d_pows;
     }
-    MZK_LAUNCH(c, "k_tau_powers", k_tau_powers, dim3(grid_for(n, 256)), dim3(256), 0, d_pows, n, p);
-    MZK_LAUNCH(c, "k_fixed_base", k_fixed_base, dim3(grid_for(n, 64)), dim3(64), 0, d_pows, n, d_comb, d_xyzz);
+    MZK_LAUNCH(c, "k_fixed_base", k_fixed_base, dim3(grid_for(n, 64)), dim3(64), 0, d_sc, n, d_comb, d_xyzz);
     if (hipGetLastError() != hipSuccess) {
       rc = MZK_ERR_HIP;
       set_error("srs: kernel launch failed");
@@ -300,6 +304,18 @@ int mzk_srs_generate(mzk_ctx* c, const uint64_t tau[4], size_t n, mzk_srs** out)
   }
   *out = s;
   return MZK_OK;
+}
+
+int mzk_srs_generate(mzk_ctx* c, const uint64_t tau[4], size_t n, mzk_srs** out) {
+  MZK_REQUIRE(c != nullptr && out != nullptr && tau != nullptr, "mzk_srs_generate: null argument");
+  MZK_REQUIRE(n >= 1 && n < (1ull << 31), "mzk_srs_generate: n out of range");
+  return srs_fixed_base(c, tau, nullptr, n, out);
+}
+
+int mzk_srs_from_scalars_dev(mzk_ctx* c, const void* d_scalars, size_t n, mzk_srs** out) {
+  MZK_REQUIRE(c != nullptr && out != nullptr && d_scalars != nullptr, "mzk_srs_from_scalars_dev: null argument");
+  MZK_REQUIRE(n >= 1 && n < (1ull << 31), "mzk_srs_from_scalars_dev: n out of range");
+  return srs_fixed_base(c, nullptr, d_scalars, n, out);
 }
 
 size_t mzk_srs_len(const mzk_srs* s) { return s ? s->n : 0; }

@@ -106,6 +106,14 @@ SIGNATURES = {
     "mzk_prof_enable": ([_P, _I], _I),
     "mzk_prof_read": ([_P, ctypes.c_char_p, _S], _I),
     "mzk_msm_last_plan": ([_P, _P], _I),
+    "mzk_srs_from_scalars_dev": ([_P, _P, _S, ctypes.POINTER(_P)], _I),
+    "mzk_fr_vec_op": ([_P, _I, _P, _P, _P, _S], _I),
+    "mzk_fr_vec_scalar": ([_P, _I, _P, _P, _P, _S], _I),
+    "mzk_fr_batch_invert": ([_P, _P, _S], _I),
+    "mzk_fr_prefix_product": ([_P, _P, _P, _S], _I),
+    "mzk_fr_eval": ([_P, _P, _S, _P, _P], _I),
+    "mzk_fr_kate_div": ([_P, _P, _S, _P, _P], _I),
+    "mzk_fr_powers": ([_P, _P, _S, _P], _I),
 }
 
 

@@ -941,15 +941,19 @@ static void fixed_base_mul(g1j_t *out, const fr_t *s_mont) {
   for (int w = 0; w < 32; w++)
     if (b[w]) g1j_add_affine(out, out, &g_comb[w * 256 + b[w]]);
 }
-typedef struct { const fr_t *tau; size_t n; g1a_t *out; } srs_ctx_t;
+typedef struct { const fr_t *tau; size_t n; g1a_t *out; const fr_t *scalars; } srs_ctx_t;
 static void srs_chunk(void *v, int c) {
   srs_ctx_t *s = (srs_ctx_t *)v;
   size_t lo = (size_t)c * 4096, hi = lo + 4096 < s->n ? lo + 4096 : s->n;
   size_t m = hi - lo;
   fr_t p;
-  fr_pow_u64(&p, s->tau, lo);
+  if (!s->scalars) fr_pow_u64(&p, s->tau, lo);
   g1j_t *j = (g1j_t *)malloc(sizeof(g1j_t) * m);
-  for (size_t i = 0; i < m; i++) { fixed_base_mul(&j[i], &p); fr_mul(&p, &p, s->tau); }
+  for (size_t i = 0; i < m; i++) {
+    if (s->scalars) { fixed_base_mul(&j[i], &s->scalars[lo + i]); continue; }
+    fixed_base_mul(&j[i], &p);
+    fr_mul(&p, &p, s->tau);
+  }
   /* batch normalisation (Montgomery's trick) */
   fp_t *pre = (fp_t *)malloc(sizeof(fp_t) * m);
   fp_t acc;
@@ -972,7 +976,7 @@ static void srs_chunk(void *v, int c) {
 /* g_i = [tau^i] G for i < n, affine Montgomery (blst_p1_affine layout) */
 void cpu_ref_srs_bases(const u64 *tau_mont, size_t n, u64 *out) {
   comb_build();
-  srs_ctx_t ctx = {(const fr_t *)tau_mont, n, (g1a_t *)out};
+  srs_ctx_t ctx = {(const fr_t *)tau_mont, n, (g1a_t *)out, NULL};
   parallel_for((int)((n + 4095) / 4096), g_threads, srs_chunk, &ctx);
 }
 
@@ -989,4 +993,95 @@ void cpu_ref_poly_eval(const u64 *coeffs, size_t n, const u64 *x, u64 *out) {
 void cpu_ref_g1_mul_generator(const u64 *s_mont, u64 *out_jac) {
   comb_build();
   fixed_base_mul((g1j_t *)out_jac, (const fr_t *)s_mont);
+}
+
+/* ------------------------------------------------------------------ scalar vectors
+ * Restatements of the prover's element-wise Polynomial arithmetic (proofs/src/poly/mod.rs),
+ * ff BatchInvert (used at permutation/prover.rs:148), the grand-product recurrence
+ * (permutation/prover.rs:190-197), kate_division (utils/arithmetic.rs:106-125) and
+ * powers (kzg/mod.rs:151-167). */
+typedef struct { int op; const fr_t *a, *b; fr_t *out; size_t n; fr_t s; } vec_ctx_t;
+#define VEC_CHUNK 16384
+static void vec_binop_chunk(void *v, int c) {
+  vec_ctx_t *x = (vec_ctx_t *)v;
+  size_t lo = (size_t)c * VEC_CHUNK, hi = lo + VEC_CHUNK < x->n ? lo + VEC_CHUNK : x->n;
+  for (size_t i = lo; i < hi; i++) {
+    fr_t r;
+    if (x->op == 0) fr_add(&r, &x->a[i], &x->b[i]);
+    else if (x->op == 1) fr_sub(&r, &x->a[i], &x->b[i]);
+    else fr_mul(&r, &x->a[i], &x->b[i]);
+    x->out[i] = r;
+  }
+}
+/* op 0: a + b, 1: a - b, 2: a * b */
+void cpu_ref_fr_vec_op(int op, const u64 *a, const u64 *b, u64 *out, size_t n) {
+  vec_ctx_t x = {op, (const fr_t *)a, (const fr_t *)b, (fr_t *)out, n, {{0, 0, 0, 0}}};
+  parallel_for((int)((n + VEC_CHUNK - 1) / VEC_CHUNK), g_threads, vec_binop_chunk, &x);
+}
+static void vec_scalar_chunk(void *v, int c) {
+  vec_ctx_t *x = (vec_ctx_t *)v;
+  size_t lo = (size_t)c * VEC_CHUNK, hi = lo + VEC_CHUNK < x->n ? lo + VEC_CHUNK : x->n;
+  for (size_t i = lo; i < hi; i++) {
+    fr_t r;
+    if (x->op == 0) fr_mul(&r, &x->a[i], &x->s);
+    else if (x->op == 1) fr_add(&r, &x->a[i], &x->s);
+    else { fr_mul(&r, &x->a[i], &x->s); fr_add(&r, &x->out[i], &r); }
+    x->out[i] = r;
+  }
+}
+/* op 0: out = a * s, 1: out = a + s, 2: out = out + a * s */
+void cpu_ref_fr_vec_scalar(int op, const u64 *a, const u64 *s, u64 *out, size_t n) {
+  vec_ctx_t x = {op, (const fr_t *)a, NULL, (fr_t *)out, n, *(const fr_t *)s};
+  parallel_for((int)((n + VEC_CHUNK - 1) / VEC_CHUNK), g_threads, vec_scalar_chunk, &x);
+}
+/* Montgomery's trick over the whole vector, zeros skipped (ff::BatchInvert) */
+void cpu_ref_fr_batch_invert(u64 *a_raw, size_t n) {
+  fr_t *a = (fr_t *)a_raw;
+  fr_t *pre = (fr_t *)malloc(sizeof(fr_t) * (n ? n : 1));
+  fr_t acc;
+  memcpy(acc.l, FR_ONE, 32);
+  for (size_t i = 0; i < n; i++) { pre[i] = acc; if (!fr_is_zero(&a[i])) fr_mul(&acc, &acc, &a[i]); }
+  fr_inv(&acc, &acc);
+  for (size_t i = n; i-- > 0;) {
+    if (fr_is_zero(&a[i])) continue;
+    fr_t t;
+    fr_mul(&t, &acc, &pre[i]);
+    fr_mul(&acc, &acc, &a[i]);
+    a[i] = t;
+  }
+  free(pre);
+}
+/* out_i = a_0 * ... * a_i */
+void cpu_ref_fr_prefix_product(const u64 *a_raw, u64 *out_raw, size_t n) {
+  const fr_t *a = (const fr_t *)a_raw;
+  fr_t *out = (fr_t *)out_raw;
+  fr_t acc;
+  memcpy(acc.l, FR_ONE, 32);
+  for (size_t i = 0; i < n; i++) { fr_mul(&acc, &acc, &a[i]); out[i] = acc; }
+}
+/* kate_division: b <- -b; walking down from the top, q = r - tmp, tmp = q * b */
+void cpu_ref_fr_kate_div(const u64 *a_raw, size_t n, const u64 *b_raw, u64 *q_raw) {
+  const fr_t *a = (const fr_t *)a_raw;
+  fr_t *q = (fr_t *)q_raw;
+  fr_t zero = {{0, 0, 0, 0}}, b, tmp = zero;
+  fr_sub(&b, &zero, (const fr_t *)b_raw);
+  for (size_t i = n - 1; i-- > 0;) {
+    fr_t lead;
+    fr_sub(&lead, &a[i + 1], &tmp);
+    q[i] = lead;
+    fr_mul(&tmp, &lead, &b);
+  }
+}
+/* out_i = x^i */
+void cpu_ref_fr_powers(const u64 *x, size_t n, u64 *out_raw) {
+  fr_t *out = (fr_t *)out_raw;
+  fr_t acc;
+  memcpy(acc.l, FR_ONE, 32);
+  for (size_t i = 0; i < n; i++) { out[i] = acc; fr_mul(&acc, &acc, (const fr_t *)x); }
+}
+/* [s_i]G for n scalars (the Lagrange bases of unsafe_setup, params.rs:174-186), affine out */
+void cpu_ref_g1_mul_generator_many(const u64 *s_mont, size_t n, u64 *out_aff) {
+  comb_build();
+  srs_ctx_t ctx = {NULL, n, (g1a_t *)out_aff, (const fr_t *)s_mont};
+  parallel_for((int)((n + 4095) / 4096), g_threads, srs_chunk, &ctx);
 }

@@ -62,6 +62,13 @@ def lib():
             "cpu_ref_srs_bases": ([P, S, P], None),
             "cpu_ref_poly_eval": ([P, S, P, P], None),
             "cpu_ref_g1_mul_generator": ([P, P], None),
+            "cpu_ref_fr_vec_op": ([ctypes.c_int, P, P, P, S], None),
+            "cpu_ref_fr_vec_scalar": ([ctypes.c_int, P, P, P, S], None),
+            "cpu_ref_fr_batch_invert": ([P, S], None),
+            "cpu_ref_fr_prefix_product": ([P, P, S], None),
+            "cpu_ref_fr_kate_div": ([P, S, P, P], None),
+            "cpu_ref_fr_powers": ([P, S, P], None),
+            "cpu_ref_g1_mul_generator_many": ([P, S, P], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -156,6 +163,60 @@ def divide_by_vanishing_poly(a: np.ndarray, k: int, ext_k: int) -> np.ndarray:
     b = np.ascontiguousarray(a, dtype=np.uint64).copy()
     lib().cpu_ref_divide_by_vanishing_poly(_p(b), k, ext_k)
     return b
+
+
+def _c(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def fr_vec_op(op: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """op 0: a + b, 1: a - b, 2: a * b (element-wise, Montgomery rows)."""
+    a, b = _c(a), _c(b)
+    out = np.empty_like(a)
+    lib().cpu_ref_fr_vec_op(op, _p(a), _p(b), _p(out), a.shape[0])
+    return out
+
+
+def fr_vec_scalar(op: int, a: np.ndarray, s_mont: np.ndarray, out: np.ndarray | None = None) -> np.ndarray:
+    """op 0: a * s, 1: a + s, 2: out + a * s (out updated in place)."""
+    a = _c(a)
+    if out is None:
+        out = np.empty_like(a)
+    lib().cpu_ref_fr_vec_scalar(op, _p(a), _p(_c(s_mont)), _p(out), a.shape[0])
+    return out
+
+
+def fr_batch_invert(a: np.ndarray) -> np.ndarray:
+    b = _c(a).copy()
+    lib().cpu_ref_fr_batch_invert(_p(b), b.shape[0])
+    return b
+
+
+def fr_prefix_product(a: np.ndarray) -> np.ndarray:
+    a = _c(a)
+    out = np.empty_like(a)
+    lib().cpu_ref_fr_prefix_product(_p(a), _p(out), a.shape[0])
+    return out
+
+
+def fr_kate_div(a: np.ndarray, b_mont: np.ndarray) -> np.ndarray:
+    a = _c(a)
+    q = np.zeros((max(0, a.shape[0] - 1), 4), dtype=np.uint64)
+    lib().cpu_ref_fr_kate_div(_p(a), a.shape[0], _p(_c(b_mont)), _p(q))
+    return q
+
+
+def fr_powers(x_mont: np.ndarray, n: int) -> np.ndarray:
+    out = np.zeros((n, 4), dtype=np.uint64)
+    lib().cpu_ref_fr_powers(_p(_c(x_mont)), n, _p(out))
+    return out
+
+
+def g1_mul_generator_many(s_mont: np.ndarray) -> np.ndarray:
+    s = _c(s_mont)
+    out = np.zeros((s.shape[0], 12), dtype=np.uint64)
+    lib().cpu_ref_g1_mul_generator_many(_p(s), s.shape[0], _p(out))
+    return out
 
 
 # ---- conversions between numpy limb arrays and Python ints (canonical)

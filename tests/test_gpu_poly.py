@@ -1,0 +1,143 @@
+"""Scalar-vector kernels of the prover (vector arithmetic, batch inversion, prefix
+products, Horner evaluation, Kate division, powers, bases from scalars) on the device
+against the oracle's C restatement, bit-exact, at sizes that cross every tile and
+chunk boundary of the kernels (GPU)."""
+import numpy as np
+import pytest
+
+from oracle import bls12_381 as O
+from oracle import cpu_ref as C
+from tests.helpers import from_dev, mzk, pkg, to_dev
+
+pytestmark = pytest.mark.gpu
+plonk = pkg.plonk
+
+
+@pytest.fixture(scope="module")
+def be():
+    C.set_threads(16)
+    b = plonk.GpuBackend()
+    yield b
+    b.ctx.close()
+
+
+def host(be, v):
+    be.sync()
+    return from_dev(v).reshape(-1, 4)
+
+
+SIZES = [1, 2, 31, 32, 33, 63, 64, 65, 4095, 4096, 4097, 65537, (1 << 20) + 3]
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_vec_ops(be, n):
+    a = C.fr_from_seed(1, n, n)
+    b = C.fr_from_seed(2, n, n)
+    da, db = to_dev(a), to_dev(b)
+    for op, f in ((0, be.add), (1, be.sub), (2, be.mul)):
+        assert np.array_equal(host(be, f(da, db)), C.fr_vec_op(op, a, b)), op
+    s = 0x1234567890ABCDEF1234567890ABCDEF % plonk.R_MOD
+    sl = plonk.mont_limbs(s)
+    assert np.array_equal(host(be, be.scale(da, s)), C.fr_vec_scalar(0, a, sl))
+    assert np.array_equal(host(be, be.add_scalar(da, s)), C.fr_vec_scalar(1, a, sl))
+    acc = C.fr_from_seed(3, n, n)
+    want = C.fr_vec_scalar(2, a, sl, acc.copy())
+    assert np.array_equal(host(be, be.axpy(to_dev(acc), da, s)), want)
+
+
+def test_vec_ops_small_values_by_integers(be):
+    a = [0, 1, 2, plonk.R_MOD - 1, 12345, plonk.R_MOD - 2]
+    b = [0, plonk.R_MOD - 1, 2, plonk.R_MOD - 1, 54321, 3]
+    da, db = to_dev(C.ints_to_fr_rows(a)), to_dev(C.ints_to_fr_rows(b))
+    R = plonk.R_MOD
+    assert C.fr_rows_to_ints(host(be, be.add(da, db))) == [(x + y) % R for x, y in zip(a, b)]
+    assert C.fr_rows_to_ints(host(be, be.sub(da, db))) == [(x - y) % R for x, y in zip(a, b)]
+    assert C.fr_rows_to_ints(host(be, be.mul(da, db))) == [x * y % R for x, y in zip(a, b)]
+
+
+@pytest.mark.parametrize("n", [1, 5, 32, 33, 1000, 65536 + 7, (1 << 20) + 1])
+def test_batch_invert(be, n):
+    a = C.fr_from_seed(4, n, n)
+    a[::7] = 0                       # zeros stay zero (BatchInvert skips them)
+    got = host(be, be.batch_invert(to_dev(a)))
+    assert np.array_equal(got, C.fr_batch_invert(a))
+    if n <= 1000:
+        ints = C.fr_rows_to_ints(a)
+        assert C.fr_rows_to_ints(got) == [pow(v, -1, O.R_MOD) if v else 0 for v in ints]
+
+
+@pytest.mark.parametrize("n", [1, 2, 255, 256, 4095, 4096, 4097, 8192 * 3 + 5, (1 << 18) + 1, (1 << 20)])
+def test_prefix_product(be, n):
+    a = C.fr_from_seed(5, n, n)
+    got = host(be, be.prefix_product(to_dev(a)))
+    assert np.array_equal(got, C.fr_prefix_product(a))
+
+
+def test_prefix_product_integers(be):
+    vals = [2, 3, 5, 7, 11, 0, 13]
+    got = C.fr_rows_to_ints(host(be, be.prefix_product(to_dev(C.ints_to_fr_rows(vals)))))
+    want, acc = [], 1
+    for v in vals:
+        acc = acc * v % O.R_MOD
+        want.append(acc)
+    assert got == want
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 16384, 16385, (1 << 18) + 9, 1 << 20])
+def test_eval(be, n):
+    a = C.fr_from_seed(6, n, n)
+    x = 0xDEADBEEF0123456789 % O.R_MOD
+    got = be.eval_many([(to_dev(a), x)])[0]
+    want = plonk.from_mont_limbs(C.poly_eval(a, plonk.mont_limbs(x)))
+    assert got == want
+    if n <= 65:
+        assert got == O.poly_eval(C.fr_rows_to_ints(a), x)
+
+
+def test_eval_empty_is_zero(be):
+    out = np.ones(4, dtype=np.uint64)
+    L = mzk.load_library()
+    rc = L.mzk_fr_eval(be.ctx.h, None, 0, plonk.mont_limbs(3).ctypes.data, out.ctypes.data)
+    assert rc == 0 and not out.any()
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 64, 65, 66, 129, 64 * 256 + 1, 64 * 256 * 3 + 17, (1 << 20) + 1])
+def test_kate_division(be, n):
+    a = C.fr_from_seed(7, n, n)
+    b = 0xABCDEF0123456789ABCDEF % O.R_MOD
+    got = host(be, be.kate_div(to_dev(a), b)) if n > 1 else np.zeros((0, 4), dtype=np.uint64)
+    want = C.fr_kate_div(a, plonk.mont_limbs(b))
+    assert got.shape == want.shape and np.array_equal(got, want)
+    if n > 1:
+        # q(X) (X - b) + a(b) = a(X), checked at a random point
+        z = 0x5A5A5A5A12345 % O.R_MOD
+        qz = be.eval_many([(to_dev(got), z)])[0]
+        ab = plonk.from_mont_limbs(C.poly_eval(a, plonk.mont_limbs(b)))
+        az = plonk.from_mont_limbs(C.poly_eval(a, plonk.mont_limbs(z)))
+        assert (qz * (z - b) + ab) % O.R_MOD == az
+
+
+@pytest.mark.parametrize("n", [1, 7, 4096, (1 << 20) + 5])
+def test_powers(be, n):
+    x = plonk.omega_for(20)
+    got = host(be, be.powers(x, n))
+    assert np.array_equal(got, C.fr_powers(plonk.mont_limbs(x), n))
+
+
+def test_bases_from_scalars(be):
+    n = 3000
+    s = C.fr_from_seed(8, 1, n)
+    srs = be.srs_from_scalars(to_dev(s))
+    got = srs.download()
+    assert np.array_equal(got, C.g1_mul_generator_many(s))
+    srs.close()
+
+
+def test_bad_arguments_are_errors(be):
+    L = mzk.load_library()
+    assert L.mzk_fr_vec_op(None, 0, None, None, None, 4) == 1
+    assert L.mzk_fr_vec_op(be.ctx.h, 7, None, None, None, 4) == 1
+    assert L.mzk_fr_kate_div(be.ctx.h, None, 0, plonk.mont_limbs(1).ctypes.data, None) == 1
+    d = to_dev(C.fr_from_seed(1, 1, 8))
+    assert L.mzk_fr_kate_div(be.ctx.h, d.data_ptr(), 8, plonk.mont_limbs(1).ctypes.data, d.data_ptr()) == 1
+    assert L.mzk_fr_prefix_product(be.ctx.h, d.data_ptr(), d.data_ptr(), (1 << 24) + 1) == 1
