@@ -5,11 +5,15 @@ A step is one KZG commitment: a G1 MSM of n = 2^20 seeded uniform Fr scalars
 against resident SRS bases g_i = [tau^i] G1, with scalars and bases already in
 HBM when the timed region starts.  The line printed reports whole-job points/s,
 the roofline of the dominant kernel (bucket accumulation) from HIP-event times
-measured in this run, the scalar-field NTT at 2^22 beside it, and the oracle's C
+measured in this run, the scalar-field NTT at 2^22 beside it, 96 uneven commitments
+and NTTs batched against per-call invocations, full k = 18 proofs of the synthetic
+PLONK circuit ("prove": median ms per proof and its phase split), and the oracle's C
 restatement of the reference CPU msm_best timed on this host on the same inputs
-(which also checks the GPU result bit for bit).
+(which also checks the GPU result bit for bit), with the same prover on the CPU at
+k = 16 beside the GPU's and the two proofs compared byte for byte.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--log-n 20] [--no-cpu-baseline]
+                  [--cpu-prove-k 16]
 """
 from __future__ import annotations
 
@@ -104,6 +108,66 @@ def bench_batched(mzk, ctx, srs, n_srs: int, njobs: int = 96, seed: int = 96) ->
     }
 
 
+def bench_prove(pkg, ctx, k: int = 18, reps: int = 5) -> dict:
+    """k-row PLONK/KZG proof of the synthetic circuit on the GPU backend: setup and
+    keygen once, witness synthesis, then `reps` proofs (median wall time); the phase
+    split comes from one more proof with a device wait at each phase end."""
+    import statistics
+    plonk = pkg.plonk
+    be = plonk.GpuBackend(ctx)
+    t0 = time.perf_counter()
+    params = plonk.Params(be, k, 0x5EED0000 + k)
+    be.sync()
+    t1 = time.perf_counter()
+    circ = plonk.SyntheticCircuit(k, seed=k)
+    pk = plonk.keygen(be, params, circ)
+    be.sync()
+    t2 = time.perf_counter()
+    wit = circ.witness(be)
+    be.sync()
+    t3 = time.perf_counter()
+    proof = plonk.create_proof(be, params, pk, wit, rng_seed=1)
+    times = []
+    for _ in range(reps):
+        be.sync()
+        a = time.perf_counter()
+        again = plonk.create_proof(be, params, pk, wit, rng_seed=1)
+        times.append(time.perf_counter() - a)
+        assert again == proof
+    phases = {}
+    plonk.create_proof(be, params, pk, wit, rng_seed=1, timings=phases)
+    prev, split = 0.0, {}
+    for name, t in phases.items():
+        split[name] = round((t - prev) * 1e3, 3)
+        prev = t
+    return {
+        "k": k, "prove_ms": round(statistics.median(times) * 1e3, 3),
+        "prove_ms_min": round(min(times) * 1e3, 3), "proofs": reps,
+        "proof_bytes": len(proof), "msms_per_proof": 13,
+        "phase_ms": split, "setup_ms": round((t1 - t0) * 1e3, 1), "keygen_ms": round((t2 - t1) * 1e3, 1),
+        "witness_ms": round((t3 - t2) * 1e3, 2),
+        "_proof": proof, "_params": params, "_circ": circ,
+    }
+
+
+def cpu_prove_sample(pkg, ctx, k: int, cores: int) -> dict:
+    """The same prover on the oracle's C restatement (the CPU prover) at size k, beside
+    the GPU prover at the same k; the two proofs must be byte-identical."""
+    from tests.cpu_backend import CpuBackend
+    plonk = pkg.plonk
+    g = bench_prove(pkg, ctx, k, reps=3)
+    cb = CpuBackend(cores)
+    params = plonk.Params(cb, k, 0x5EED0000 + k)
+    circ = plonk.SyntheticCircuit(k, seed=k)
+    pk = plonk.keygen(cb, params, circ)
+    wit = circ.witness(cb)
+    t0 = time.perf_counter()
+    proof = plonk.create_proof(cb, params, pk, wit, rng_seed=1)
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    return {"k": k, "cpu_prove_ms": round(cpu_ms, 1), "gpu_prove_ms": g["prove_ms"], "cores": cores,
+            "identical_proof_bytes": proof == g["_proof"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +176,7 @@ def main() -> int:
     ap.add_argument("--log-n", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-prove-k", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the MSM/NTT path has no multi-GPU form in this build "
@@ -204,6 +269,13 @@ def main() -> int:
     except Exception as e:
         batched = {"error": str(e)}
 
+    # ---- full proofs at k = 18 on the GPU backend
+    try:
+        prove = bench_prove(pkg, ctx, 18)
+        prove = {k: v for k, v in prove.items() if not k.startswith("_")}
+    except Exception as e:
+        prove = {"error": repr(e)}
+
     # ---- CPU baseline: oracle's C restatement of msm_best, same inputs; also the bit-exact check
     cpu = None
     bit_exact = None
@@ -221,6 +293,10 @@ def main() -> int:
         cpu = {"value": round(n / cpu_s, 1), "unit": "points/s", "cores": cores, "kind": "port",
                "sample": f"one msm_best call (C restatement of curves/src/msm.rs:460) at n=2^{args.log_n} "
                          f"on the same bases and scalars, {cpu_s:.2f} s"}
+        try:
+            cpu["prove_sample"] = cpu_prove_sample(pkg, ctx, args.cpu_prove_k, cores)
+        except Exception as e:
+            cpu["prove_sample"] = {"error": repr(e)}
 
     line = {
         "metric": METRIC,
@@ -257,6 +333,7 @@ def main() -> int:
         "bit_exact_vs_cpu": bit_exact,
         "ntt": ntt,
         "batched": batched,
+        "prove": prove,
         "setup": {"srs_generate_ms": round(srs_ms, 2)},
     }
     print(json.dumps(line), flush=True)
