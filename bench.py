@@ -150,6 +150,51 @@ def bench_prove(pkg, ctx, k: int = 18, reps: int = 5) -> dict:
     }
 
 
+def bench_serve(pkg, ctx, nreq: int, rates, ks=(12, 14, 16, 18, 20), seed: int = 5,
+                workers: int = 4, slots: int = 2) -> dict:
+    """Poisson arrivals of proof requests (k uniform over ks, per-request witness seeds)
+    at fixed offered loads, each trace served twice: by the server (`workers` contexts,
+    hence streams, with `slots` batched proofs each) and one proof at a time (one
+    context, one slot).  A sample of served proofs is compared with the same request
+    proven alone."""
+    serve = pkg.serve
+    plonk = pkg.plonk
+    bes = [plonk.GpuBackend(ctx)] + [plonk.GpuBackend() for _ in range(workers - 1)]
+    t0 = time.perf_counter()
+    server = serve.ProofServer(bes, ks=ks, hbm_budget=64 << 30, slots=slots)
+    setup_s = time.perf_counter() - t0
+    for be in bes:                      # warm every size on every context (plans, allocator)
+        for k in ks:
+            server.prove_alone(serve.ProofRequest(rid=-1, k=k, witness_seed=1), be)
+        be.sync()
+    per_k = {}
+    for k in ks:
+        t1 = time.perf_counter()
+        server.prove_alone(serve.ProofRequest(rid=-1, k=k, witness_seed=2))
+        bes[0].sync()
+        per_k[k] = round((time.perf_counter() - t1) * 1e3, 2)
+    out = {"requests_per_trace": nreq, "ks": list(ks), "workers": workers, "slots_per_worker": slots,
+           "alone_ms_by_k": per_k, "setup_s": round(setup_s, 2), "loads": []}
+    same = True
+    for rate in rates:
+        trace = serve.poisson_trace(nreq, rate, ks, seed)
+        res_b, st_b = server.run(trace, workers=workers, slots=slots)
+        res_s, st_s = server.run(trace, workers=1, slots=1)
+        out["loads"].append({
+            "offered_per_s": rate,
+            "served": {**serve.latency_stats(res_b), "rounds": st_b["rounds"], "max_inflight": st_b["max_inflight"]},
+            "one_at_a_time": serve.latency_stats(res_s),
+        })
+        byid = {r.rid: r for r in res_s}
+        same = same and all(r.proof == byid[r.rid].proof for r in res_b)
+    sample = serve.poisson_trace(nreq, rates[0], ks, seed)[::max(1, nreq // 6)]
+    out["served_equals_one_at_a_time"] = same
+    out["served_equals_alone_sample"] = all(
+        byid_r.proof == server.prove_alone(byid_r_req) for byid_r_req, byid_r in
+        ((r, next(x for x in res_b if x.rid == r.rid)) for r in sample))
+    return out
+
+
 def cpu_prove_sample(pkg, ctx, k: int, cores: int) -> dict:
     """The same prover on the oracle's C restatement (the CPU prover) at size k, beside
     the GPU prover at the same k; the two proofs must be byte-identical."""
@@ -177,6 +222,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-prove-k", type=int, default=16)
+    ap.add_argument("--serve-requests", type=int, default=120)
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[20.0, 40.0])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the MSM/NTT path has no multi-GPU form in this build "
@@ -276,6 +323,12 @@ def main() -> int:
     except Exception as e:
         prove = {"error": repr(e)}
 
+    # ---- serving: Poisson arrivals of mixed-size proofs, continuous batching vs one at a time
+    try:
+        served = bench_serve(pkg, ctx, args.serve_requests, args.serve_rates)
+    except Exception as e:
+        served = {"error": repr(e)}
+
     # ---- CPU baseline: oracle's C restatement of msm_best, same inputs; also the bit-exact check
     cpu = None
     bit_exact = None
@@ -334,6 +387,7 @@ def main() -> int:
         "ntt": ntt,
         "batched": batched,
         "prove": prove,
+        "serve": served,
         "setup": {"srs_generate_ms": round(srs_ms, 2)},
     }
     print(json.dumps(line), flush=True)

@@ -86,7 +86,9 @@ const void *mzk_srs_bases_dev(const mzk_srs *srs);
 int mzk_srs_download(mzk_srs *srs, size_t offset, size_t count, uint64_t *out);
 void mzk_srs_destroy(mzk_srs *srs);
 /* KZG commit against resident bases (kzg/mod.rs:82-108 commit_many -> msm_specific):
- * sum_i coeffs[i] * g[offset + i], coeffs device-resident. */
+ * sum_i coeffs[i] * g[offset + i], coeffs device-resident.  The bases are read-only after
+ * creation: any context on the same device may commit against them (the work runs on
+ * ctx's stream with ctx's scratch), so several contexts can share one SRS. */
 int mzk_msm_srs_dev(mzk_ctx *ctx, const mzk_srs *srs, size_t offset, const void *d_coeffs,
                     size_t n, uint64_t out[18]);
 
@@ -170,6 +172,11 @@ int mzk_fr_batch_invert(mzk_ctx *ctx, void *d_a, size_t n);
 int mzk_fr_prefix_product(mzk_ctx *ctx, const void *d_a, void *d_out, size_t n);
 /* sum_i a_i x^i to host memory (eval_polynomial, utils/arithmetic.rs:59-87). */
 int mzk_fr_eval(mzk_ctx *ctx, const void *d_a, size_t n, const uint64_t x[4], uint64_t out[4]);
+/* njobs evaluations in one launch and one wait: outs[4j..4j+4) = sum_i polys[j][i] x_j^i,
+ * x_j = xs[4j..4j+4) (the batched per-query evaluations of prover.rs:820-843 and
+ * kzg/mod.rs:259-260).  Each job may have its own length and point. */
+int mzk_fr_eval_batch(mzk_ctx *ctx, const void *const *d_polys, const size_t *ns, const uint64_t *xs,
+                      size_t njobs, uint64_t *outs);
 /* Quotient of a(X) - a(b) by (X - b): n - 1 coefficients into d_q (kate_division,
  * utils/arithmetic.rs:106-125).  n >= 1; d_q must not alias d_a. */
 int mzk_fr_kate_div(mzk_ctx *ctx, const void *d_a, size_t n, const uint64_t b[4], void *d_q);

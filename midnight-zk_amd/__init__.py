@@ -6,5 +6,6 @@ the reference's MSM / NTT interface over the C ABI in libmzk_gpu.so.
 """
 from . import mzk  # noqa: F401
 from . import plonk  # noqa: F401
+from . import serve  # noqa: F401
 from .mzk import (Context, EvaluationDomain, MzkError, ResidentSrs,  # noqa: F401
                   load_library)

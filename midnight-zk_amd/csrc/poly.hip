@@ -197,6 +197,51 @@ __global__ void __launch_bounds__(256) k_eval(const uint32_t* __restrict__ a, si
   if (threadIdx.x == 0) store<FrP>(partials + 8 * blockIdx.x, acc);
 }
 
+// Many evaluations in one launch: block b belongs to the job whose block range holds it.
+struct EvalJob {
+  const uint32_t* a;
+  uint64_t n;
+  uint32_t first_block;
+  uint32_t pad;
+};
+
+__global__ void __launch_bounds__(256) k_eval_batch(const EvalJob* __restrict__ jobs, int njobs,
+                                                    const uint32_t* __restrict__ xpows,
+                                                    uint32_t* __restrict__ partials) {
+  __shared__ uint32_t lds[8 * 256];
+  int lo_j = 0, hi_j = njobs - 1;   // last job with first_block <= blockIdx.x
+  while (lo_j < hi_j) {
+    const int mid = (lo_j + hi_j + 1) >> 1;
+    if (jobs[mid].first_block <= blockIdx.x) lo_j = mid;
+    else hi_j = mid - 1;
+  }
+  const EvalJob J = jobs[lo_j];
+  const uint32_t* P = xpows + (size_t)lo_j * 32 * 8;   // x^(2^j), j < 32
+  const size_t t = (size_t)(blockIdx.x - J.first_block) * blockDim.x + threadIdx.x;
+  const size_t lo = t * EVAL_CHUNK;
+  fr_t acc = fr_t::zero();
+  if (lo < J.n) {
+    const size_t hi = lo + EVAL_CHUNK < J.n ? lo + EVAL_CHUNK : J.n;
+    const fr_t xx = load<FrP>(P);
+    for (size_t i = hi; i-- > lo;) acc = add(mul(acc, xx), load<FrP>(J.a + 8 * i));
+    fr_t s = fr_t::one();
+    uint64_t e = lo;
+    for (int j = 0; e; j++, e >>= 1)
+      if (e & 1) s = mul(s, load<FrP>(P + 8 * j));
+    acc = mul(acc, s);
+  }
+  store<FrP>(lds + 8 * threadIdx.x, acc);
+  __syncthreads();
+  for (int d = 128; d >= 1; d >>= 1) {
+    if ((int)threadIdx.x < d) {
+      acc = add(acc, load<FrP>(lds + 8 * (threadIdx.x + d)));
+      store<FrP>(lds + 8 * threadIdx.x, acc);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) store<FrP>(partials + 8 * blockIdx.x, acc);
+}
+
 // ---------------------------------------------------------------- Kate division
 // q_i = a_(i+1) + b q_(i+1), i < m = n - 1, with q_m = 0 (kate_division).  Chunk c holds
 // q indices [c L, c L + L): q_lo = loc_c + b^L q_hi, composed from the top chunk down.
@@ -394,6 +439,61 @@ int mzk_fr_eval(mzk_ctx* c, const void* a, size_t n, const uint64_t x[4], uint64
     acc = host::fr_add(acc, v);
   }
   memcpy(out, acc.l, 32);
+  return MZK_OK;
+}
+
+int mzk_fr_eval_batch(mzk_ctx* c, const void* const* polys, const size_t* ns, const uint64_t* xs, size_t njobs,
+                      uint64_t* outs) {
+  MZK_REQUIRE(c != nullptr, "null context");
+  if (njobs == 0) return MZK_OK;
+  MZK_REQUIRE(polys && ns && xs && outs, "mzk_fr_eval_batch: null argument");
+  MZK_REQUIRE(njobs <= (1u << 20), "mzk_fr_eval_batch: too many jobs");
+  std::vector<EvalJob> jobs(njobs);
+  std::vector<uint64_t> pows(njobs * 32 * 4);
+  size_t blocks = 0;
+  for (size_t j = 0; j < njobs; j++) {
+    MZK_REQUIRE(ns[j] == 0 || polys[j] != nullptr, "mzk_fr_eval_batch: job %zu has null data", j);
+    const size_t threads = (ns[j] + EVAL_CHUNK - 1) / EVAL_CHUNK;
+    const size_t nb = threads == 0 ? 1 : (threads + 255) / 256;
+    jobs[j].a = (const uint32_t*)polys[j];
+    jobs[j].n = ns[j];
+    jobs[j].first_block = (uint32_t)blocks;
+    jobs[j].pad = 0;
+    blocks += nb;
+    MZK_REQUIRE(blocks < (1ull << 31), "mzk_fr_eval_batch: too much work in one call");
+    host::Fr t;
+    memcpy(t.l, xs + 4 * j, 32);
+    for (int k = 0; k < 32; k++) {
+      memcpy(&pows[(j * 32 + k) * 4], t.l, 32);
+      t = host::fr_mul(t, t);
+    }
+  }
+  MZK_TRY(vec_ready(c));
+  std::lock_guard<std::mutex> g(c->mu);
+  const size_t job_bytes = njobs * sizeof(EvalJob), pow_bytes = pows.size() * 8;
+  MZK_TRY(c->scratch[SCR_IN1].reserve(job_bytes + pow_bytes + 256));
+  MZK_TRY(c->scratch[SCR_MSM_D].reserve(blocks * 32 + 64));
+  char* tab = (char*)c->scratch[SCR_IN1].ptr;
+  const size_t pow_off = (job_bytes + 255) & ~(size_t)255;
+  uint32_t* partials = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
+  MZK_HIP(hipMemcpyAsync(tab, jobs.data(), job_bytes, hipMemcpyHostToDevice, c->stream));
+  MZK_HIP(hipMemcpyAsync(tab + pow_off, pows.data(), pow_bytes, hipMemcpyHostToDevice, c->stream));
+  MZK_LAUNCH(c, "k_eval_batch", k_eval_batch, dim3((unsigned)blocks), dim3(256), 0, (const EvalJob*)tab, (int)njobs,
+             (const uint32_t*)(tab + pow_off), partials);
+  MZK_HIP(hipGetLastError());
+  std::vector<uint64_t> h(4 * blocks);
+  MZK_HIP(hipMemcpyAsync(h.data(), partials, 32 * blocks, hipMemcpyDeviceToHost, c->stream));
+  MZK_HIP(hipStreamSynchronize(c->stream));
+  for (size_t j = 0; j < njobs; j++) {
+    const size_t b0 = jobs[j].first_block, b1 = j + 1 < njobs ? jobs[j + 1].first_block : blocks;
+    host::Fr acc = host::fr_zero();
+    for (size_t b = b0; b < b1; b++) {
+      host::Fr v;
+      memcpy(v.l, &h[4 * b], 32);
+      acc = host::fr_add(acc, v);
+    }
+    memcpy(outs + 4 * j, acc.l, 32);
+  }
   return MZK_OK;
 }
 

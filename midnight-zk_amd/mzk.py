@@ -112,6 +112,7 @@ SIGNATURES = {
     "mzk_fr_batch_invert": ([_P, _P, _S], _I),
     "mzk_fr_prefix_product": ([_P, _P, _P, _S], _I),
     "mzk_fr_eval": ([_P, _P, _S, _P, _P], _I),
+    "mzk_fr_eval_batch": ([_P, _P, _P, _P, _S, _P], _I),
     "mzk_fr_kate_div": ([_P, _P, _S, _P, _P], _I),
     "mzk_fr_powers": ([_P, _P, _S, _P], _I),
 }
@@ -358,9 +359,12 @@ class ResidentSrs:
     def bases_ptr(self) -> int:
         return load_library().mzk_srs_bases_dev(self.h)
 
-    def commit_batch(self, jobs) -> np.ndarray:
+    def commit_batch(self, jobs, ctx: "Context | None" = None) -> np.ndarray:
         """Many commitments in one pipeline.  jobs: iterable of (device coeffs, n, offset);
-        returns an (len(jobs), 18) array of Jacobian results."""
+        returns an (len(jobs), 18) array of Jacobian results.  ctx: the context (stream
+        and scratch) to run on; the bases are read-only, so any context of the same
+        device may use them (default: the one that built them)."""
+        ctx = ctx or self.ctx
         jobs = list(jobs)
         arr = (MsmJob * max(1, len(jobs)))()
         for i, (d, n, off) in enumerate(jobs):
@@ -368,7 +372,7 @@ class ResidentSrs:
             arr[i].n = n
             arr[i].offset = off
         out = np.zeros((max(1, len(jobs)), 18), dtype=np.uint64)
-        _check(load_library().mzk_msm_srs_batch_dev(self.ctx.h, self.h, ctypes.cast(arr, ctypes.c_void_p),
+        _check(load_library().mzk_msm_srs_batch_dev(ctx.h, self.h, ctypes.cast(arr, ctypes.c_void_p),
                                                      len(jobs), _np_ptr(out)))
         return out[:len(jobs)]
 

@@ -198,10 +198,12 @@ class SyntheticCircuit:
         allr = np.arange(u)
         return [mul, add, add, allr, np.zeros(0, dtype=np.int64)]
 
-    def witness(self, be):
-        """Advice values a..e (length n; rows >= u zero, the prover blinds them)."""
+    def witness(self, be, wseed: int | None = None):
+        """Advice values a..e (length n; rows >= u zero, the prover blinds them).  The
+        values come from the seeded stream of wseed (default: the layout seed)."""
         n, u = self.n, self.usable
-        cols = [be.from_seed(self.seed, 16 + s, u) for s in range(4)]   # a, b, d, e
+        ws = self.seed if wseed is None else wseed
+        cols = [be.from_seed(ws, 16 + s, u) for s in range(4)]   # a, b, d, e
         flat = be.cat(cols)
         flat = be.scatter(flat, self.copy_cells, be.gather(flat, self.copy_src))
         a, b, d, e = (be.slice(flat, s * u, (s + 1) * u) for s in range(4))
@@ -215,11 +217,15 @@ class Params:
     """ParamsKZG from unsafe_setup (params.rs:154-210): monomial bases g_i = [tau^i]G
     and Lagrange bases [L_i(tau)]G, L_i(tau) = (tau^n - 1)/n * omega^i / (tau - omega^i)."""
 
-    def __init__(self, be, k: int, tau: int):
+    def __init__(self, be, k: int, tau: int, g=None):
+        """g: monomial bases of the same tau with at least 2^k points to share (commits
+        use the prefix g[..len], kzg/mod.rs:99-103); generated when absent."""
         self.k = k
         self.n = n = 1 << k
         self.tau = tau % R_MOD
-        self.g = be.srs_monomial(self.tau, n)
+        if g is not None and g.n < n:
+            raise ValueError("shared monomial bases are shorter than 2^k")
+        self.g = g if g is not None else be.srs_monomial(self.tau, n)
         w = be.powers(omega_for(k), n)
         den = be.batch_invert(be.add_scalar(be.neg(w), self.tau))
         c = (pow(self.tau, n, R_MOD) - 1) * pow(n, -1, R_MOD) % R_MOD
@@ -311,19 +317,24 @@ def _powers(x: int, m: int):
     return out
 
 
-def create_proof(be, params: Params, pk: ProvingKey, advice_values, rng_seed: int,
-                 timings: dict | None = None) -> bytes:
-    """create_proof (prover.rs:432-474) for the synthetic circuit.  advice_values are the
-    five Lagrange columns from SyntheticCircuit.witness (rows >= n - 7 are overwritten
-    with blinding values)."""
+# Requests a proof yields to whoever drives it; the driver answers each with a list of
+# results in order.  ("commit", [(srs, vector)]) -> affine points;
+# ("ntt", [(op, vector, k, extended_k)]) -> vectors; ("eval", [(vector, x)]) -> ints.
+def prove_steps(be, params: Params, pk: ProvingKey, advice_values, rng_seed: int,
+                timings: dict | None = None):
+    """create_proof (prover.rs:432-474) for the synthetic circuit, as a generator of
+    commit / NTT / evaluation requests; returns the proof bytes.  advice_values are the
+    five Lagrange columns from SyntheticCircuit.witness (rows >= n - 7 are replaced with
+    blinding values)."""
     T0 = time.perf_counter()
-    k, n, dom = pk.k, pk.n, pk.dom
-    N = pk.ext_n
-    scale = N // n
+    k, n = pk.k, pk.n
+    ek = pk.dom.extended_k
+    scale = pk.ext_n // n
     bf = BLINDING_FACTORS
     u = n - bf - 1
     tr = Transcript()
     rng_off = 0
+    g_mono, g_lag = params.g, params.g_lagrange
 
     def draws(m):
         nonlocal rng_off
@@ -338,10 +349,8 @@ def create_proof(be, params: Params, pk: ProvingKey, advice_values, rng_seed: in
 
     # VK hash, instances (none), advice (prover.rs:101-108, 541-631)
     tr.common_fr(pk.transcript_repr)
-    advice = []
-    for v in advice_values:
-        advice.append(be.cat([be.slice(v, 0, u), draws(n - u)]))
-    for c in be.commit_many(params, [(v, "lagrange") for v in advice]):
+    advice = [be.cat([be.slice(v, 0, u), draws(n - u)]) for v in advice_values]
+    for c in (yield ("commit", [(g_lag, v) for v in advice])):
         tr.write_point(g1_compress(c))
     mark("advice_commit")
     theta = tr.squeeze()  # noqa: F841  (no lookups in this circuit)
@@ -350,9 +359,8 @@ def create_proof(be, params: Params, pk: ProvingKey, advice_values, rng_seed: in
     perm_blind = [draws(bf) for _ in range(2)]
 
     # permutation grand products (permutation/prover.rs:81-255)
-    omega = omega_for(k)
-    w = be.powers(omega, n)
-    sets = [list(range(0, 3)), list(range(3, 5))]
+    w = be.powers(omega_for(k), n)
+    sets = [list(range(0, CHUNK_LEN)), list(range(CHUNK_LEN, NUM_ADVICE))]
     zs = []
     last_uncorrected = []
     for s, cols in enumerate(sets):
@@ -374,18 +382,18 @@ def create_proof(be, params: Params, pk: ProvingKey, advice_values, rng_seed: in
     for s in range(1, len(zs)):
         corr = last_uncorrected[s - 1] if corr is None else be.mul(corr, last_uncorrected[s - 1])
         zs[s] = be.mul(zs[s], be.broadcast(corr, n))
-    for c in be.commit_many(params, [(z, "lagrange") for z in zs]):
+    for c in (yield ("commit", [(g_lag, z) for z in zs])):
         tr.write_point(g1_compress(c))
     mark("permutation_commit")
-    z_polys = [dom.lagrange_to_coeff(z) for z in zs]
     trash = tr.squeeze()  # noqa: F841  (no trashcans)
     y = tr.squeeze()
-    advice_polys = [dom.lagrange_to_coeff(v) for v in advice]
+    polys = yield ("ntt", [("lagrange_to_coeff", v, k, ek) for v in advice + zs])
+    advice_polys, z_polys = polys[:NUM_ADVICE], polys[NUM_ADVICE:]
     mark("advice_ifft")
 
     # quotient numerator on the extended coset (evaluation.rs:818-940)
-    adv_c = [dom.coeff_to_extended(p) for p in advice_polys]
-    z_c = [dom.coeff_to_extended(p) for p in z_polys]
+    cosets = yield ("ntt", [("coeff_to_extended", p, k, ek) for p in advice_polys + z_polys])
+    adv_c, z_c = cosets[:NUM_ADVICE], cosets[NUM_ADVICE:]
     mark("coset_ntt")
     qm, ql, qr, qo, qc = pk.fixed_cosets
     a, b, c = adv_c[0], adv_c[1], adv_c[2]
@@ -410,11 +418,11 @@ def create_proof(be, params: Params, pk: ProvingKey, advice_values, rng_seed: in
             dx = be.scale(pk.coset_x, beta * pow(DELTA, j, R_MOD) % R_MOD)
             right = be.mul(right, be.add_scalar(be.add(adv_c[j], dx), gamma))
         val = be.add(be.scale(val, y), be.mul(be.sub(left, right), pk.l_active_row))
-    del adv_c, z_c
+    del adv_c, z_c, cosets
     mark("numerator")
 
     # h = nu / (X^n - 1), limbs (prover.rs:701-772)
-    h = dom.extended_to_coeff(dom.divide_by_vanishing_poly(val))
+    (h,) = yield ("ntt", [("extended_to_coeff", be.divide_by_vanishing(val, k, ek), k, ek)])
     del val
     h = be.slice(h, 0, (n - 1) * QUOTIENT_DEGREE)
     limb_blind = draws(QUOTIENT_DEGREE - 1)
@@ -427,7 +435,7 @@ def create_proof(be, params: Params, pk: ProvingKey, advice_values, rng_seed: in
             first = be.sub(be.slice(limb, 0, 1), be.slice(limb_blind, i - 1, i))
             limb = be.cat([first, be.slice(limb, 1, n)])
         limbs.append(limb)
-    for c in be.commit_many(params, [(l, "monomial") for l in limbs]):
+    for c in (yield ("commit", [(g_mono, l) for l in limbs])):
         tr.write_point(g1_compress(c))
     mark("quotient_commit")
     x = tr.squeeze()
@@ -435,10 +443,10 @@ def create_proof(be, params: Params, pk: ProvingKey, advice_values, rng_seed: in
     # evaluations (prover.rs:798-869, permutation/prover.rs:266-340)
     x_next = _rotate(x, 1, k)
     x_last = _rotate(x, -(bf + 1), k)
-    evals = be.eval_many([(p, x) for p in advice_polys] + [(p, x) for p in pk.fixed_polys]
-                         + [(p, x) for p in pk.sigma_polys]
-                         + [(z_polys[0], x), (z_polys[0], x_next), (z_polys[0], x_last),
-                            (z_polys[1], x), (z_polys[1], x_next)])
+    evals = yield ("eval", [(p, x) for p in advice_polys] + [(p, x) for p in pk.fixed_polys]
+                   + [(p, x) for p in pk.sigma_polys]
+                   + [(z_polys[0], x), (z_polys[0], x_next), (z_polys[0], x_last),
+                      (z_polys[1], x), (z_polys[1], x_next)])
     for e in evals:
         tr.write_fr(e)
     mark("evaluations")
@@ -447,24 +455,49 @@ def create_proof(be, params: Params, pk: ProvingKey, advice_values, rng_seed: in
     # identity is fully evaluated and L'(X) = -(x^n - 1) sum_j (x^(n-1))^j h_j(X)
     split = pow(x, n - 1, R_MOD)
     xn = split * x % R_MOD
-    lin = be.zeros(n)
     sp = (xn - 1) % R_MOD
-    for limb in limbs:
-        lin = be.axpy(lin, limb, (-sp) % R_MOD)
+    coeffs = []
+    for _ in limbs:
+        coeffs.append((-sp) % R_MOD)
         sp = sp * split % R_MOD
+    lin = be.inner_product(limbs, coeffs)
 
     # queries in compute_queries order (prover.rs:871-944)
-    L_ADV = [("advice", i) for i in range(NUM_ADVICE)]
-    queries = [(L_ADV[i], x, advice_polys[i]) for i in range(NUM_ADVICE)]
+    queries = [(("advice", i), x, advice_polys[i]) for i in range(NUM_ADVICE)]
     queries += [(("perm", 0), x, z_polys[0]), (("perm", 0), x_next, z_polys[0]),
                 (("perm", 1), x, z_polys[1]), (("perm", 1), x_next, z_polys[1]),
                 (("perm", 0), x_last, z_polys[0])]
     queries += [(("fixed", i), x, pk.fixed_polys[i]) for i in range(NUM_FIXED)]
     queries += [(("sigma", i), x, pk.sigma_polys[i]) for i in range(NUM_ADVICE)]
     queries += [(("lin", 0), x, lin)]
-    multi_open(be, params, queries, tr)
+    yield from multi_open(be, g_mono, queries, tr)
     mark("multi_open")
     return bytes(tr.proof)
+
+
+def serve_request(be, req):
+    """Answers one request of prove_steps on backend be."""
+    kind, items = req
+    if kind == "commit":
+        return be.commit_jobs(items)
+    if kind == "ntt":
+        return be.ntt_many(items)
+    if kind == "eval":
+        return be.eval_many(items)
+    raise ValueError(f"unknown request {kind!r}")
+
+
+def create_proof(be, params: Params, pk: ProvingKey, advice_values, rng_seed: int,
+                 timings: dict | None = None) -> bytes:
+    """create_proof (prover.rs:432-474): one proof, every request answered at once."""
+    gen = prove_steps(be, params, pk, advice_values, rng_seed, timings)
+    resp = None
+    while True:
+        try:
+            req = gen.send(resp)
+        except StopIteration as stop:
+            return stop.value
+        resp = serve_request(be, req)
 
 
 def construct_intermediate_sets(queries):
@@ -495,8 +528,8 @@ def construct_intermediate_sets(queries):
     return com_labels, com_sets, point_sets
 
 
-def multi_open(be, params, queries, tr: Transcript):
-    """KZGCommitmentScheme::multi_open (kzg/mod.rs:110-289)."""
+def multi_open(be, g_mono, queries, tr: Transcript):
+    """KZGCommitmentScheme::multi_open (kzg/mod.rs:110-289), as a request generator."""
     x1 = tr.squeeze()
     x2 = tr.squeeze()
     label_poly = {}
@@ -517,16 +550,16 @@ def multi_open(be, params, queries, tr: Transcript):
             p = be.kate_div(p, pt)
         f_parts.append(p)
     f_poly = be.inner_product(f_parts, _powers(x2, len(f_parts)))
-    (f_com,) = be.commit_many(params, [(f_poly, "monomial")])
+    (f_com,) = yield ("commit", [(g_mono, f_poly)])
     tr.write_point(g1_compress(f_com))
     x3 = tr.squeeze()
-    for e in be.eval_many([(q, x3) for q in q_polys]):
+    for e in (yield ("eval", [(q, x3) for q in q_polys])):
         tr.write_fr(e)
     x4 = tr.squeeze()
     final = be.inner_product(q_polys + [f_poly], _powers(x4, len(q_polys) + 1))
-    (v,) = be.eval_many([(final, x3)])
+    (v,) = yield ("eval", [(final, x3)])
     pi = be.kate_div(be.sub_scalar_at0(final, v), x3)
-    (pi_com,) = be.commit_many(params, [(pi, "monomial")])
+    (pi_com,) = yield ("commit", [(g_mono, pi)])
     tr.write_point(g1_compress(pi_com))
 
 
@@ -729,14 +762,20 @@ class GpuBackend:
         return out
 
     def eval_many(self, items):
-        res = []
-        out = np.zeros(4, dtype=np.uint64)
-        for p, x in items:
-            xl = mont_limbs(x)
-            self.mzk._check(self._L.mzk_fr_eval(self.ctx.h, self._p(p), p.shape[0], xl.ctypes.data,
-                                                out.ctypes.data))
-            res.append(from_mont_limbs(out))
-        return res
+        """sum_i p_i x^i for every (p, x): one launch and one wait for all of them."""
+        import ctypes
+        items = list(items)
+        if not items:
+            return []
+        m = len(items)
+        ptrs = (ctypes.c_void_p * m)(*[self._p(p) if p.shape[0] else None for p, _ in items])
+        ns = (ctypes.c_size_t * m)(*[p.shape[0] for p, _ in items])
+        xs = np.concatenate([mont_limbs(x) for _, x in items])
+        outs = np.zeros(4 * m, dtype=np.uint64)
+        self.mzk._check(self._L.mzk_fr_eval_batch(self.ctx.h, ctypes.cast(ptrs, ctypes.c_void_p),
+                                                  ctypes.cast(ns, ctypes.c_void_p), xs.ctypes.data, m,
+                                                  outs.ctypes.data))
+        return [from_mont_limbs(outs[4 * j:4 * j + 4]) for j in range(m)]
 
     def kate_div(self, a, b: int):
         q = self._empty(max(0, a.shape[0] - 1))
@@ -758,20 +797,55 @@ class GpuBackend:
         self.mzk._check(self._L.mzk_srs_from_scalars_dev(self.ctx.h, self._p(s), s.shape[0], ctypes.byref(h)))
         return self.mzk.ResidentSrs(self.ctx, h)
 
-    def commit_many(self, params: Params, items):
-        """KZG commits (kzg/mod.rs:82-108): one batched MSM per basis.  Returns affine
-        12-limb points."""
+    def divide_by_vanishing(self, v, k: int, ek: int):
+        """divide_by_vanishing_poly (domain.rs:361-380) on this backend's stream."""
+        doms = self.__dict__.setdefault("_doms", {})
+        if (k, ek) not in doms:
+            doms[(k, ek)] = self.mzk.EvaluationDomain(self.ctx, 1 + (1 << (ek - k)), k)
+        out = v.clone()
+        doms[(k, ek)].divide_by_vanishing_poly(out)
+        return out
+
+    def ntt_many(self, items):
+        """EvaluationDomain transforms of any sizes in one batched launch pair
+        (lagrange_to_coeff, coeff_to_extended, extended_to_coeff; domain.rs:249-328)."""
+        mzk = self.mzk
+        jobs, outs = [], []
+        for op, v, k, ek in items:
+            n, N = 1 << k, 1 << ek
+            if op == "lagrange_to_coeff":
+                out = self._empty(n)
+                jobs.append(mzk.NttJob.make(v, out, k, mzk.NTT_INVERSE, n))
+            elif op == "coeff_to_extended":
+                out = self._empty(N)
+                jobs.append(mzk.NttJob.make(v, out, ek, mzk.NTT_COSET, n))
+            elif op == "extended_to_coeff":
+                out = self._empty(N)
+                jobs.append(mzk.NttJob.make(v, out, ek, mzk.NTT_INVERSE | mzk.NTT_COSET, N))
+            else:
+                raise ValueError(f"unknown transform {op!r}")
+            if v.shape[0] != (N if op == "extended_to_coeff" else n):
+                raise ValueError(f"{op}: input length {v.shape[0]} does not match the domain")
+            outs.append(out)
+        if jobs:
+            self.ctx.ntt_batch(jobs)
+        return outs
+
+    def commit_jobs(self, items):
+        """KZG commits (kzg/mod.rs:82-108): [(srs, coefficients)] -> affine points; all
+        commits against one SRS go through one batched MSM pipeline."""
         out = [None] * len(items)
-        for basis in ("monomial", "lagrange"):
-            idx = [i for i, (_, b) in enumerate(items) if b == basis]
-            if not idx:
-                continue
-            srs = params.g if basis == "monomial" else params.g_lagrange
-            jobs = [(items[i][0], items[i][0].shape[0], 0) for i in idx]
-            for n in (j[1] for j in jobs):
-                if n > srs.n:
-                    raise ValueError(f"polynomial of length {n} exceeds the {srs.n} SRS bases")
-            jac = srs.commit_batch(jobs)
+        groups = {}
+        for i, (srs, v) in enumerate(items):
+            if v.shape[0] > srs.n:
+                raise ValueError(f"polynomial of length {v.shape[0]} exceeds the {srs.n} SRS bases")
+            groups.setdefault(id(srs), (srs, []))[1].append(i)
+        for srs, idx in groups.values():
+            jac = srs.commit_batch([(items[i][1], items[i][1].shape[0], 0) for i in idx], ctx=self.ctx)
             for r, i in enumerate(idx):
                 out[i] = self.mzk.g1_to_affine(jac[r])
         return out
+
+    def commit_many(self, params: Params, items):
+        """[(coefficients, "monomial" | "lagrange")] against params' bases."""
+        return self.commit_jobs([(params.g if b == "monomial" else params.g_lagrange, v) for v, b in items])

@@ -146,13 +146,31 @@ class CpuBackend:
     def srs_from_scalars(self, s):
         return CpuSrs(C.g1_mul_generator_many(s))
 
-    def commit_many(self, params, items):
+    def divide_by_vanishing(self, v, k, ek):
+        return C.divide_by_vanishing_poly(v, k, ek)
+
+    def ntt_many(self, items):
+        outs = []
+        for op, v, k, ek in items:
+            if op == "lagrange_to_coeff":
+                outs.append(C.intt(v, k))
+            elif op == "coeff_to_extended":
+                outs.append(C.coeff_to_extended(v, k, ek))
+            elif op == "extended_to_coeff":
+                outs.append(C.extended_to_coeff(v, ek))
+            else:
+                raise ValueError(op)
+        return outs
+
+    def commit_jobs(self, items):
         out = []
-        for v, basis in items:
-            srs = params.g if basis == "monomial" else params.g_lagrange
+        for srs, v in items:
             n = v.shape[0]
             if n > srs.n:
                 raise ValueError("polynomial longer than the SRS")
             jac = C.msm_specific(np.ascontiguousarray(v), np.ascontiguousarray(srs.bases[:n]))
             out.append(C.g1_to_affine(jac))
         return out
+
+    def commit_many(self, params, items):
+        return self.commit_jobs([(params.g if b == "monomial" else params.g_lagrange, v) for v, b in items])

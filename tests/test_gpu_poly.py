@@ -141,3 +141,13 @@ def test_bad_arguments_are_errors(be):
     d = to_dev(C.fr_from_seed(1, 1, 8))
     assert L.mzk_fr_kate_div(be.ctx.h, d.data_ptr(), 8, plonk.mont_limbs(1).ctypes.data, d.data_ptr()) == 1
     assert L.mzk_fr_prefix_product(be.ctx.h, d.data_ptr(), d.data_ptr(), (1 << 24) + 1) == 1
+
+
+def test_eval_batch_mixed_sizes_and_points(be):
+    sizes = [0, 1, 64, 65, 1000, 16384, 16385, (1 << 18) + 3, 7]
+    polys = [C.fr_from_seed(9, i, n) if n else np.zeros((0, 4), dtype=np.uint64) for i, n in enumerate(sizes)]
+    xs = [(0x1111 * (i + 3)) ** 5 % O.R_MOD for i in range(len(sizes))]
+    got = be.eval_many([(to_dev(p), x) for p, x in zip(polys, xs)])
+    want = [plonk.from_mont_limbs(C.poly_eval(p, plonk.mont_limbs(x))) if p.shape[0] else 0
+            for p, x in zip(polys, xs)]
+    assert got == want
