@@ -125,3 +125,70 @@ def test_coset_extension_matches_python(k, ext_k):
     assert C.fr_rows_to_ints(back) == vals + [0] * ((1 << ext_k) - (1 << k))
     div = C.divide_by_vanishing_poly(ext, k, ext_k)
     assert C.fr_rows_to_ints(div) == O.divide_by_vanishing_poly(O.coeff_to_extended(vals, ext_k), k, ext_k)
+
+
+# ---- the prover's scalar-vector restatements, against integer arithmetic
+def test_vector_ops_match_integers():
+    R = O.R_MOD
+    vals_a = [0, 1, 2, R - 1, R - 2, 12345] + [O.fr_from_seed(3, 1, i) for i in range(40)]
+    vals_b = [0, R - 1, 2, R - 1, 3, 54321] + [O.fr_from_seed(3, 2, i) for i in range(40)]
+    a, b = C.ints_to_fr_rows(vals_a), C.ints_to_fr_rows(vals_b)
+    assert C.fr_rows_to_ints(C.fr_vec_op(0, a, b)) == [(x + y) % R for x, y in zip(vals_a, vals_b)]
+    assert C.fr_rows_to_ints(C.fr_vec_op(1, a, b)) == [(x - y) % R for x, y in zip(vals_a, vals_b)]
+    assert C.fr_rows_to_ints(C.fr_vec_op(2, a, b)) == [x * y % R for x, y in zip(vals_a, vals_b)]
+    s = O.fr_from_seed(9, 9, 0)
+    sl = O.to_limbs(O.fr_to_mont(s), 4)
+    sm = np.array(sl, dtype=np.uint64)
+    assert C.fr_rows_to_ints(C.fr_vec_scalar(0, a, sm)) == [x * s % R for x in vals_a]
+    assert C.fr_rows_to_ints(C.fr_vec_scalar(1, a, sm)) == [(x + s) % R for x in vals_a]
+    acc = b.copy()
+    C.fr_vec_scalar(2, a, sm, acc)
+    assert C.fr_rows_to_ints(acc) == [(y + x * s) % R for x, y in zip(vals_a, vals_b)]
+
+
+def test_batch_invert_prefix_powers_match_integers():
+    R = O.R_MOD
+    vals = [O.fr_from_seed(4, 1, i) for i in range(70)]
+    vals[5] = vals[33] = 0
+    a = C.ints_to_fr_rows(vals)
+    assert C.fr_rows_to_ints(C.fr_batch_invert(a)) == [pow(v, -1, R) if v else 0 for v in vals]
+    acc, want = 1, []
+    for v in vals:
+        acc = acc * v % R
+        want.append(acc)
+    assert C.fr_rows_to_ints(C.fr_prefix_product(a)) == want
+    x = O.fr_from_seed(4, 2, 0)
+    xm = np.array(O.to_limbs(O.fr_to_mont(x), 4), dtype=np.uint64)
+    assert C.fr_rows_to_ints(C.fr_powers(xm, 50)) == [pow(x, i, R) for i in range(50)]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 10, 100])
+def test_kate_division_is_synthetic_division(n):
+    """q = (a(X) - a(b)) / (X - b) as exact polynomial long division."""
+    R = O.R_MOD
+    vals = [O.fr_from_seed(5, n, i) for i in range(n)]
+    b = O.fr_from_seed(5, 0, 7)
+    bm = np.array(O.to_limbs(O.fr_to_mont(b), 4), dtype=np.uint64)
+    q = C.fr_rows_to_ints(C.fr_kate_div(C.ints_to_fr_rows(vals), bm))
+    assert len(q) == n - 1
+    # long division from the top: r_{n-1} = a_{n-1}; q_{i-1} = r_i; r_{i-1} = a_{i-1} + b q_{i-1}
+    want = [0] * (n - 1)
+    carry = 0
+    for i in range(n - 1, 0, -1):
+        carry = (vals[i] + b * carry) % R
+        want[i - 1] = carry
+    assert q == want
+    # and q(X)(X - b) + a(b) = a(X) coefficient by coefficient
+    ab = O.poly_eval(vals, b)
+    prod = [0] * n
+    for i, c in enumerate(q):
+        prod[i + 1] = (prod[i + 1] + c) % R
+        prod[i] = (prod[i] - b * c) % R
+    prod[0] = (prod[0] + ab) % R
+    assert prod == [v % R for v in vals]
+
+
+def test_bases_from_scalars_match_python():
+    s = [O.fr_from_seed(6, 1, i) for i in range(20)] + [0, 1, O.R_MOD - 1]
+    rows = C.g1_mul_generator_many(C.ints_to_fr_rows(s))
+    assert aff_rows_to_points(rows) == [O.g1_mul(O.G1_GEN, v) for v in s]
