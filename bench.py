@@ -33,6 +33,20 @@ FP_MUL_PER_MADD = 10        # XYZZ mixed addition: 8M + 2S
 MAD_PER_FP_MUL = 300        # 12-limb CIOS: 2 * 12^2 + 12 32x32+64 multiply-adds
 MAD_PER_FR_MUL = 136        # 8-limb CIOS: 2 * 8^2 + 8
 SEED = 0x5EED
+COPY_BYTES = 256 << 20      # calibration pass of the traffic probe
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_msm_accum_2p20_traffic.md")
+
+
+def committed_traffic():
+    """HBM bytes per k_msm_accum launch at 2^20 from the committed rocprofv3 counter
+    summary (line 'traffic_bytes_per_launch: N'), or None."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        return None
+    return None
 
 
 def measure_mad_peak(mzk, ctx) -> float:
@@ -224,6 +238,10 @@ def main() -> int:
     ap.add_argument("--cpu-prove-k", type=int, default=16)
     ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[20.0, 40.0])
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only run a 256 MiB elementwise copy and 3 MSM commits (for rocprofv3 --pmc runs)")
+    ap.add_argument("--prove-only", action="store_true",
+                    help="only run keygen and 3 proofs at k=18 (for rocprofv3 kernel traces of the prover)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the MSM/NTT path has no multi-GPU form in this build "
@@ -238,6 +256,11 @@ def main() -> int:
     ctx = mzk.Context(0)
     n = 1 << args.log_n
 
+    if args.prove_only:
+        r = bench_prove(pkg, ctx, 18, reps=3)
+        print(json.dumps({k: v for k, v in r.items() if not k.startswith("_")}), flush=True)
+        return 0
+
     # ---- resident inputs: tau, SRS bases on the device, scalars on the device
     tau_d = torch.zeros((1, 4), dtype=torch.int64, device="cuda")
     ctx.fr_from_seed_dev(SEED, 0, 1, tau_d)
@@ -250,6 +273,18 @@ def main() -> int:
     ctx.fr_from_seed_dev(SEED, 1, n, sc)
     ctx.synchronize()
     torch.cuda.synchronize()
+
+    if args.traffic_probe:
+        # calibration: one elementwise pass that reads and writes exactly COPY_BYTES each
+        a = torch.zeros(COPY_BYTES // 8, dtype=torch.int64, device="cuda")
+        b = torch.empty_like(a)
+        torch.bitwise_xor(a, 1, out=b)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            srs.commit(sc, n)
+        ctx.synchronize()
+        print(json.dumps({"traffic_probe": True, "copy_bytes": COPY_BYTES, "msm_n": n}), flush=True)
+        return 0
 
     # ---- per-call MSM: warmup, then K timed steps between synchronisations
     for _ in range(args.warmup):
@@ -378,7 +413,7 @@ def main() -> int:
             "peak": round(peak / 1e12, 4),
             "unit": "Tmad/s",
             "frac": round(achieved / peak, 4),
-            "traffic": None,
+            "traffic": committed_traffic() if args.log_n == 20 else None,
             "per_launch": {"madds": digits, "mads_algorithmic": alg_mads, "avg_ms": round(acc_ms, 4)},
         },
         "kernels_ms_per_step": {k: round(v[0] / v[1], 4) for k, v in sorted(prof.items())},
