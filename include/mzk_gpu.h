@@ -219,7 +219,11 @@ int mzk_diag_g1_op(mzk_ctx *ctx, int op, const void *d_a, const void *d_b, void 
  *   which 4: dependent 32x32+64 multiply-add chains with no other instruction
  *            (v_mad_u64_u32 issue rate: the INT32 multiply roof of the chip);
  *   which 5: fp product-scanning (default) products, many chains (throughput);
- *   which 6: fr product-scanning (default) products, many chains (throughput). */
+ *   which 6: fr product-scanning (default) products, many chains (throughput);
+ *   which 7: fp products in the radix-2^28 form of the MSM accumulation, many chains;
+ *   which 8: XYZZ mixed additions, 32-bit form (one accumulator per thread);
+ *   which 9: XYZZ mixed additions, radix-2^28 form;
+ *   which 10: dependent 32-bit multiply-lows (v_mul_lo_u32 issue rate). */
 int mzk_diag_bench(mzk_ctx *ctx, int which, int blocks, int threads, int iters, double *ms,
                    double *ops);
 

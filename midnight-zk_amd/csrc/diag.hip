@@ -2,6 +2,7 @@
 // that measure the chip's 32x32+64 multiply-add rate (the roof for MSM and NTT).
 #include "ctx.hpp"
 #include "g1.hpp"
+#include "g1_28.hpp"
 
 namespace mzk {
 
@@ -94,6 +95,66 @@ __global__ void k_bench_mul(int iters, const uint32_t* __restrict__ src, uint32_
   if (x.v[0] == 0x12345678u && x.v[1] == t) store<P>(sink, x);
 }
 
+// radix-2^28 products, many independent chains
+__global__ void k_bench_mul28(int iters, const uint32_t* __restrict__ src, uint32_t* __restrict__ sink) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  f28 x = f28_from_std(src), y = f28_from_std(src + 12);
+  x.l[0] ^= t & 0xff;
+  for (int it = 0; it < iters; it++) x = f28_mul(x, y);
+  if (x.l[0] == 0x1234567u && x.l[1] == t) sink[t] = x.l[2];
+}
+
+// mixed additions into one accumulator per thread: the 32-bit form (g1.hpp) and the
+// radix-2^28 form (g1_28.hpp); the added points vary per iteration
+__global__ void k_bench_madd(int iters, int form28, const uint32_t* __restrict__ src, uint32_t* __restrict__ sink) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t xs[12], ys[12];
+  for (int i = 0; i < 12; i++) {
+    xs[i] = src[i];
+    ys[i] = src[12 + i];
+  }
+  xs[0] ^= t & 0xff;
+  if (form28) {
+    g1x28 acc = g1x28_from_affine(f28_from_std(src + 4), f28_from_std(src + 12), false);
+    for (int it = 0; it < iters; it++) {
+      xs[1] += it;
+      acc = g1x28_madd(acc, f28_from_std(xs), f28_from_std(ys), (it & 1) != 0);
+    }
+    if (acc.X.l[0] == 0x1234567u && acc.X.l[1] == t) sink[t] = acc.Y.l[0];
+  } else {
+    g1_aff_t q0;
+    q0.x = load<FpP>(src + 4);
+    q0.y = load<FpP>(src + 12);
+    g1_xyzz_t acc = g1_xyzz_t::from_affine(q0);
+    for (int it = 0; it < iters; it++) {
+      xs[1] += it;
+      g1_aff_t q;
+      for (int i = 0; i < 12; i++) {
+        q.x.v[i] = xs[i];
+        q.y.v[i] = ys[i];
+      }
+      if (it & 1) q = g1_neg(q);
+      acc = g1_add_affine(acc, q);
+    }
+    if (acc.X.v[0] == 0x12345678u && acc.X.v[1] == t) sink[t] = acc.Y.v[0];
+  }
+}
+
+// dependent 32-bit multiply-lows (v_mul_lo_u32 issue rate)
+__global__ void k_bench_mullo(int iters, const uint32_t* __restrict__ src, uint32_t* __restrict__ sink) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t a[8];
+  for (int k = 0; k < 8; k++) a[k] = src[k] ^ t;
+  const uint32_t m = src[9] | 1u;
+  for (int it = 0; it < iters; it++) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) a[k] = a[k] * m;
+  }
+  uint32_t s = 0;
+  for (int k = 0; k < 8; k++) s ^= a[k];
+  if (s == 0x12345678u) sink[t] = s;
+}
+
 }  // namespace mzk
 
 using namespace mzk;
@@ -135,7 +196,7 @@ int mzk_diag_g1_op(mzk_ctx* c, int op, const void* a, const void* b, void* out, 
 
 int mzk_diag_bench(mzk_ctx* c, int which, int blocks, int threads, int iters, double* ms, double* ops) {
   MZK_REQUIRE(c != nullptr && ms && ops, "null argument");
-  MZK_REQUIRE(which >= 0 && which <= 6, "which out of range");
+  MZK_REQUIRE(which >= 0 && which <= 10, "which out of range");
   MZK_REQUIRE(blocks > 0 && threads > 0 && threads <= 1024 && iters > 0, "bad launch shape");
   std::lock_guard<std::mutex> g(c->mu);
   MZK_HIP(hipSetDevice(c->device));
@@ -169,6 +230,20 @@ int mzk_diag_bench(mzk_ctx* c, int which, int blocks, int threads, int iters, do
     case 6:
       hipLaunchKernelGGL((k_bench_mul<FrP, true>), dim3(blocks), dim3(threads), 0, c->stream, iters, buf, sink);
       per_thread = iters;
+      break;
+    case 7:
+      hipLaunchKernelGGL(k_bench_mul28, dim3(blocks), dim3(threads), 0, c->stream, iters, buf, sink);
+      per_thread = iters;
+      break;
+    case 8:
+    case 9:
+      hipLaunchKernelGGL(k_bench_madd, dim3(blocks), dim3(threads), 0, c->stream, iters, which == 9 ? 1 : 0, buf,
+                         sink);
+      per_thread = iters;
+      break;
+    case 10:
+      hipLaunchKernelGGL(k_bench_mullo, dim3(blocks), dim3(threads), 0, c->stream, iters, buf, sink);
+      per_thread = 8.0 * iters;
       break;
     case 4:
       hipLaunchKernelGGL(k_bench_mad_pure, dim3(blocks), dim3(threads), 0, c->stream, iters, buf, (uint64_t*)sink);

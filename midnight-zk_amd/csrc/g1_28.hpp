@@ -1,0 +1,195 @@
+// XYZZ bucket accumulation and bucket reduction in the radix-2^28 field form of
+// fp28.hpp.
+//
+// Same formulas as g1.hpp (madd-2008-s with the exceptional cases handled exactly,
+// mdbl-2008-s-1 for the doubling case), restated on f28 with the value bounds the
+// form needs.  Invariants of an accumulator entering or leaving g1_28_madd:
+//   X < 10p, Y < 6p (normalised), ZZ, ZZZ < 2p (product outputs), or the identity
+//   (ZZ literally zero).
+// Affine inputs come straight from HBM in the blst layout; from_std leaves them
+// below 2^389 < 256 p.
+#pragma once
+#include "fp28.hpp"
+
+// words of one point in this form (4 coordinates x 14 limbs; 224 bytes, 16-byte aligned)
+#define MZK_X28_WORDS 56
+
+namespace mzk {
+
+struct g1x28 {
+  f28 X, Y, ZZ, ZZZ;
+};
+
+MZK_HD g1x28 g1x28_inf() {
+  g1x28 r;
+  r.X = f28_zero();
+  r.Y = f28_zero();
+  r.ZZ = f28_zero();
+  r.ZZZ = f28_zero();
+  return r;
+}
+
+MZK_HD bool g1x28_is_inf(const g1x28& p) { return f28_is_literal_zero(p.ZZ); }
+
+// An affine input as an accumulator: coordinates brought below 2p (one product each
+// by 2^392 mod p), y negated when neg.
+MZK_HD g1x28 g1x28_from_affine(const f28& x, const f28& y, bool neg) {
+  g1x28 r;
+  const f28 one = f28_one();
+  r.X = f28_mul(x, one);                      // < 2p
+  f28 yy = f28_mul(y, one);                   // < 2p
+  if (neg) yy = f28_norm(f28_sub<4, 1>(f28_zero(), yy));   // 4p - y < 4p
+  r.Y = yy;
+  r.ZZ = one;
+  r.ZZZ = one;
+  return r;
+}
+
+// 2 (x, +-y) for an affine point (mdbl-2008-s-1); coordinates as from_affine makes
+// them (x < 2p, y < 4p, normalised).
+MZK_HD g1x28 g1x28_dbl_affine(const f28& x, const f28& y) {
+  g1x28 r;
+  const f28 U = f28_add(y, y);                     // < 8p, limbs < 2^29
+  const f28 V = f28_sqr(U);                        // 64 p^2 -> < 2p
+  const f28 W = f28_mul(U, V);
+  const f28 S = f28_mul(x, V);
+  const f28 x2 = f28_sqr(x);
+  const f28 M = f28_add(f28_add(x2, x2), x2);      // < 6p, limbs < 3 2^28
+  const f28 S2 = f28_add(S, S);                    // < 4p, limbs < 2^29
+  // X3 = M^2 + 8p - 2S: lower limbs of 8p raised by 2 2^28 cover 2S's (< 2^29)
+  r.X = f28_norm(f28_sub<8, 2>(f28_sqr(M), S2));   // < 10p
+  const f28 T = f28_sub<16, 1>(S, r.X);            // S + 16p - X3 < 18p
+  r.Y = f28_norm(f28_sub<4, 1>(f28_mul(M, T), f28_mul(W, y)));   // < 6p
+  r.ZZ = V;
+  r.ZZZ = W;
+  return r;
+}
+
+// acc + (x, +-y) (madd-2008-s).  x, y from from_std (< 2^389); the affine identity
+// is the caller's to skip.
+MZK_HD g1x28 g1x28_madd(const g1x28& p, const f28& x, const f28& y, bool neg) {
+  if (g1x28_is_inf(p)) return g1x28_from_affine(x, y, neg);
+  const f28 U2 = f28_mul(x, p.ZZ);                 // 512 p^2 -> < 2p
+  const f28 S2 = f28_mul(y, p.ZZZ);                // < 2p
+  // P = U2 - X < 18p (X < 10p; 16p's top limb minus 1 covers X's)
+  const f28 P = f28_sub<16, 1>(U2, p.X);
+  // R = +-S2 - Y: S2 + 8p - Y < 10p, or 16p - S2 - Y < 16p (lower limbs raised by 2 2^28)
+  const f28 R = neg ? f28_neg_sum<16, 2>(S2, p.Y) : f28_sub<8, 1>(S2, p.Y);
+  if (f28_is_zero_mod_p<18>(P)) {
+    if (f28_is_zero_mod_p<16>(R)) {
+      const g1x28 q = g1x28_from_affine(x, y, neg);
+      return g1x28_dbl_affine(q.X, q.Y);
+    }
+    return g1x28_inf();
+  }
+  const f28 PP = f28_sqr(P);                       // 324 p^2 -> < 2p
+  const f28 PPP = f28_mul(P, PP);
+  const f28 Q = f28_mul(p.X, PP);
+  g1x28 r;
+  // X3 = R^2 + 8p - PPP - 2Q: lower limbs of 8p raised by 3 2^28 cover PPP + 2Q
+  const f28 Q2 = f28_add(Q, Q);
+  r.X = f28_norm(f28_sub<8, 3>(f28_sqr(R), f28_add(PPP, Q2)));   // < 10p
+  // Y3 = R (Q + 16p - X3) - Y PPP + 4p
+  const f28 T = f28_sub<16, 1>(Q, r.X);            // < 18p
+  r.Y = f28_norm(f28_sub<4, 1>(f28_mul(R, T), f28_mul(p.Y, PPP)));   // < 6p
+  r.ZZ = f28_mul(p.ZZ, PP);
+  r.ZZZ = f28_mul(p.ZZZ, PPP);
+  return r;
+}
+
+// 2 P (dbl-2008-s-1) for P within the accumulator invariants
+MZK_HD g1x28 g1x28_dbl(const g1x28& p) {
+  g1x28 r;
+  const f28 U = f28_add(p.Y, p.Y);                 // < 12p, limbs < 2^29
+  const f28 V = f28_sqr(U);                        // 144 p^2 -> < 2p
+  const f28 W = f28_mul(U, V);
+  const f28 S = f28_mul(p.X, V);
+  const f28 x2 = f28_sqr(p.X);                     // 100 p^2
+  const f28 M = f28_add(f28_add(x2, x2), x2);      // < 6p, limbs < 3 2^28
+  r.X = f28_norm(f28_sub<8, 2>(f28_sqr(M), f28_add(S, S)));      // < 10p
+  const f28 T = f28_sub<16, 1>(S, r.X);            // < 18p
+  r.Y = f28_norm(f28_sub<4, 1>(f28_mul(M, T), f28_mul(W, p.Y)));   // < 6p
+  r.ZZ = f28_mul(V, p.ZZ);
+  r.ZZZ = f28_mul(W, p.ZZZ);
+  return r;
+}
+
+// P + Q (add-2008-s), both within the accumulator invariants
+MZK_HD g1x28 g1x28_add(const g1x28& p, const g1x28& q) {
+  if (g1x28_is_inf(p)) return q;
+  if (g1x28_is_inf(q)) return p;
+  const f28 U1 = f28_mul(p.X, q.ZZ);              // 20 p^2 -> < 2p
+  const f28 U2 = f28_mul(q.X, p.ZZ);
+  const f28 S1 = f28_mul(p.Y, q.ZZZ);
+  const f28 S2 = f28_mul(q.Y, p.ZZZ);
+  const f28 P = f28_sub<4, 1>(U2, U1);            // < 6p
+  const f28 R = f28_sub<4, 1>(S2, S1);            // < 6p
+  if (f28_is_zero_mod_p<6>(P)) {
+    if (f28_is_zero_mod_p<6>(R)) return g1x28_dbl(p);
+    return g1x28_inf();
+  }
+  const f28 PP = f28_sqr(P);
+  const f28 PPP = f28_mul(P, PP);
+  const f28 Q = f28_mul(U1, PP);
+  g1x28 r;
+  r.X = f28_norm(f28_sub<8, 3>(f28_sqr(R), f28_add(PPP, f28_add(Q, Q))));   // < 10p
+  const f28 T = f28_sub<16, 1>(Q, r.X);            // < 18p
+  r.Y = f28_norm(f28_sub<4, 1>(f28_mul(R, T), f28_mul(S1, PPP)));   // < 6p
+  r.ZZ = f28_mul(f28_mul(p.ZZ, q.ZZ), PP);
+  r.ZZZ = f28_mul(f28_mul(p.ZZZ, q.ZZZ), PPP);
+  return r;
+}
+
+// accumulator -> blst-layout XYZZ (4 x 12 canonical Montgomery limbs)
+MZK_HD void g1x28_to_std(const g1x28& p, uint32_t out[48]) {
+  if (g1x28_is_inf(p)) {
+    for (int i = 0; i < 48; i++) out[i] = 0;
+    return;
+  }
+  f28_to_std(p.X, out);
+  f28_to_std(p.Y, out + 12);
+  f28_to_std(p.ZZ, out + 24);
+  f28_to_std(p.ZZZ, out + 36);
+}
+
+}  // namespace mzk
+
+#if defined(__HIPCC__)
+namespace mzk {
+__device__ __forceinline__ g1x28 load_x28(const uint32_t* __restrict__ p) {
+  g1x28 r;
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+  uint32_t w[MZK_X28_WORDS];
+#pragma unroll
+  for (int i = 0; i < MZK_X28_WORDS / 4; i++) {
+    const uint4 v = q[i];
+    w[4 * i] = v.x;
+    w[4 * i + 1] = v.y;
+    w[4 * i + 2] = v.z;
+    w[4 * i + 3] = v.w;
+  }
+#pragma unroll
+  for (int i = 0; i < 14; i++) {
+    r.X.l[i] = w[i];
+    r.Y.l[i] = w[14 + i];
+    r.ZZ.l[i] = w[28 + i];
+    r.ZZZ.l[i] = w[42 + i];
+  }
+  return r;
+}
+
+__device__ __forceinline__ void store_x28_raw(uint32_t* __restrict__ p, const g1x28& a) {
+  uint32_t w[MZK_X28_WORDS];
+#pragma unroll
+  for (int i = 0; i < 14; i++) {
+    w[i] = a.X.l[i];
+    w[14 + i] = a.Y.l[i];
+    w[28 + i] = a.ZZ.l[i];
+    w[42 + i] = a.ZZZ.l[i];
+  }
+  uint4* q = reinterpret_cast<uint4*>(p);
+#pragma unroll
+  for (int i = 0; i < MZK_X28_WORDS / 4; i++) q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+}
+}  // namespace mzk
+#endif

@@ -13,7 +13,8 @@
 //  4. k_sort_bins     per bin, a counting sort on the low key bits: references
 //                     grouped by bucket, and the bucket offsets
 //  5. k_msm_accum     each thread adds a fixed chunk of K sorted references into XYZZ
-//                     partial bucket sums (mixed additions, 8M + 2S): equal work per
+//                     partial bucket sums (mixed additions, 8M + 2S, in the radix-2^28
+//                     field form of fp28.hpp): equal work per
 //                     thread whatever the bucket sizes
 //  6. k_bucket_seg    per segment of L = 4 buckets: merge each bucket's chunk
 //                     partials, running sums -> A_t = sum_i (i+1) S, R_t = sum_i S
@@ -33,6 +34,7 @@
 
 #include "ctx.hpp"
 #include "g1.hpp"
+#include "g1_28.hpp"
 #include "host_ec.hpp"
 #include "msm.hpp"
 
@@ -254,7 +256,9 @@ __global__ void k_scan_add(uint32_t* __restrict__ out, size_t n, const uint32_t*
 
 // --------------------------------------------------------------------- accumulation
 // Thread k adds refs[kK, (k+1)K) into running XYZZ sums, one per bucket run, and
-// stores the sum for bucket b at partial slot k + b.  Slot k + b is written by
+// stores the sum for bucket b at partial slot k + b.  The running sums use the
+// radix-2^28 field form (g1_28.hpp: one v_mad_u64_u32 per limb product, no carry
+// instruction); points are read in the blst layout and partials written in it.  Slot k + b is written by
 // exactly the chunks k that hold entries of bucket b, so no initialisation is needed.
 // The grid covers the largest possible M; M itself is read from off[NB].
 __global__ void __launch_bounds__(64) k_msm_accum(const uint32_t* __restrict__ refs,
@@ -275,22 +279,32 @@ __global__ void __launch_bounds__(64) k_msm_accum(const uint32_t* __restrict__ r
   }
   uint32_t cb = lo;
   uint32_t next = off[cb + 1];
-  g1_xyzz_t acc = g1_xyzz_t::inf();
+  g1x28 acc = g1x28_inf();
   for (uint32_t e = e0; e < e1; e++) {
     if (e >= next) {
-      store_xyzz(partials + 48 * (size_t)(k + cb), acc);
-      acc = g1_xyzz_t::inf();
+      store_x28_raw(partials + MZK_X28_WORDS * (size_t)(k + cb), acc);
+      acc = g1x28_inf();
       do {
         cb++;
         next = off[cb + 1];
       } while (next <= e);
     }
     const uint32_t r = refs[e];
-    g1_aff_t q = load_affine(bases + 24 * (size_t)(r & ~NEG_FLAG));
-    if (r & NEG_FLAG) q = g1_neg(q);
-    acc = g1_add_affine(acc, q);
+    const uint4* q = reinterpret_cast<const uint4*>(bases + 24 * (size_t)(r & ~NEG_FLAG));
+    uint32_t xs[12], ys[12];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const uint4 a = q[i], b = q[3 + i];
+      xs[4 * i] = a.x; xs[4 * i + 1] = a.y; xs[4 * i + 2] = a.z; xs[4 * i + 3] = a.w;
+      ys[4 * i] = b.x; ys[4 * i + 1] = b.y; ys[4 * i + 2] = b.z; ys[4 * i + 3] = b.w;
+    }
+    uint32_t any = 0;
+#pragma unroll
+    for (int i = 0; i < 12; i++) any |= xs[i] | ys[i];
+    if (any == 0) continue;   // the affine identity adds nothing
+    acc = g1x28_madd(acc, f28_from_std(xs), f28_from_std(ys), (r & NEG_FLAG) != 0);
   }
-  store_xyzz(partials + 48 * (size_t)(k + cb), acc);
+  store_x28_raw(partials + MZK_X28_WORDS * (size_t)(k + cb), acc);
 }
 
 // Segment t of L buckets of one set (b = t L + i): S_b = sum of the chunk partials
@@ -302,29 +316,29 @@ __global__ void __launch_bounds__(64) k_bucket_seg(const uint32_t* __restrict__ 
                                                    uint32_t* __restrict__ A, uint32_t* __restrict__ R) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if ((size_t)t * L >= NB) return;
-  g1_xyzz_t run = g1_xyzz_t::inf(), acc = g1_xyzz_t::inf();
+  g1x28 run = g1x28_inf(), acc = g1x28_inf();
   for (uint32_t i = L; i-- > 0;) {
     const uint32_t b = t * L + i;
     const uint32_t lo = off[b], hi = off[b + 1];
     if (hi > lo) {
       const uint32_t k0 = lo / K, k1 = (hi - 1) / K;
-      for (uint32_t k = k0; k <= k1; k++) run = g1_add(run, load_xyzz(partials + 48 * (size_t)(k + b)));
+      for (uint32_t k = k0; k <= k1; k++) run = g1x28_add(run, load_x28(partials + MZK_X28_WORDS * (size_t)(k + b)));
     }
-    acc = g1_add(acc, run);
+    acc = g1x28_add(acc, run);
   }
-  store_xyzz(A + 48 * (size_t)t, acc);
-  store_xyzz(R + 48 * (size_t)t, run);
+  store_x28_raw(A + MZK_X28_WORDS * (size_t)t, acc);
+  store_x28_raw(R + MZK_X28_WORDS * (size_t)t, run);
 }
 
-// Block-wide sum of one XYZZ point per thread (LDS tree); thread 0 returns it.
-__device__ __forceinline__ g1_xyzz_t block_sum(g1_xyzz_t x, uint32_t* lds) {
+// Block-wide sum of one point per thread (LDS tree); thread 0 returns it.
+__device__ __forceinline__ g1x28 block_sum(g1x28 x, uint32_t* lds) {
   const uint32_t t = threadIdx.x;
-  store_xyzz(lds + 48 * t, x);
+  store_x28_raw(lds + MZK_X28_WORDS * t, x);
   __syncthreads();
   for (uint32_t d = blockDim.x / 2; d >= 1; d >>= 1) {
     if (t < d) {
-      x = g1_add(x, load_xyzz(lds + 48 * (t + d)));
-      store_xyzz(lds + 48 * t, x);
+      x = g1x28_add(x, load_x28(lds + MZK_X28_WORDS * (t + d)));
+      store_x28_raw(lds + MZK_X28_WORDS * t, x);
     }
     __syncthreads();
   }
@@ -343,22 +357,29 @@ __global__ void __launch_bounds__(CHAN_BLOCK) k_chan_sum(const uint32_t* __restr
   const uint32_t h = sh % nch, s = sh / nch;
   const uint32_t per = T / parts;
   const uint32_t* src = h == 0 ? A : R;
-  g1_xyzz_t acc = g1_xyzz_t::inf();
+  g1x28 acc = g1x28_inf();
   for (uint32_t t = q * per + threadIdx.x; t < (q + 1) * per; t += CHAN_BLOCK)
-    if (h == 0 || ((t >> (h - 1)) & 1u)) acc = g1_add(acc, load_xyzz(src + 48 * ((size_t)s * T + t)));
+    if (h == 0 || ((t >> (h - 1)) & 1u)) acc = g1x28_add(acc, load_x28(src + MZK_X28_WORDS * ((size_t)s * T + t)));
   acc = block_sum(acc, lds);
-  if (threadIdx.x == 0) store_xyzz(out + 48 * (size_t)blockIdx.x, acc);
+  if (threadIdx.x == 0) store_x28_raw(out + MZK_X28_WORDS * (size_t)blockIdx.x, acc);
 }
 
-// Sums `len` consecutive points per segment; one block per segment.
+// Sums `len` consecutive points per segment; one block per segment.  Writes the
+// blst-layout XYZZ (canonical 12 x 32-bit limbs) the host tail reads.
 __global__ void __launch_bounds__(CHAN_BLOCK) k_seg_sum(const uint32_t* __restrict__ in, uint32_t len,
                                                         uint32_t* __restrict__ out) {
   extern __shared__ uint32_t lds[];
-  g1_xyzz_t acc = g1_xyzz_t::inf();
+  g1x28 acc = g1x28_inf();
   for (uint32_t i = threadIdx.x; i < len; i += blockDim.x)
-    acc = g1_add(acc, load_xyzz(in + 48 * ((size_t)blockIdx.x * len + i)));
+    acc = g1x28_add(acc, load_x28(in + MZK_X28_WORDS * ((size_t)blockIdx.x * len + i)));
   acc = block_sum(acc, lds);
-  if (threadIdx.x == 0) store_xyzz(out + 48 * (size_t)blockIdx.x, acc);
+  if (threadIdx.x == 0) {
+    uint32_t o[48];
+    g1x28_to_std(acc, o);
+    uint4* qv = reinterpret_cast<uint4*>(out + 48 * (size_t)blockIdx.x);
+#pragma unroll
+    for (int i = 0; i < 12; i++) qv[i] = make_uint4(o[4 * i], o[4 * i + 1], o[4 * i + 2], o[4 * i + 3]);
+  }
 }
 
 // --------------------------------------------------------------------- host side
@@ -533,18 +554,19 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   uint32_t* tmp_key = (uint32_t*)c->scratch[SCR_MSM_B].ptr;
   uint32_t* tmp_ref = tmp_key + M_max;
   uint32_t* refs = tmp_ref + M_max;
-  // scratch C: chunk partials (nchunks + NB slots) then segment sums A, R (sets * T each)
+  // scratch C: chunk partials (nchunks + NB slots) then segment sums A, R (sets * T each),
+  // all in the radix-2^28 point form (MZK_X28_WORDS words each)
   const size_t nseg = sets * T;
-  size_t c_words = (nchunks + NB) * 48 + 2 * nseg * 48;
+  size_t c_words = (nchunks + NB) * MZK_X28_WORDS + 2 * nseg * MZK_X28_WORDS;
   MZK_TRY(c->scratch[SCR_MSM_C].reserve(c_words * 4));
   uint32_t* partials = (uint32_t*)c->scratch[SCR_MSM_C].ptr;
-  uint32_t* segA = partials + (nchunks + NB) * 48;
-  uint32_t* segR = segA + nseg * 48;
+  uint32_t* segA = partials + (nchunks + NB) * MZK_X28_WORDS;
+  uint32_t* segR = segA + nseg * MZK_X28_WORDS;
   // scratch D: channel partials and channel sums
-  size_t d_words = (nchan * parts + nchan) * 48 + 64;
+  size_t d_words = nchan * parts * MZK_X28_WORDS + nchan * 48 + 64;
   MZK_TRY(c->scratch[SCR_MSM_D].reserve(d_words * 4));
   uint32_t* chan_parts = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
-  uint32_t* chan = chan_parts + nchan * parts * 48;
+  uint32_t* chan = chan_parts + nchan * parts * MZK_X28_WORDS;
 
   MZK_LAUNCH(c, "k_sort_hist", k_sort_hist, dim3(sa.G), dim3(SORT_BLOCK), sa.D1 * 4, sa, canon, hist);
   MZK_LAUNCH(c, "k_scan_tiles", k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, hist, hoff, HN, tsums);
@@ -558,9 +580,9 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
              partials);
   MZK_LAUNCH(c, "k_bucket_seg", k_bucket_seg, dim3(grid_for(nseg, 64)), dim3(64), 0, off, NB, K, L, partials, segA,
              segR);
-  MZK_LAUNCH(c, "k_chan_sum", k_chan_sum, dim3((unsigned)(nchan * parts)), dim3(CHAN_BLOCK), CHAN_BLOCK * 48 * 4, segA,
+  MZK_LAUNCH(c, "k_chan_sum", k_chan_sum, dim3((unsigned)(nchan * parts)), dim3(CHAN_BLOCK), CHAN_BLOCK * MZK_X28_WORDS * 4, segA,
              segR, T, nch, parts, chan_parts);
-  MZK_LAUNCH(c, "k_seg_sum", k_seg_sum, dim3((unsigned)nchan), dim3(64), 64 * 48 * 4, chan_parts, parts, chan);
+  MZK_LAUNCH(c, "k_seg_sum", k_seg_sum, dim3((unsigned)nchan), dim3(64), 64 * MZK_X28_WORDS * 4, chan_parts, parts, chan);
   MZK_HIP(hipGetLastError());
 
   std::vector<uint32_t> h(nchan * 48);
