@@ -79,6 +79,13 @@ int mzk_srs_generate(mzk_ctx *ctx, const uint64_t tau[4], size_t n, mzk_srs **ou
 /* Bases [s_i] G1 for n device scalars s_i: the Lagrange basis of unsafe_setup
  * (params.rs:174-186, g_lagrange_i = [L_i(tau)] G1) takes s_i = L_i(tau). */
 int mzk_srs_from_scalars_dev(mzk_ctx *ctx, const void *d_scalars, size_t n, mzk_srs **out);
+/* g_to_lagrange (proofs/src/utils/arithmetic.rs:34-54): the Lagrange bases of a
+ * monomial SRS without its trapdoor -- the inverse DFT over G1 of g[0, 2^k) with
+ * omega = ROOT^(2^(32-k)), times 1/2^k, normalised to affine.  A new SRS of 2^k bases. */
+int mzk_srs_to_lagrange(mzk_ctx *ctx, const mzk_srs *g, uint32_t k, mzk_srs **out);
+/* suffix_sum (proofs/src/poly/kzg/params.rs:47-57): out_i = sum_(j >= i) in_j, the
+ * g_lagrange_delta bases (and, applied to those, g_lagrange_double_delta). */
+int mzk_srs_suffix_sum(mzk_ctx *ctx, const mzk_srs *in, mzk_srs **out);
 size_t mzk_srs_len(const mzk_srs *srs);
 /* Device pointer to the n x 12 u64 affine bases. */
 const void *mzk_srs_bases_dev(const mzk_srs *srs);

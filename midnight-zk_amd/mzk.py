@@ -107,6 +107,8 @@ SIGNATURES = {
     "mzk_prof_read": ([_P, ctypes.c_char_p, _S], _I),
     "mzk_msm_last_plan": ([_P, _P], _I),
     "mzk_srs_from_scalars_dev": ([_P, _P, _S, ctypes.POINTER(_P)], _I),
+    "mzk_srs_to_lagrange": ([_P, _P, _U32, ctypes.POINTER(_P)], _I),
+    "mzk_srs_suffix_sum": ([_P, _P, ctypes.POINTER(_P)], _I),
     "mzk_fr_vec_op": ([_P, _I, _P, _P, _P, _S], _I),
     "mzk_fr_vec_scalar": ([_P, _I, _P, _P, _P, _S], _I),
     "mzk_fr_batch_invert": ([_P, _P, _S], _I),
@@ -355,6 +357,18 @@ class ResidentSrs:
         out = np.zeros((count, 12), dtype=np.uint64)
         _check(load_library().mzk_srs_download(self.h, offset, count, _np_ptr(out)))
         return out
+
+    def to_lagrange(self, k: int) -> "ResidentSrs":
+        """g_to_lagrange (utils/arithmetic.rs:34-54): Lagrange bases of the first 2^k bases."""
+        h = ctypes.c_void_p()
+        _check(load_library().mzk_srs_to_lagrange(self.ctx.h, self.h, k, ctypes.byref(h)))
+        return ResidentSrs(self.ctx, h)
+
+    def suffix_sum(self) -> "ResidentSrs":
+        """suffix_sum (kzg/params.rs:47-57): the delta bases of these bases."""
+        h = ctypes.c_void_p()
+        _check(load_library().mzk_srs_suffix_sum(self.ctx.h, self.h, ctypes.byref(h)))
+        return ResidentSrs(self.ctx, h)
 
     def bases_ptr(self) -> int:
         return load_library().mzk_srs_bases_dev(self.h)
