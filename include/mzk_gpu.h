@@ -190,6 +190,39 @@ int mzk_fr_kate_div(mzk_ctx *ctx, const void *d_a, size_t n, const uint64_t b[4]
 /* out_i = x^i for i < n (powers, kzg/mod.rs:151-167 and the domain's omega tables). */
 int mzk_fr_powers(mzk_ctx *ctx, const uint64_t x[4], size_t n, void *d_out);
 
+/* ------------------------------------------------------------------ quotient numerator
+ * evaluate_numerator (proofs/src/plonk/evaluation.rs:818-940) fused with
+ * divide_by_vanishing_poly (proofs/src/poly/domain.rs:361-380) for a circuit whose
+ * only gate is the standard one, q_m a b + q_l a + q_r b - q_o c + q_c, and whose
+ * permutation argument covers ncols columns in sets of `chunk`: one pass over the
+ * extended-domain (coset) evaluations, element i of
+ *   (((gate y + (1 - z_0) l_0) y + (z_l^2 - z_l) l_last) y + sum over sets s > 0 of
+ *    (z_s - z_(s-1)(omega^last X)) l_0 ... ) y + (left_s - right_s) l_active ...
+ * times t_evals[i mod t_count], in the reference's order of identities and powers of y:
+ *   left_s  = z_s(omega X) prod_j (col_j + beta sigma_j + gamma),
+ *   right_s = z_s(X) prod_j (col_j + beta delta^j X + gamma),
+ * with X = coset_x[i] (zeta omega_ext^i).  Rotations are index shifts on the extended
+ * domain.  All vectors are n_ext Montgomery scalars; scalars are Montgomery. */
+typedef struct {
+  uint64_t n_ext;
+  uint32_t rot_next;      /* index shift of omega X: 2^(extended_k - k) */
+  uint32_t rot_last;      /* index shift of omega^-(blinding+1) X, as a positive count */
+  uint32_t ncols, nsets, chunk, t_count;
+  const void *a, *b, *c, *qm, *ql, *qr, *qo, *qc;
+  const void *col[8];
+  const void *sigma[8];
+  const void *z[4];
+  const void *l0, *l_last, *l_active, *coset_x;
+  uint64_t y[4], beta[4], gamma[4];
+  uint64_t delta_beta[8][4];   /* beta delta^j for column j */
+  uint64_t t_evals[16][4];     /* 1 / ((zeta omega_ext^i)^n - 1), i < t_count */
+} mzk_plonk_numerator;
+int mzk_plonk_numerator_dev(mzk_ctx *ctx, const mzk_plonk_numerator *args, void *d_out);
+/* sum_j s_j p_j (zero-extended to the longest): poly_inner_product (kzg/mod.rs:125-147)
+ * in one launch for up to 64 polynomials; out has max_j len_j elements. */
+int mzk_fr_lincomb_dev(mzk_ctx *ctx, const void *const *d_polys, const size_t *lens, const uint64_t *scalars,
+                       size_t npolys, void *d_out, size_t out_len);
+
 /* ------------------------------------------------------------------ synthetic inputs
  * Element i of the seeded stream (seed, stream): eight splitmix64 words
  * w_j = splitmix64(key + 8 (offset + i) + j), key = splitmix64(seed ^ stream * 0xD1B54A32D192ED03),

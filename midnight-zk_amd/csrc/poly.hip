@@ -340,6 +340,76 @@ static Pows make_pows(const host::Fr& x) {
   return P;
 }
 
+// ---------------------------------------------------------------- quotient numerator
+struct NumArgs {
+  uint64_t N;
+  uint32_t rot_next, rot_last, ncols, nsets, chunk, tmask;
+  const uint32_t *a, *b, *c, *qm, *ql, *qr, *qo, *qc;
+  const uint32_t* col[8];
+  const uint32_t* sigma[8];
+  const uint32_t* z[4];
+  const uint32_t *l0, *l_last, *l_active, *coset_x;
+  FrArg y, beta, gamma;
+  FrArg delta_beta[8];
+  FrArg t_evals[16];
+};
+
+__global__ void __launch_bounds__(256) k_plonk_numerator(NumArgs A, uint32_t* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.N) return;
+  const size_t N = A.N;
+  const size_t in = (i + A.rot_next) & (N - 1);
+  const size_t il = (i + N - A.rot_last) & (N - 1);
+  const fr_t y = arg_fr(A.y), beta = arg_fr(A.beta), gamma = arg_fr(A.gamma);
+  const fr_t a = load<FrP>(A.a + 8 * i), b = load<FrP>(A.b + 8 * i), c = load<FrP>(A.c + 8 * i);
+  // gate: a b q_m + a q_l + b q_r - c q_o + q_c
+  fr_t v = add(add(add(mul(mul(a, b), load<FrP>(A.qm + 8 * i)), mul(a, load<FrP>(A.ql + 8 * i))),
+                   mul(b, load<FrP>(A.qr + 8 * i))),
+               sub(load<FrP>(A.qc + 8 * i), mul(c, load<FrP>(A.qo + 8 * i))));
+  const fr_t l0 = load<FrP>(A.l0 + 8 * i);
+  // l_0 (1 - z_0)
+  v = add(mul(v, y), mul(sub(fr_t::one(), load<FrP>(A.z[0] + 8 * i)), l0));
+  // l_last (z_l^2 - z_l)
+  const fr_t zl = load<FrP>(A.z[A.nsets - 1] + 8 * i);
+  v = add(mul(v, y), mul(sub(sqr(zl), zl), load<FrP>(A.l_last + 8 * i)));
+  // l_0 (z_s - z_(s-1)(omega^last X))
+  for (uint32_t s = 1; s < A.nsets; s++)
+    v = add(mul(v, y), mul(sub(load<FrP>(A.z[s] + 8 * i), load<FrP>(A.z[s - 1] + 8 * il)), l0));
+  const fr_t x = load<FrP>(A.coset_x + 8 * i);
+  const fr_t lact = load<FrP>(A.l_active + 8 * i);
+  for (uint32_t s = 0; s < A.nsets; s++) {
+    fr_t left = load<FrP>(A.z[s] + 8 * in);
+    fr_t right = load<FrP>(A.z[s] + 8 * i);
+    const uint32_t j0 = s * A.chunk, j1 = j0 + A.chunk < A.ncols ? j0 + A.chunk : A.ncols;
+    for (uint32_t j = j0; j < j1; j++) {
+      const fr_t col = load<FrP>(A.col[j] + 8 * i);
+      left = mul(left, add(add(col, mul(load<FrP>(A.sigma[j] + 8 * i), beta)), gamma));
+      right = mul(right, add(add(col, mul(x, arg_fr(A.delta_beta[j]))), gamma));
+    }
+    v = add(mul(v, y), mul(sub(left, right), lact));
+  }
+  store<FrP>(out + 8 * i, mul(v, arg_fr(A.t_evals[i & A.tmask])));
+}
+
+// sum_j s_j p_j over up to LC_MAX polynomials (zero-extended)
+static constexpr int LC_MAX = 64;
+struct LcArgs {
+  const uint32_t* p[LC_MAX];
+  uint64_t len[LC_MAX];
+  FrArg s[LC_MAX];
+  uint32_t np;
+};
+
+// (the argument block, 3 KiB, travels by value in the kernel arguments)
+__global__ void __launch_bounds__(256) k_lincomb(LcArgs A, uint32_t* __restrict__ out, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  fr_t acc = fr_t::zero();
+  for (uint32_t j = 0; j < A.np; j++)
+    if (i < A.len[j]) acc = add(acc, mul(load<FrP>(A.p[j] + 8 * i), arg_fr(A.s[j])));
+  store<FrP>(out + 8 * i, acc);
+}
+
 }  // namespace mzk
 
 using namespace mzk;
@@ -517,6 +587,83 @@ int mzk_fr_kate_div(mzk_ctx* c, const void* a, size_t n, const uint64_t b[4], vo
   MZK_LAUNCH(c, "k_kate_carries", k_kate_carries, dim3(1), dim3(256), 0, loc, C, to_arg(gL), carry);
   MZK_LAUNCH(c, "k_kate_final", k_kate_final, dim3(grid_for(C, 64)), dim3(64), 0, (const uint32_t*)a, m, to_arg(b),
              carry, (uint32_t*)q);
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
+}
+
+int mzk_plonk_numerator_dev(mzk_ctx* c, const mzk_plonk_numerator* a, void* out) {
+  MZK_REQUIRE(c != nullptr && a != nullptr && out != nullptr, "mzk_plonk_numerator_dev: null argument");
+  const uint64_t N = a->n_ext;
+  MZK_REQUIRE(N >= 2 && (N & (N - 1)) == 0, "mzk_plonk_numerator_dev: n_ext must be a power of two");
+  MZK_REQUIRE(a->nsets >= 1 && a->nsets <= 4 && a->ncols >= 1 && a->ncols <= 8 && a->chunk >= 1 &&
+                  (size_t)a->nsets * a->chunk >= a->ncols && (size_t)(a->nsets - 1) * a->chunk < a->ncols,
+              "mzk_plonk_numerator_dev: permutation shape (%u columns, %u sets of %u) unsupported", a->ncols,
+              a->nsets, a->chunk);
+  MZK_REQUIRE(a->t_count >= 1 && a->t_count <= 16 && (a->t_count & (a->t_count - 1)) == 0,
+              "mzk_plonk_numerator_dev: t_count must be a power of two <= 16");
+  MZK_REQUIRE(a->rot_next < N && a->rot_last < N, "mzk_plonk_numerator_dev: rotation out of range");
+  const void* need[] = {a->a, a->b, a->c, a->qm, a->ql, a->qr, a->qo, a->qc, a->l0, a->l_last, a->l_active,
+                        a->coset_x};
+  for (const void* p : need) MZK_REQUIRE(p != nullptr, "mzk_plonk_numerator_dev: null vector");
+  for (uint32_t j = 0; j < a->ncols; j++)
+    MZK_REQUIRE(a->col[j] && a->sigma[j], "mzk_plonk_numerator_dev: null permutation column %u", j);
+  for (uint32_t s = 0; s < a->nsets; s++) MZK_REQUIRE(a->z[s], "mzk_plonk_numerator_dev: null z %u", s);
+  NumArgs A;
+  memset(&A, 0, sizeof A);
+  A.N = N;
+  A.rot_next = a->rot_next;
+  A.rot_last = a->rot_last;
+  A.ncols = a->ncols;
+  A.nsets = a->nsets;
+  A.chunk = a->chunk;
+  A.tmask = a->t_count - 1;
+  A.a = (const uint32_t*)a->a;
+  A.b = (const uint32_t*)a->b;
+  A.c = (const uint32_t*)a->c;
+  A.qm = (const uint32_t*)a->qm;
+  A.ql = (const uint32_t*)a->ql;
+  A.qr = (const uint32_t*)a->qr;
+  A.qo = (const uint32_t*)a->qo;
+  A.qc = (const uint32_t*)a->qc;
+  for (uint32_t j = 0; j < a->ncols; j++) {
+    A.col[j] = (const uint32_t*)a->col[j];
+    A.sigma[j] = (const uint32_t*)a->sigma[j];
+    A.delta_beta[j] = to_arg(a->delta_beta[j]);
+  }
+  for (uint32_t s = 0; s < a->nsets; s++) A.z[s] = (const uint32_t*)a->z[s];
+  A.l0 = (const uint32_t*)a->l0;
+  A.l_last = (const uint32_t*)a->l_last;
+  A.l_active = (const uint32_t*)a->l_active;
+  A.coset_x = (const uint32_t*)a->coset_x;
+  A.y = to_arg(a->y);
+  A.beta = to_arg(a->beta);
+  A.gamma = to_arg(a->gamma);
+  for (uint32_t t = 0; t < a->t_count; t++) A.t_evals[t] = to_arg(a->t_evals[t]);
+  MZK_TRY(vec_ready(c));
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_LAUNCH(c, "k_plonk_numerator", k_plonk_numerator, dim3(grid_for(N, 256)), dim3(256), 0, A, (uint32_t*)out);
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
+}
+
+int mzk_fr_lincomb_dev(mzk_ctx* c, const void* const* polys, const size_t* lens, const uint64_t* scalars,
+                       size_t np, void* out, size_t out_len) {
+  MZK_REQUIRE(c != nullptr, "null context");
+  MZK_REQUIRE(np <= (size_t)LC_MAX, "mzk_fr_lincomb_dev: at most %d polynomials", LC_MAX);
+  if (out_len == 0) return MZK_OK;
+  MZK_REQUIRE(out != nullptr && (np == 0 || (polys && lens && scalars)), "mzk_fr_lincomb_dev: null argument");
+  LcArgs A;
+  memset(&A, 0, sizeof A);
+  A.np = (uint32_t)np;
+  for (size_t j = 0; j < np; j++) {
+    MZK_REQUIRE(lens[j] == 0 || polys[j] != nullptr, "mzk_fr_lincomb_dev: polynomial %zu has no data", j);
+    A.p[j] = (const uint32_t*)polys[j];
+    A.len[j] = lens[j] < out_len ? lens[j] : out_len;
+    A.s[j] = to_arg(scalars + 4 * j);
+  }
+  MZK_TRY(vec_ready(c));
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_LAUNCH(c, "k_lincomb", k_lincomb, dim3(grid_for(out_len, 256)), dim3(256), 0, A, (uint32_t*)out, out_len);
   MZK_HIP(hipGetLastError());
   return MZK_OK;
 }

@@ -57,6 +57,20 @@ class NttJob(ctypes.Structure):
         return j
 
 
+class PlonkNumerator(ctypes.Structure):
+    """mzk_plonk_numerator: the fused quotient-numerator arguments."""
+    _fields_ = [("n_ext", ctypes.c_uint64), ("rot_next", ctypes.c_uint32), ("rot_last", ctypes.c_uint32),
+                ("ncols", ctypes.c_uint32), ("nsets", ctypes.c_uint32), ("chunk", ctypes.c_uint32),
+                ("t_count", ctypes.c_uint32),
+                ("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("c", ctypes.c_void_p), ("qm", ctypes.c_void_p),
+                ("ql", ctypes.c_void_p), ("qr", ctypes.c_void_p), ("qo", ctypes.c_void_p), ("qc", ctypes.c_void_p),
+                ("col", ctypes.c_void_p * 8), ("sigma", ctypes.c_void_p * 8), ("z", ctypes.c_void_p * 4),
+                ("l0", ctypes.c_void_p), ("l_last", ctypes.c_void_p), ("l_active", ctypes.c_void_p),
+                ("coset_x", ctypes.c_void_p),
+                ("y", ctypes.c_uint64 * 4), ("beta", ctypes.c_uint64 * 4), ("gamma", ctypes.c_uint64 * 4),
+                ("delta_beta", (ctypes.c_uint64 * 4) * 8), ("t_evals", (ctypes.c_uint64 * 4) * 16)]
+
+
 _lib = None
 
 # (name, argtypes, restype); every int-returning function is a status code
@@ -115,6 +129,8 @@ SIGNATURES = {
     "mzk_fr_prefix_product": ([_P, _P, _P, _S], _I),
     "mzk_fr_eval": ([_P, _P, _S, _P, _P], _I),
     "mzk_fr_eval_batch": ([_P, _P, _P, _P, _S, _P], _I),
+    "mzk_plonk_numerator_dev": ([_P, _P, _P], _I),
+    "mzk_fr_lincomb_dev": ([_P, _P, _P, _P, _S, _P, _S], _I),
     "mzk_fr_kate_div": ([_P, _P, _S, _P, _P], _I),
     "mzk_fr_powers": ([_P, _P, _S, _P], _I),
 }

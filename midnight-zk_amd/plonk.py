@@ -317,6 +317,47 @@ def _powers(x: int, m: int):
     return out
 
 
+def numerator_composed(be, pk: "ProvingKey", adv_c, z_c, y: int, beta: int, gamma: int):
+    """evaluate_numerator (evaluation.rs:818-940) for this circuit from element-wise
+    vector operations: the gate, then the permutation identities in the reference's
+    order, each step v <- v y + term."""
+    bf = BLINDING_FACTORS
+    scale = pk.ext_n // pk.n
+    sets = [list(range(0, CHUNK_LEN)), list(range(CHUNK_LEN, NUM_ADVICE))]
+    qm, ql, qr, qo, qc = pk.fixed_cosets
+    a, b, c = adv_c[0], adv_c[1], adv_c[2]
+    val = be.add(be.add(be.add(be.mul(be.mul(a, b), qm), be.mul(a, ql)), be.mul(b, qr)),
+                 be.sub(qc, be.mul(c, qo)))
+    rot_next = scale
+    rot_last = -(bf + 1) * scale
+    # l0 (1 - z_0)
+    val = be.add(be.scale(val, y), be.mul(be.add_scalar(be.neg(z_c[0]), 1), pk.l0))
+    # l_last (z_l^2 - z_l)
+    zl = z_c[-1]
+    val = be.add(be.scale(val, y), be.mul(be.sub(be.mul(zl, zl), zl), pk.l_last))
+    # l0 (z_i - z_(i-1)(omega^last X))
+    for s in range(1, len(z_c)):
+        val = be.add(be.scale(val, y), be.mul(be.sub(z_c[s], be.roll(z_c[s - 1], rot_last)), pk.l0))
+    # l_active (z(omega X) prod(p + beta s + gamma) - z(X) prod(p + delta^j beta X + gamma))
+    for s, cols in enumerate(sets):
+        left = be.roll(z_c[s], rot_next)
+        right = z_c[s]
+        for j in cols:
+            left = be.mul(left, be.add_scalar(be.add(adv_c[j], be.scale(pk.sigma_cosets[j], beta)), gamma))
+            dx = be.scale(pk.coset_x, beta * pow(DELTA, j, R_MOD) % R_MOD)
+            right = be.mul(right, be.add_scalar(be.add(adv_c[j], dx), gamma))
+        val = be.add(be.scale(val, y), be.mul(be.sub(left, right), pk.l_active_row))
+    return val
+
+
+def t_evaluations(k: int, ek: int):
+    """1 / ((zeta omega_ext^i)^n - 1) for i < 2^(ek - k) (domain.rs:118-141)."""
+    n = 1 << k
+    w = omega_for(ek)
+    return [pow((pow(ZETA * pow(w, i, R_MOD) % R_MOD, n, R_MOD) - 1) % R_MOD, -1, R_MOD)
+            for i in range(1 << (ek - k))]
+
+
 # Requests a proof yields to whoever drives it; the driver answers each with a list of
 # results in order.  ("commit", [(srs, vector)]) -> affine points;
 # ("ntt", [(op, vector, k, extended_k)]) -> vectors; ("eval", [(vector, x)]) -> ints.
@@ -391,39 +432,21 @@ def prove_steps(be, params: Params, pk: ProvingKey, advice_values, rng_seed: int
     advice_polys, z_polys = polys[:NUM_ADVICE], polys[NUM_ADVICE:]
     mark("advice_ifft")
 
-    # quotient numerator on the extended coset (evaluation.rs:818-940)
+    # quotient numerator on the extended coset (evaluation.rs:818-940), divided by X^n - 1
     cosets = yield ("ntt", [("coeff_to_extended", p, k, ek) for p in advice_polys + z_polys])
     adv_c, z_c = cosets[:NUM_ADVICE], cosets[NUM_ADVICE:]
     mark("coset_ntt")
-    qm, ql, qr, qo, qc = pk.fixed_cosets
-    a, b, c = adv_c[0], adv_c[1], adv_c[2]
-    val = be.add(be.add(be.add(be.mul(be.mul(a, b), qm), be.mul(a, ql)), be.mul(b, qr)),
-                 be.sub(qc, be.mul(c, qo)))
-    rot_next = scale
-    rot_last = -(bf + 1) * scale
-    # l0 (1 - z_0)
-    val = be.add(be.scale(val, y), be.mul(be.add_scalar(be.neg(z_c[0]), 1), pk.l0))
-    # l_last (z_l^2 - z_l)
-    zl = z_c[-1]
-    val = be.add(be.scale(val, y), be.mul(be.sub(be.mul(zl, zl), zl), pk.l_last))
-    # l0 (z_i - z_(i-1)(omega^last X))
-    for s in range(1, len(z_c)):
-        val = be.add(be.scale(val, y), be.mul(be.sub(z_c[s], be.roll(z_c[s - 1], rot_last)), pk.l0))
-    # l_active (z(omega X) prod(p + beta s + gamma) - z(X) prod(p + delta^j beta X + gamma))
-    for s, cols in enumerate(sets):
-        left = be.roll(z_c[s], rot_next)
-        right = z_c[s]
-        for j in cols:
-            left = be.mul(left, be.add_scalar(be.add(adv_c[j], be.scale(pk.sigma_cosets[j], beta)), gamma))
-            dx = be.scale(pk.coset_x, beta * pow(DELTA, j, R_MOD) % R_MOD)
-            right = be.mul(right, be.add_scalar(be.add(adv_c[j], dx), gamma))
-        val = be.add(be.scale(val, y), be.mul(be.sub(left, right), pk.l_active_row))
+    fused = getattr(be, "quotient_numerator", None)
+    if fused is not None:
+        h_ext = fused(pk, adv_c, z_c, y, beta, gamma)
+    else:
+        h_ext = be.divide_by_vanishing(numerator_composed(be, pk, adv_c, z_c, y, beta, gamma), k, ek)
     del adv_c, z_c, cosets
     mark("numerator")
 
     # h = nu / (X^n - 1), limbs (prover.rs:701-772)
-    (h,) = yield ("ntt", [("extended_to_coeff", be.divide_by_vanishing(val, k, ek), k, ek)])
-    del val
+    (h,) = yield ("ntt", [("extended_to_coeff", h_ext, k, ek)])
+    del h_ext
     h = be.slice(h, 0, (n - 1) * QUOTIENT_DEGREE)
     limb_blind = draws(QUOTIENT_DEGREE - 1)
     limbs = []
@@ -732,12 +755,57 @@ class GpuBackend:
         return out
 
     def inner_product(self, polys, scalars):
-        """poly_inner_product (kzg/mod.rs:125-147): sum_i s_i p_i, zero-extended."""
+        """poly_inner_product (kzg/mod.rs:125-147): sum_i s_i p_i, zero-extended, one
+        launch per 64 polynomials."""
+        import ctypes
+        polys, scalars = list(polys), list(scalars)[:len(polys)]
         m = max(p.shape[0] for p in polys)
-        acc = self.zeros(m)
-        for p, s in zip(polys, scalars):
-            self._scalar(2, p, s, acc[:p.shape[0]])
-        return acc
+        out = None
+        for lo in range(0, len(polys), 64):
+            part, sc = polys[lo:lo + 64], scalars[lo:lo + 64]
+            q = len(part)
+            ptrs = (ctypes.c_void_p * q)(*[self._p(p) if p.shape[0] else None for p in part])
+            lens = (ctypes.c_size_t * q)(*[p.shape[0] for p in part])
+            ss = np.concatenate([mont_limbs(x) for x in sc])
+            res = self._empty(m)
+            self.mzk._check(self._L.mzk_fr_lincomb_dev(self.ctx.h, ctypes.cast(ptrs, ctypes.c_void_p),
+                                                       ctypes.cast(lens, ctypes.c_void_p), ss.ctypes.data, q,
+                                                       self._p(res), m))
+            out = res if out is None else self.add(out, res)
+        return out
+
+    def quotient_numerator(self, pk, adv_c, z_c, y: int, beta: int, gamma: int):
+        """The numerator of the quotient on the extended coset, divided by X^n - 1, in one
+        kernel (mzk_plonk_numerator_dev); equal to numerator_composed followed by
+        divide_by_vanishing."""
+        mzk = self.mzk
+        A = mzk.PlonkNumerator()
+        k, ek = pk.k, pk.dom.extended_k
+        A.n_ext = pk.ext_n
+        A.rot_next = pk.ext_n // pk.n
+        A.rot_last = (BLINDING_FACTORS + 1) * (pk.ext_n // pk.n)
+        A.ncols, A.nsets, A.chunk = NUM_ADVICE, 2, CHUNK_LEN
+        te = t_evaluations(k, ek)
+        A.t_count = len(te)
+        A.a, A.b, A.c = self._p(adv_c[0]), self._p(adv_c[1]), self._p(adv_c[2])
+        A.qm, A.ql, A.qr, A.qo, A.qc = (self._p(v) for v in pk.fixed_cosets)
+        for j in range(NUM_ADVICE):
+            A.col[j] = self._p(adv_c[j])
+            A.sigma[j] = self._p(pk.sigma_cosets[j])
+            A.delta_beta[j][:] = [int(w) for w in mont_limbs(beta * pow(DELTA, j, R_MOD) % R_MOD)]
+        for s in range(2):
+            A.z[s] = self._p(z_c[s])
+        A.l0, A.l_last, A.l_active, A.coset_x = (self._p(pk.l0), self._p(pk.l_last), self._p(pk.l_active_row),
+                                                 self._p(pk.coset_x))
+        A.y[:] = [int(w) for w in mont_limbs(y)]
+        A.beta[:] = [int(w) for w in mont_limbs(beta)]
+        A.gamma[:] = [int(w) for w in mont_limbs(gamma)]
+        for i, t in enumerate(te):
+            A.t_evals[i][:] = [int(w) for w in mont_limbs(t)]
+        out = self._empty(pk.ext_n)
+        import ctypes
+        mzk._check(self._L.mzk_plonk_numerator_dev(self.ctx.h, ctypes.addressof(A), self._p(out)))
+        return out
 
     def sub_scalar_at0(self, a, v: int):
         """a(X) - v (subtracts from the constant coefficient)."""
