@@ -119,30 +119,100 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_hist(SortArgs a, uint32_t* 
   for (uint32_t b = threadIdx.x; b < a.D1; b += SORT_BLOCK) hist[(size_t)b * a.G + blockIdx.x] = cnt[b];
 }
 
-// 3. scatter into bins: tmp_key[pos] = key & (2^l2 - 1), tmp_ref[pos] = reference | sign
+// 3. scatter into bins: tmp_key[pos] = key & (2^l2 - 1), tmp_ref[pos] = reference | sign.
+//    Each round stages the digits of SORT_BLOCK scalars x SCAT_WG windows in LDS in bin
+//    order, then writes each bin's entries of the round as one contiguous run (one
+//    scattered 4-byte store per digit would make this kernel bound by partial-line
+//    HBM writes).
+static constexpr int SCAT_WG = 8;
+static constexpr int SCAT_SLOTS = SORT_BLOCK * SCAT_WG;
+
 __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __restrict__ canon, SortArgs a,
                                                              const uint32_t* __restrict__ hoff,
                                                              uint32_t* __restrict__ tmp_key,
                                                              uint32_t* __restrict__ tmp_ref) {
-  extern __shared__ uint32_t cur[];   // D1 cursors
-  for (uint32_t b = threadIdx.x; b < a.D1; b += SORT_BLOCK) cur[b] = hoff[(size_t)b * a.G + blockIdx.x];
-  __syncthreads();
+  extern __shared__ uint32_t sm[];
+  uint32_t* cur = sm;                   // D1: next output position of each bin
+  uint32_t* cnt = cur + a.D1;           // D1: entries of the round per bin
+  uint32_t* offs = cnt + a.D1;          // D1: LDS offset of each bin's entries in the round
+  uint32_t* rank = offs + a.D1;         // SCAT_SLOTS: rank of each (thread, window) slot in its bin
+  uint32_t* skey = rank + SCAT_SLOTS;   // staged keys, bin order
+  uint32_t* sref = skey + SCAT_SLOTS;   // staged references, bin order
+  uint32_t* psum = sref + SCAT_SLOTS;   // SORT_BLOCK partial sums of the offset scan
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t b = tid; b < a.D1; b += SORT_BLOCK) cur[b] = hoff[(size_t)b * a.G + blockIdx.x];
   const uint32_t mask2 = (1u << a.l2) - 1u;
   const TileDesc tl = a.tiles[blockIdx.x];
   const JobDesc j = a.jobs[tl.job];
-  for (uint32_t i = tl.lo + threadIdx.x; i < tl.hi; i += SORT_BLOCK) {
-    fr_t s = load<FrP>(canon + 8 * ((size_t)j.canon_off + i));
-    uint32_t carry = 0;
-    for (int w = 0; w < a.W; w++) {
-      int d = window_digit(s.v, w, a.c, carry);
-      if (d != 0) {
-        const uint32_t key = bucket_key(a, j, w, d);
-        uint32_t ref = j.ref_offset + i;
-        if (!a.per_window) ref += (uint32_t)w * a.ref_stride;
-        uint32_t pos = atomicAdd(&cur[key >> a.l2], 1u);
-        tmp_key[pos] = key & mask2;
-        tmp_ref[pos] = ref | (d < 0 ? NEG_FLAG : 0u);
+  const uint32_t per_t = (a.D1 + SORT_BLOCK - 1) / SORT_BLOCK;   // bins per thread in the scan
+  const uint32_t b0 = tid * per_t, b1 = b0 + per_t < a.D1 ? b0 + per_t : a.D1;
+  for (uint32_t base = tl.lo; base < tl.hi; base += SORT_BLOCK) {
+    const uint32_t i = base + tid;
+    const bool live = i < tl.hi;
+    fr_t s = fr_t::zero();
+    if (live) s = load<FrP>(canon + 8 * ((size_t)j.canon_off + i));
+    uint32_t carry0 = 0;   // recoding carry entering window w0
+    for (int w0 = 0; w0 < a.W; w0 += SCAT_WG) {
+      for (uint32_t b = tid; b < a.D1; b += SORT_BLOCK) cnt[b] = 0;
+      __syncthreads();
+      // pass 1: each digit's rank within its bin
+      uint32_t carry = carry0;
+      for (int wl = 0; wl < SCAT_WG; wl++) {
+        const int w = w0 + wl;
+        if (!live || w >= a.W) break;
+        const int d = window_digit(s.v, w, a.c, carry);
+        if (d != 0) rank[tid * SCAT_WG + wl] = atomicAdd(&cnt[bucket_key(a, j, w, d) >> a.l2], 1u);
       }
+      __syncthreads();
+      // exclusive scan of the counts: per-thread ranges, then the SORT_BLOCK partials
+      uint32_t part = 0;
+      for (uint32_t b = b0; b < b1; b++) part += cnt[b];
+      psum[tid] = part;
+      __syncthreads();
+      if (tid == 0) {
+        uint32_t run = 0;
+        for (uint32_t t = 0; t < SORT_BLOCK; t++) {
+          const uint32_t v = psum[t];
+          psum[t] = run;
+          run += v;
+        }
+      }
+      __syncthreads();
+      uint32_t run = psum[tid];
+      for (uint32_t b = b0; b < b1; b++) {
+        offs[b] = run;
+        run += cnt[b];
+      }
+      __syncthreads();
+      // pass 2: place every digit at its bin's offset plus its rank
+      carry = carry0;
+      for (int wl = 0; wl < SCAT_WG; wl++) {
+        const int w = w0 + wl;
+        if (!live || w >= a.W) break;
+        const int d = window_digit(s.v, w, a.c, carry);
+        if (d != 0) {
+          const uint32_t key = bucket_key(a, j, w, d);
+          uint32_t ref = j.ref_offset + i;
+          if (!a.per_window) ref += (uint32_t)w * a.ref_stride;
+          const uint32_t p = offs[key >> a.l2] + rank[tid * SCAT_WG + wl];
+          skey[p] = key;
+          sref[p] = ref | (d < 0 ? NEG_FLAG : 0u);
+        }
+      }
+      carry0 = carry;
+      __syncthreads();
+      // write out in LDS order: consecutive entries of a bin go to consecutive positions
+      const uint32_t staged = offs[a.D1 - 1] + cnt[a.D1 - 1];
+      for (uint32_t e = tid; e < staged; e += SORT_BLOCK) {
+        const uint32_t key = skey[e];
+        const uint32_t bin = key >> a.l2;
+        const uint32_t pos = cur[bin] + (e - offs[bin]);
+        tmp_key[pos] = key & mask2;
+        tmp_ref[pos] = sref[e];
+      }
+      __syncthreads();
+      for (uint32_t b = tid; b < a.D1; b += SORT_BLOCK) cur[b] += cnt[b];
+      __syncthreads();
     }
   }
 }
@@ -512,9 +582,10 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   sa.c = p.c;
   sa.W = p.W;
   sa.per_window = p.per_window;
+  // top-level bins: 512 when the second level's 2^l2 counters allow it (longer runs per
+  // bin in the scatter), more for large key spaces (the second level keeps <= 2^11)
   const int nb_bits = ceil_log2(NB);
-  const int l1 = std::min(nb_bits, 11);
-  sa.l2 = (uint32_t)(nb_bits - l1);
+  sa.l2 = (uint32_t)std::min(11, std::max(0, nb_bits - 9));
   sa.D1 = (uint32_t)((NB + (1u << sa.l2) - 1) >> sa.l2);
   sa.ref_stride = ref_stride;
   const size_t HN = (size_t)sa.D1 * sa.G;   // histogram entries
@@ -572,8 +643,8 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   MZK_LAUNCH(c, "k_scan_tiles", k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, hist, hoff, HN, tsums);
   MZK_LAUNCH(c, "k_scan_sums", k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, tsums, (int)tiles, hoff, HN);
   MZK_LAUNCH(c, "k_scan_add", k_scan_add, dim3(grid_for(HN, 256)), dim3(256), 0, hoff, HN, tsums);
-  MZK_LAUNCH(c, "k_sort_scatter", k_sort_scatter, dim3(sa.G), dim3(SORT_BLOCK), sa.D1 * 4, canon, sa, hoff, tmp_key,
-             tmp_ref);
+  MZK_LAUNCH(c, "k_sort_scatter", k_sort_scatter, dim3(sa.G), dim3(SORT_BLOCK),
+             (3 * sa.D1 + 3 * SCAT_SLOTS + SORT_BLOCK) * 4, canon, sa, hoff, tmp_key, tmp_ref);
   MZK_LAUNCH(c, "k_sort_bins", k_sort_bins, dim3(sa.D1), dim3(SORT_BLOCK), (1u << sa.l2) * 4, hoff, sa, NB, tmp_key,
              tmp_ref, refs, off);
   MZK_LAUNCH(c, "k_msm_accum", k_msm_accum, dim3(grid_for(nchunks, 64)), dim3(64), 0, refs, off, NB, K, d_bases,
