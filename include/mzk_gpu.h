@@ -223,6 +223,30 @@ int mzk_plonk_numerator_dev(mzk_ctx *ctx, const mzk_plonk_numerator *args, void 
 int mzk_fr_lincomb_dev(mzk_ctx *ctx, const void *const *d_polys, const size_t *lens, const uint64_t *scalars,
                        size_t npolys, void *d_out, size_t out_len);
 
+/* ------------------------------------------------------------------ native prover
+ * create_proof (proofs/src/plonk/prover.rs:432-474) with multi_open
+ * (proofs/src/poly/kzg/mod.rs:110-289) for the synthetic circuit (advice a..e, fixed
+ * q_m..q_c, gate a b q_m + a q_l + b q_r - c q_o + q_c, all advice columns in the
+ * permutation, quotient degree 4, 6 blinding rows), run for `nproofs` proofs of one
+ * size in lockstep: each phase is one batched MSM / NTT / evaluation call for the
+ * whole group.  The key's vectors are device-resident Montgomery scalars (n = 2^k
+ * values, or 2^(k+2) for the coset forms), as the Python keygen builds them.  advice
+ * holds nproofs x 5 device pointers to the Lagrange columns (rows n - 7.. are replaced
+ * by blinding values drawn from the seeded stream of rng_seeds[p]).  Writes nproofs x
+ * MZK_PLONK_PROOF_BYTES.  phase_ms, if not null, receives 8 cumulative times (ms). */
+#define MZK_PLONK_PROOF_BYTES 1360
+typedef struct {
+  uint32_t k;
+  const mzk_srs *g, *g_lagrange;                 /* monomial (>= 2^k) and Lagrange (2^k) bases */
+  const void *fixed_polys[5], *fixed_cosets[5];
+  const void *sigma_values[5], *sigma_polys[5], *sigma_cosets[5];
+  const void *l0, *l_last, *l_active, *coset_x;  /* extended-domain vectors */
+  const void *omega_powers;                      /* omega^i, i < n */
+  uint64_t transcript_repr[4];                   /* VK hash (Montgomery) */
+} mzk_plonk_key;
+int mzk_plonk_prove(mzk_ctx *ctx, const mzk_plonk_key *key, size_t nproofs, const void *const *advice,
+                    const uint64_t *rng_seeds, uint8_t *proofs, double *phase_ms);
+
 /* ------------------------------------------------------------------ synthetic inputs
  * Element i of the seeded stream (seed, stream): eight splitmix64 words
  * w_j = splitmix64(key + 8 (offset + i) + j), key = splitmix64(seed ^ stream * 0xD1B54A32D192ED03),

@@ -25,6 +25,7 @@ enum ScratchSlot {
   SCR_MSM_D,        // MSM small reductions
   SCR_IN0,          // staging for host-pointer entry points
   SCR_IN1,
+  SCR_PROVER,       // the native prover's per-proof vectors
   SCR_COUNT
 };
 

@@ -56,38 +56,45 @@ __global__ void k_vec_scalar(int op, const uint32_t* __restrict__ a, FrArg s, ui
   store<FrP>(out + 8 * i, r);
 }
 
-// a^(r-2) (one dependent chain, used once per chunk)
-__device__ fr_t fr_inv_dev(const fr_t& a) {
-  constexpr uint32_t E[8] = {0xffffffffu, 0xfffffffeu, 0xfffe5bfeu, 0x53bda402u,
-                             0x09a1d805u, 0x3339d808u, 0x299d7d48u, 0x73eda753u};
-  fr_t r = fr_t::one();
-  for (int bit = 254; bit >= 0; bit--) {
-    r = sqr(r);
-    if ((E[bit >> 5] >> (bit & 31)) & 1u) r = mul(r, a);
-  }
-  return r;
-}
-
 static constexpr int INV_CHUNK = 32;
 
-// In place, INV_CHUNK elements per thread, one inversion per chunk; zeros stay zero.
-__global__ void k_batch_invert(uint32_t* __restrict__ a, size_t n, uint32_t* __restrict__ pre) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t i0 = t * INV_CHUNK;
-  if (i0 >= n) return;
-  const size_t i1 = i0 + INV_CHUNK < n ? i0 + INV_CHUNK : n;
+// Batch inversion as a 32-ary product tree: the up pass turns each chunk of 32 values
+// into its product (zeros skipped) and the exclusive prefix products inside the chunk;
+// the inverse of the root is taken on the host (a Fermat inversion on one GPU lane is
+// ~400 dependent products); the down pass turns each chunk's inverse product back
+// into the inverses of its values (Montgomery's trick, walked from the chunk's top).
+__global__ void k_binv_up(const uint32_t* __restrict__ x, size_t m, uint32_t* __restrict__ pre,
+                          uint32_t* __restrict__ prod) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t lo = c * INV_CHUNK;
+  if (lo >= m) return;
+  const size_t hi = lo + INV_CHUNK < m ? lo + INV_CHUNK : m;
   fr_t acc = fr_t::one();
-  for (size_t i = i0; i < i1; i++) {
+  for (size_t i = lo; i < hi; i++) {
     store<FrP>(pre + 8 * i, acc);
-    fr_t x = load<FrP>(a + 8 * i);
-    if (!x.is_zero()) acc = mul(acc, x);
+    const fr_t v = load<FrP>(x + 8 * i);
+    if (!v.is_zero()) acc = mul(acc, v);
   }
-  fr_t inv = fr_inv_dev(acc);
-  for (size_t i = i1; i-- > i0;) {
-    fr_t x = load<FrP>(a + 8 * i);
-    if (x.is_zero()) continue;
-    store<FrP>(a + 8 * i, mul(inv, load<FrP>(pre + 8 * i)));
-    inv = mul(inv, x);
+  store<FrP>(prod + 8 * c, acc);
+}
+
+// inv[i] = x[i]^-1 (0 for 0) from the chunk inverses cinv (root_inv when cinv is null:
+// the level holds one chunk).  inv may be x: each value is read before it is replaced.
+__global__ void k_binv_down(const uint32_t* x, size_t m, const uint32_t* __restrict__ pre,
+                            const uint32_t* __restrict__ cinv, FrArg root_inv, uint32_t* inv) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t lo = c * INV_CHUNK;
+  if (lo >= m) return;
+  const size_t hi = lo + INV_CHUNK < m ? lo + INV_CHUNK : m;
+  fr_t acc = cinv ? load<FrP>(cinv + 8 * c) : arg_fr(root_inv);
+  for (size_t i = hi; i-- > lo;) {
+    const fr_t v = load<FrP>(x + 8 * i);
+    if (v.is_zero()) {
+      store<FrP>(inv + 8 * i, fr_t::zero());
+      continue;
+    }
+    store<FrP>(inv + 8 * i, mul(acc, load<FrP>(pre + 8 * i)));
+    acc = mul(acc, v);
   }
 }
 
@@ -245,14 +252,20 @@ __global__ void __launch_bounds__(256) k_eval_batch(const EvalJob* __restrict__ 
 // ---------------------------------------------------------------- Kate division
 // q_i = a_(i+1) + b q_(i+1), i < m = n - 1, with q_m = 0 (kate_division).  Chunk c holds
 // q indices [c L, c L + L): q_lo = loc_c + b^L q_hi, composed from the top chunk down.
-static constexpr int KATE_L = 64;
+// L balances the chunk walks (L steps) against the carry pass (m / (256 L) steps).
+static uint32_t kate_chunk(size_t m) {
+  uint32_t L = 4;
+  while (L < 64 && (size_t)L * L * 256 < m) L <<= 1;
+  return L;
+}
 
 // Phase 1: loc_c (the chunk's bottom value with a zero carry-in).
-__global__ void k_kate_local(const uint32_t* __restrict__ a, size_t m, FrArg b, uint32_t* __restrict__ loc) {
+__global__ void k_kate_local(const uint32_t* __restrict__ a, size_t m, uint32_t KL, FrArg b,
+                             uint32_t* __restrict__ loc) {
   const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t lo = c * KATE_L;
+  const size_t lo = c * KL;
   if (lo >= m) return;
-  const size_t hi = lo + KATE_L < m ? lo + KATE_L : m;
+  const size_t hi = lo + KL < m ? lo + KL : m;
   const fr_t bb = arg_fr(b);
   fr_t q = fr_t::zero();
   for (size_t i = hi; i-- > lo;) q = add(load<FrP>(a + 8 * (i + 1)), mul(bb, q));
@@ -309,12 +322,12 @@ __global__ void __launch_bounds__(256) k_kate_carries(const uint32_t* __restrict
 }
 
 // Phase 3: recompute each chunk with its true carry-in; write q.
-__global__ void k_kate_final(const uint32_t* __restrict__ a, size_t m, FrArg b, const uint32_t* __restrict__ carry,
-                             uint32_t* __restrict__ q) {
+__global__ void k_kate_final(const uint32_t* __restrict__ a, size_t m, uint32_t KL, FrArg b,
+                             const uint32_t* __restrict__ carry, uint32_t* __restrict__ q) {
   const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t lo = c * KATE_L;
+  const size_t lo = c * KL;
   if (lo >= m) return;
-  const size_t hi = lo + KATE_L < m ? lo + KATE_L : m;
+  const size_t hi = lo + KL < m ? lo + KL : m;
   const fr_t bb = arg_fr(b);
   fr_t v = load<FrP>(carry + 8 * c);
   for (size_t i = hi; i-- > lo;) {
@@ -454,10 +467,43 @@ int mzk_fr_batch_invert(mzk_ctx* c, void* a, size_t n) {
   MZK_REQUIRE(a != nullptr, "mzk_fr_batch_invert: null data");
   MZK_TRY(vec_ready(c));
   std::lock_guard<std::mutex> g(c->mu);
-  MZK_TRY(c->scratch[SCR_MSM_D].reserve(n * 32 + 64));
-  const size_t threads = (n + INV_CHUNK - 1) / INV_CHUNK;
-  MZK_LAUNCH(c, "k_batch_invert", k_batch_invert, dim3(grid_for(threads, 64)), dim3(64), 0, (uint32_t*)a, n,
-             (uint32_t*)c->scratch[SCR_MSM_D].ptr);
+  // levels: sizes m_0 = n, m_(l+1) = ceil(m_l / 32), down to one value
+  std::vector<size_t> sizes = {n};
+  while (sizes.back() > 1) sizes.push_back((sizes.back() + INV_CHUNK - 1) / INV_CHUNK);
+  size_t total = 0;
+  for (size_t l = 0; l < sizes.size(); l++) total += sizes[l];
+  // scratch: per level a prefix buffer (m_l) and the level's values (m_l, from level 1)
+  MZK_TRY(c->scratch[SCR_MSM_D].reserve(2 * total * 32 + 256));
+  uint32_t* base = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
+  std::vector<uint32_t*> val(sizes.size()), pre(sizes.size());
+  size_t o = 0;
+  for (size_t l = 0; l < sizes.size(); l++) {
+    val[l] = l == 0 ? (uint32_t*)a : base + 8 * o;
+    o += sizes[l];
+    pre[l] = base + 8 * o;
+    o += sizes[l];
+  }
+  for (size_t l = 0; l + 1 < sizes.size(); l++)
+    MZK_LAUNCH(c, "k_binv_up", k_binv_up, dim3(grid_for(sizes[l + 1], 64)), dim3(64), 0, val[l], sizes[l], pre[l],
+               val[l + 1]);
+  // the root (one value) is inverted on the host and enters the top down pass by value
+  host::Fr root;
+  MZK_HIP(hipMemcpyAsync(root.l, val.back(), 32, hipMemcpyDeviceToHost, c->stream));
+  MZK_HIP(hipStreamSynchronize(c->stream));
+  const bool zero = (root.l[0] | root.l[1] | root.l[2] | root.l[3]) == 0;
+  const host::Fr rinv = zero ? host::fr_zero() : host::fr_inv(root);
+  if (sizes.size() == 1) {   // n == 1: the value itself
+    MZK_HIP(hipMemcpyAsync(a, rinv.l, 32, hipMemcpyHostToDevice, c->stream));
+    MZK_HIP(hipStreamSynchronize(c->stream));
+    return MZK_OK;
+  }
+  // level l's chunk inverses are the inverted values of level l + 1, written in place
+  const uint32_t* up_inv = nullptr;
+  for (size_t l = sizes.size() - 1; l-- > 0;) {
+    MZK_LAUNCH(c, "k_binv_down", k_binv_down, dim3(grid_for(sizes[l + 1], 64)), dim3(64), 0, val[l], sizes[l],
+               pre[l], up_inv, to_arg(rinv), val[l]);
+    up_inv = val[l];
+  }
   MZK_HIP(hipGetLastError());
   return MZK_OK;
 }
@@ -573,7 +619,8 @@ int mzk_fr_kate_div(mzk_ctx* c, const void* a, size_t n, const uint64_t b[4], vo
   if (n == 1) return MZK_OK;   // the quotient has no coefficients
   MZK_REQUIRE(a && q && a != q, "mzk_fr_kate_div: null or aliased data");
   const size_t m = n - 1;
-  const size_t C = (m + KATE_L - 1) / KATE_L;
+  const uint32_t KL = kate_chunk(m);
+  const size_t C = (m + KL - 1) / KL;
   MZK_TRY(vec_ready(c));
   std::lock_guard<std::mutex> g(c->mu);
   MZK_TRY(c->scratch[SCR_MSM_D].reserve(2 * C * 32 + 64));
@@ -581,12 +628,12 @@ int mzk_fr_kate_div(mzk_ctx* c, const void* a, size_t n, const uint64_t b[4], vo
   uint32_t* carry = loc + 8 * C;
   host::Fr hb;
   memcpy(hb.l, b, 32);
-  const host::Fr gL = host::fr_pow(hb, KATE_L);
-  MZK_LAUNCH(c, "k_kate_local", k_kate_local, dim3(grid_for(C, 64)), dim3(64), 0, (const uint32_t*)a, m, to_arg(b),
-             loc);
+  const host::Fr gL = host::fr_pow(hb, KL);
+  MZK_LAUNCH(c, "k_kate_local", k_kate_local, dim3(grid_for(C, 64)), dim3(64), 0, (const uint32_t*)a, m, KL,
+             to_arg(b), loc);
   MZK_LAUNCH(c, "k_kate_carries", k_kate_carries, dim3(1), dim3(256), 0, loc, C, to_arg(gL), carry);
-  MZK_LAUNCH(c, "k_kate_final", k_kate_final, dim3(grid_for(C, 64)), dim3(64), 0, (const uint32_t*)a, m, to_arg(b),
-             carry, (uint32_t*)q);
+  MZK_LAUNCH(c, "k_kate_final", k_kate_final, dim3(grid_for(C, 64)), dim3(64), 0, (const uint32_t*)a, m, KL,
+             to_arg(b), carry, (uint32_t*)q);
   MZK_HIP(hipGetLastError());
   return MZK_OK;
 }

@@ -1,0 +1,709 @@
+// Native prover for the synthetic PLONK circuit (five advice columns a..e, fixed
+// q_m..q_c, gate a b q_m + a q_l + b q_r - c q_o + q_c, every advice column in the
+// permutation, quotient degree 4), for a group of proofs of one size in lockstep.
+//
+// It follows create_proof (proofs/src/plonk/prover.rs:432-474 -> compute_trace :66-299
+// -> finalise_proof :302-423) and multi_open (proofs/src/poly/kzg/mod.rs:110-289) step
+// for step, with the same deliberate differences as the Python form of this prover
+// (midnight-zk_amd/plonk.py): seeded random draws, Lagrange-basis permutation commits,
+// a descriptor string in the VK hash (the caller passes the hash).  Every phase issues
+// one batched call for the whole group (commitments through mzk_msm_srs_batch_dev,
+// transforms through mzk_ntt_batch_dev, evaluations through mzk_fr_eval_batch), so a
+// group of proofs shares each pipeline; everything else is the library's vector
+// kernels.  Host work is the transcript (BLAKE2b), challenges and point encodings.
+#include <algorithm>
+#include <chrono>
+#include <vector>
+
+#include "blake2b.hpp"
+#include "ctx.hpp"
+#include "host_ec.hpp"
+#include "srs.hpp"
+
+namespace mzk {
+namespace prover {
+
+using host::Fr;
+
+constexpr int NADV = 5;
+constexpr int NFIX = 5;
+constexpr int BF = 6;          // blinding factors (circuit.rs:2203-2246 for this circuit)
+constexpr int QDEG = 4;        // quotient limbs
+constexpr int CHUNK = 3;       // permutation columns per set
+constexpr int NSETS = 2;
+constexpr uint64_t RNG_STREAM = 0x524E47;
+constexpr int NDRAWS = NADV * (BF + 1) + NSETS * BF + (QDEG - 1);   // 50
+
+static Fr fr_of(const uint64_t v[4]) {
+  Fr r;
+  memcpy(r.l, v, 32);
+  return r;
+}
+
+static Fr fr_canon(const Fr& a) {   // Montgomery -> canonical limbs
+  Fr one = host::fr_zero();
+  one.l[0] = 1;
+  return host::fr_mul(a, one);
+}
+
+static Fr fr_neg(const Fr& a) { return host::fr_sub(host::fr_zero(), a); }
+
+// Fq::from_uniform_bytes (fq.rs:712-736): 512-bit little-endian integer mod r
+static Fr fr_from_uniform(const uint8_t b[64]) {
+  Fr lo, hi;
+  memcpy(lo.l, b, 32);
+  memcpy(hi.l, b + 32, 32);
+  const Fr r2 = host::fr_const(host::kFrR2);
+  const Fr r3 = host::fr_mul(r2, r2);   // R^3
+  return host::fr_add(host::fr_mul(lo, r2), host::fr_mul(hi, r3));
+}
+
+struct Transcript {
+  host::Blake2b h;
+  std::vector<uint8_t> proof;
+  Transcript() : h((const uint8_t*)"Domain separator for transcript", 31) {}
+  void absorb(const uint8_t* d, size_t len) {
+    const uint8_t pre = 1;
+    h.update(&pre, 1);
+    h.update(d, len);
+  }
+  void common_fr(const Fr& a) {
+    const Fr c = fr_canon(a);
+    absorb((const uint8_t*)c.l, 32);
+  }
+  void write_fr(const Fr& a) {
+    const Fr c = fr_canon(a);
+    absorb((const uint8_t*)c.l, 32);
+    proof.insert(proof.end(), (const uint8_t*)c.l, (const uint8_t*)c.l + 32);
+  }
+  void write_point(const uint8_t c[48]) {
+    absorb(c, 48);
+    proof.insert(proof.end(), c, c + 48);
+  }
+  Fr squeeze() {
+    const uint8_t pre = 0;
+    h.update(&pre, 1);
+    uint8_t d[64];
+    h.digest(d);
+    return fr_from_uniform(d);
+  }
+};
+
+// G1Affine::to_compressed (ZCash encoding) of a Jacobian result
+static void compress(const uint64_t jac[18], uint8_t out[48]) {
+  uint64_t aff[12];
+  mzk_g1_jacobian_to_affine(jac, aff);
+  bool inf = true;
+  for (int i = 0; i < 12; i++) inf &= aff[i] == 0;
+  memset(out, 0, 48);
+  if (inf) {
+    out[0] = 0xc0;
+    return;
+  }
+  host::Fp one = host::fp_zero();
+  one.l[0] = 1;
+  host::Fp x, y;
+  memcpy(x.l, aff, 48);
+  memcpy(y.l, aff + 6, 48);
+  x = host::fp_mul(x, one);
+  y = host::fp_mul(y, one);
+  for (int i = 0; i < 6; i++)
+    for (int b = 0; b < 8; b++) out[47 - (8 * i + b)] = (uint8_t)(x.l[i] >> (8 * b));
+  out[0] |= 0x80;
+  // (p - 1) / 2
+  static const uint64_t HALF[6] = {0xdcff7fffffffd555ull, 0x0f55ffff58a9ffffull, 0xb39869507b587b12ull,
+                                   0xb23ba5c279c2895full, 0x258dd3db21a5d66bull, 0x0d0088f51cbff34dull};
+  bool larger = false;
+  for (int i = 5; i >= 0; i--) {
+    if (y.l[i] != HALF[i]) {
+      larger = y.l[i] > HALF[i];
+      break;
+    }
+  }
+  if (larger) out[0] |= 0x20;
+}
+
+static Fr fr_pow_u64(const Fr& a, uint64_t e) { return host::fr_pow(a, e); }
+
+}  // namespace prover
+}  // namespace mzk
+
+using namespace mzk;
+using namespace mzk::prover;
+
+#define PV_TRY(expr)                  \
+  do {                                \
+    int rc_ = (expr);                 \
+    if (rc_ != MZK_OK) return rc_;    \
+  } while (0)
+
+namespace {
+
+struct ProofBufs {
+  uint32_t* adv[NADV];
+  uint32_t* polys[NADV + NSETS];   // advice then z, coefficient form
+  uint32_t* z[NSETS];              // Lagrange form
+  uint32_t* cos[NADV + NSETS];
+  uint32_t* hext;
+  uint32_t* h;
+  uint32_t* limbs[QDEG];
+  uint32_t* lin;
+  uint32_t* q0;
+  uint32_t* k1;   // Kate chain temporaries
+  uint32_t* k2;
+  uint32_t* fa;
+  uint32_t* fb;
+  uint32_t* fc;
+  uint32_t* f;
+  uint32_t* fin;
+  uint32_t* pi;
+  uint32_t* mv;
+  uint32_t* t;
+  uint32_t* draws;
+};
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int msm_batch(mzk_ctx* c, const mzk_srs* srs, const std::vector<std::pair<const void*, size_t>>& jobs,
+              std::vector<uint64_t>& outs) {
+  std::vector<mzk_msm_job> j(jobs.size());
+  for (size_t i = 0; i < jobs.size(); i++) {
+    j[i].d_coeffs = jobs[i].first;
+    j[i].n = jobs[i].second;
+    j[i].offset = 0;
+  }
+  outs.assign(18 * jobs.size(), 0);
+  return mzk_msm_srs_batch_dev(c, srs, j.data(), j.size(), outs.data());
+}
+
+int eval_batch(mzk_ctx* c, const std::vector<std::pair<const void*, size_t>>& polys, const std::vector<Fr>& xs,
+               std::vector<Fr>& out) {
+  const size_t m = polys.size();
+  std::vector<const void*> ptr(m);
+  std::vector<size_t> ns(m);
+  std::vector<uint64_t> x(4 * m), o(4 * m);
+  for (size_t i = 0; i < m; i++) {
+    ptr[i] = polys[i].first;
+    ns[i] = polys[i].second;
+    memcpy(&x[4 * i], xs[i].l, 32);
+  }
+  PV_TRY(mzk_fr_eval_batch(c, ptr.data(), ns.data(), x.data(), m, o.data()));
+  out.resize(m);
+  for (size_t i = 0; i < m; i++) memcpy(out[i].l, &o[4 * i], 32);
+  return MZK_OK;
+}
+
+int lincomb(mzk_ctx* c, const std::vector<std::pair<const void*, size_t>>& polys, const std::vector<Fr>& s,
+            void* out, size_t out_len) {
+  std::vector<const void*> ptr;
+  std::vector<size_t> lens;
+  std::vector<uint64_t> sc;
+  for (size_t i = 0; i < polys.size(); i++) {
+    ptr.push_back(polys[i].first);
+    lens.push_back(polys[i].second);
+    sc.insert(sc.end(), s[i].l, s[i].l + 4);
+  }
+  return mzk_fr_lincomb_dev(c, ptr.data(), lens.data(), sc.data(), ptr.size(), out, out_len);
+}
+
+int vscal(mzk_ctx* c, int op, const void* a, const Fr& s, void* out, size_t n) {
+  return mzk_fr_vec_scalar(c, op, a, s.l, out, n);
+}
+
+// construct_intermediate_sets (kzg/utils.rs:40-133) for (label, point) queries
+void intermediate_sets(const std::vector<std::pair<int, Fr>>& q, std::vector<int>& labels, std::vector<int>& set_of,
+                       std::vector<std::vector<Fr>>& point_sets) {
+  std::vector<Fr> points;
+  std::vector<std::vector<int>> lpts;
+  for (auto& e : q) {
+    int pi = -1;
+    for (size_t i = 0; i < points.size(); i++)
+      if (host::fr_eq(points[i], e.second)) pi = (int)i;
+    if (pi < 0) {
+      pi = (int)points.size();
+      points.push_back(e.second);
+    }
+    auto it = std::find(labels.begin(), labels.end(), e.first);
+    if (it == labels.end()) {
+      labels.push_back(e.first);
+      lpts.push_back({pi});
+    } else {
+      lpts[it - labels.begin()].push_back(pi);
+    }
+  }
+  std::vector<std::vector<int>> keys;
+  for (auto& p : lpts) {
+    std::vector<int> k = p;
+    std::sort(k.begin(), k.end());
+    auto it = std::find(keys.begin(), keys.end(), k);
+    if (it == keys.end()) {
+      set_of.push_back((int)keys.size());
+      keys.push_back(k);
+    } else {
+      set_of.push_back((int)(it - keys.begin()));
+    }
+  }
+  for (auto& k : keys) {
+    std::vector<Fr> ps;
+    for (int i : k) ps.push_back(points[i]);
+    point_sets.push_back(ps);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void* const* advice,
+                    const uint64_t* rng_seeds, uint8_t* proofs, double* phase_ms) {
+  MZK_REQUIRE(c && key && (np == 0 || (advice && rng_seeds && proofs)), "mzk_plonk_prove: null argument");
+  if (np == 0) return MZK_OK;
+  const uint32_t k = key->k;
+  MZK_REQUIRE(k >= 4 && k <= 24, "mzk_plonk_prove: k = %u out of range", k);
+  const size_t n = (size_t)1 << k, N = 4 * n, u = n - BF - 1;
+  MZK_REQUIRE(key->g && key->g_lagrange && key->g->n >= n && key->g_lagrange->n >= n,
+              "mzk_plonk_prove: SRS shorter than 2^%u", k);
+  const void* need[] = {key->l0, key->l_last, key->l_active, key->coset_x, key->omega_powers};
+  for (const void* p : need) MZK_REQUIRE(p, "mzk_plonk_prove: key vector missing");
+  for (int j = 0; j < NFIX; j++) MZK_REQUIRE(key->fixed_polys[j] && key->fixed_cosets[j], "mzk_plonk_prove: fixed %d", j);
+  for (int j = 0; j < NADV; j++)
+    MZK_REQUIRE(key->sigma_values[j] && key->sigma_polys[j] && key->sigma_cosets[j], "mzk_plonk_prove: sigma %d", j);
+  for (size_t p = 0; p < np * NADV; p++) MZK_REQUIRE(advice[p], "mzk_plonk_prove: advice column %zu missing", p);
+  double t0 = now_ms();
+  double* ph = phase_ms;
+  auto mark = [&](int i) {
+    if (!ph) return;
+    mzk_ctx_synchronize(c);
+    ph[i] = now_ms() - t0;
+  };
+
+  // ---- device arena: per proof 69 n scalars plus the draws
+  const size_t per = (size_t)(NADV + (NADV + NSETS) + NSETS + 4 * (NADV + NSETS) + 4 + 4 + QDEG + 12) * n + 64;
+  {
+    std::lock_guard<std::mutex> g(c->mu);
+    MZK_HIP(hipSetDevice(c->device));
+    PV_TRY(c->scratch[SCR_PROVER].reserve(per * 32 * np + 256));
+  }
+  uint32_t* arena = (uint32_t*)c->scratch[SCR_PROVER].ptr;
+  std::vector<ProofBufs> B(np);
+  for (size_t p = 0; p < np; p++) {
+    uint32_t* q = arena + 8 * per * p;
+    auto take = [&](size_t elems) {
+      uint32_t* r = q;
+      q += 8 * elems;
+      return r;
+    };
+    ProofBufs& b = B[p];
+    for (int j = 0; j < NADV; j++) b.adv[j] = take(n);
+    for (int j = 0; j < NADV + NSETS; j++) b.polys[j] = take(n);
+    for (int s = 0; s < NSETS; s++) b.z[s] = take(n);
+    for (int j = 0; j < NADV + NSETS; j++) b.cos[j] = take(N);
+    b.hext = take(N);
+    b.h = take(N);
+    for (int i = 0; i < QDEG; i++) b.limbs[i] = take(n);
+    b.lin = take(n);
+    b.q0 = take(n);
+    b.k1 = take(n);
+    b.k2 = take(n);
+    b.fa = take(n);
+    b.fb = take(n);
+    b.fc = take(n);
+    b.f = take(n);
+    b.fin = take(n);
+    b.pi = take(n);
+    b.mv = take(n);
+    b.t = take(n);
+    b.draws = take(64);
+  }
+  hipStream_t st = c->stream;
+  std::vector<Transcript> T(np);
+  const Fr one = host::fr_one();
+  const Fr omega = host::fr_root_of_unity(k), omega_inv = host::fr_inv(omega);
+  const Fr delta = fr_pow_u64(host::fr_from_u64(7), 1ull << 32);   // fq.rs:501-509
+
+  // ---- VK hash, advice with blinding rows, advice commitments (prover.rs:101-108, 541-631)
+  for (size_t p = 0; p < np; p++) {
+    T[p].common_fr(fr_of(key->transcript_repr));
+    PV_TRY(mzk_fr_from_seed_dev(c, rng_seeds[p], RNG_STREAM, 0, NDRAWS, B[p].draws));
+    for (int j = 0; j < NADV; j++) {
+      MZK_HIP(hipMemcpyAsync(B[p].adv[j], advice[p * NADV + j], u * 32, hipMemcpyDeviceToDevice, st));
+      MZK_HIP(hipMemcpyAsync(B[p].adv[j] + 8 * u, B[p].draws + 8 * (size_t)(j * (BF + 1)), (BF + 1) * 32,
+                             hipMemcpyDeviceToDevice, st));
+    }
+  }
+  std::vector<uint64_t> jac;
+  {
+    std::vector<std::pair<const void*, size_t>> jobs;
+    for (size_t p = 0; p < np; p++)
+      for (int j = 0; j < NADV; j++) jobs.push_back({B[p].adv[j], n});
+    PV_TRY(msm_batch(c, key->g_lagrange, jobs, jac));
+    for (size_t p = 0; p < np; p++)
+      for (int j = 0; j < NADV; j++) {
+        uint8_t cb[48];
+        compress(&jac[18 * (p * NADV + j)], cb);
+        T[p].write_point(cb);
+      }
+  }
+  mark(0);
+  std::vector<Fr> beta(np), gamma(np), y(np);
+  for (size_t p = 0; p < np; p++) {
+    T[p].squeeze();   // theta (no lookups)
+    beta[p] = T[p].squeeze();
+    gamma[p] = T[p].squeeze();
+  }
+
+  // ---- permutation grand products (permutation/prover.rs:81-255)
+  for (size_t p = 0; p < np; p++) {
+    ProofBufs& b = B[p];
+    Fr corr = one;
+    for (int s = 0; s < NSETS; s++) {
+      const int j0 = s * CHUNK, j1 = std::min(NADV, j0 + CHUNK);
+      for (int j = j0; j < j1; j++) {   // prod (beta sigma_j + adv_j + gamma)
+        uint32_t* dst = j == j0 ? b.mv : b.t;
+        PV_TRY(vscal(c, 0, key->sigma_values[j], beta[p], dst, n));
+        PV_TRY(mzk_fr_vec_op(c, 0, dst, b.adv[j], dst, n));
+        PV_TRY(vscal(c, 1, dst, gamma[p], dst, n));
+        if (j != j0) PV_TRY(mzk_fr_vec_op(c, 2, b.mv, b.t, b.mv, n));
+      }
+      PV_TRY(mzk_fr_batch_invert(c, b.mv, n));
+      for (int j = j0; j < j1; j++) {   // times prod (beta delta^j omega^i + adv_j + gamma)
+        const Fr bd = host::fr_mul(beta[p], fr_pow_u64(delta, (uint64_t)j));
+        PV_TRY(vscal(c, 0, key->omega_powers, bd, b.t, n));
+        PV_TRY(mzk_fr_vec_op(c, 0, b.t, b.adv[j], b.t, n));
+        PV_TRY(vscal(c, 1, b.t, gamma[p], b.t, n));
+        PV_TRY(mzk_fr_vec_op(c, 2, b.mv, b.t, b.mv, n));
+      }
+      // z_0 = 1, z_i = z_(i-1) mv_(i-1); blinding rows; set s > 0 scaled by the earlier sets' last values
+      uint32_t* z = b.z[s];
+      MZK_HIP(hipMemcpyAsync(z, one.l, 32, hipMemcpyHostToDevice, st));
+      PV_TRY(mzk_fr_prefix_product(c, b.mv, z + 8, n - 1));
+      MZK_HIP(hipMemcpyAsync(z + 8 * (n - BF), b.draws + 8 * (size_t)(NADV * (BF + 1) + s * BF), BF * 32,
+                             hipMemcpyDeviceToDevice, st));
+      Fr last;
+      MZK_HIP(hipMemcpyAsync(last.l, z + 8 * u, 32, hipMemcpyDeviceToHost, st));
+      MZK_HIP(hipStreamSynchronize(st));
+      if (s > 0) PV_TRY(vscal(c, 0, z, corr, z, n));
+      corr = host::fr_mul(corr, last);
+    }
+  }
+  {
+    std::vector<std::pair<const void*, size_t>> jobs;
+    for (size_t p = 0; p < np; p++)
+      for (int s = 0; s < NSETS; s++) jobs.push_back({B[p].z[s], n});
+    PV_TRY(msm_batch(c, key->g_lagrange, jobs, jac));
+    for (size_t p = 0; p < np; p++)
+      for (int s = 0; s < NSETS; s++) {
+        uint8_t cb[48];
+        compress(&jac[18 * (p * NSETS + s)], cb);
+        T[p].write_point(cb);
+      }
+  }
+  mark(1);
+  for (size_t p = 0; p < np; p++) {
+    T[p].squeeze();   // trash challenge (no trashcans)
+    y[p] = T[p].squeeze();
+  }
+
+  // ---- to coefficients, to the extended coset, numerator / (X^n - 1), back (prover.rs:634-772)
+  {
+    std::vector<mzk_ntt_job> jobs;
+    for (size_t p = 0; p < np; p++)
+      for (int j = 0; j < NADV + NSETS; j++) {
+        mzk_ntt_job t;
+        t.d_in = j < NADV ? (const void*)B[p].adv[j] : (const void*)B[p].z[j - NADV];
+        t.d_out = B[p].polys[j];
+        t.n_real = n;
+        t.log_n = k;
+        t.flags = MZK_NTT_INVERSE;
+        jobs.push_back(t);
+      }
+    PV_TRY(mzk_ntt_batch_dev(c, jobs.data(), jobs.size()));
+    mark(2);
+    jobs.clear();
+    for (size_t p = 0; p < np; p++)
+      for (int j = 0; j < NADV + NSETS; j++) {
+        mzk_ntt_job t;
+        t.d_in = B[p].polys[j];
+        t.d_out = B[p].cos[j];
+        t.n_real = n;
+        t.log_n = k + 2;
+        t.flags = MZK_NTT_COSET;
+        jobs.push_back(t);
+      }
+    PV_TRY(mzk_ntt_batch_dev(c, jobs.data(), jobs.size()));
+    mark(3);
+  }
+  std::vector<Fr> te(4);
+  {
+    const Fr wext = host::fr_root_of_unity(k + 2), zeta = host::fr_const(host::kFrZeta);
+    Fr cur = zeta;
+    for (int i = 0; i < 4; i++) {
+      te[i] = host::fr_inv(host::fr_sub(fr_pow_u64(cur, n), one));
+      cur = host::fr_mul(cur, wext);
+    }
+  }
+  for (size_t p = 0; p < np; p++) {
+    mzk_plonk_numerator A;
+    memset(&A, 0, sizeof A);
+    A.n_ext = N;
+    A.rot_next = 4;
+    A.rot_last = (BF + 1) * 4;
+    A.ncols = NADV;
+    A.nsets = NSETS;
+    A.chunk = CHUNK;
+    A.t_count = 4;
+    const ProofBufs& b = B[p];
+    A.a = b.cos[0];
+    A.b = b.cos[1];
+    A.c = b.cos[2];
+    A.qm = key->fixed_cosets[0];
+    A.ql = key->fixed_cosets[1];
+    A.qr = key->fixed_cosets[2];
+    A.qo = key->fixed_cosets[3];
+    A.qc = key->fixed_cosets[4];
+    for (int j = 0; j < NADV; j++) {
+      A.col[j] = b.cos[j];
+      A.sigma[j] = key->sigma_cosets[j];
+      const Fr bd = host::fr_mul(beta[p], fr_pow_u64(delta, (uint64_t)j));
+      memcpy(A.delta_beta[j], bd.l, 32);
+    }
+    for (int s = 0; s < NSETS; s++) A.z[s] = b.cos[NADV + s];
+    A.l0 = key->l0;
+    A.l_last = key->l_last;
+    A.l_active = key->l_active;
+    A.coset_x = key->coset_x;
+    memcpy(A.y, y[p].l, 32);
+    memcpy(A.beta, beta[p].l, 32);
+    memcpy(A.gamma, gamma[p].l, 32);
+    for (int i = 0; i < 4; i++) memcpy(A.t_evals[i], te[i].l, 32);
+    PV_TRY(mzk_plonk_numerator_dev(c, &A, b.hext));
+  }
+  mark(4);
+  {
+    std::vector<mzk_ntt_job> jobs;
+    for (size_t p = 0; p < np; p++) {
+      mzk_ntt_job t;
+      t.d_in = B[p].hext;
+      t.d_out = B[p].h;
+      t.n_real = N;
+      t.log_n = k + 2;
+      t.flags = MZK_NTT_INVERSE | MZK_NTT_COSET;
+      jobs.push_back(t);
+    }
+    PV_TRY(mzk_ntt_batch_dev(c, jobs.data(), jobs.size()));
+  }
+  // limbs of n - 1 coefficients, each with a blinding tail; limb i > 0 minus the previous tail
+  for (size_t p = 0; p < np; p++) {
+    ProofBufs& b = B[p];
+    const uint32_t* blind = b.draws + 8 * (size_t)(NADV * (BF + 1) + NSETS * BF);
+    for (int i = 0; i < QDEG; i++) {
+      MZK_HIP(hipMemcpyAsync(b.limbs[i], b.h + 8 * (size_t)i * (n - 1), (n - 1) * 32, hipMemcpyDeviceToDevice, st));
+      if (i < QDEG - 1)
+        MZK_HIP(hipMemcpyAsync(b.limbs[i] + 8 * (n - 1), blind + 8 * i, 32, hipMemcpyDeviceToDevice, st));
+      else
+        MZK_HIP(hipMemsetAsync(b.limbs[i] + 8 * (n - 1), 0, 32, st));
+      if (i > 0) PV_TRY(mzk_fr_vec_op(c, 1, b.limbs[i], blind + 8 * (i - 1), b.limbs[i], 1));
+    }
+  }
+  {
+    std::vector<std::pair<const void*, size_t>> jobs;
+    for (size_t p = 0; p < np; p++)
+      for (int i = 0; i < QDEG; i++) jobs.push_back({B[p].limbs[i], n});
+    PV_TRY(msm_batch(c, key->g, jobs, jac));
+    for (size_t p = 0; p < np; p++)
+      for (int i = 0; i < QDEG; i++) {
+        uint8_t cb[48];
+        compress(&jac[18 * (p * QDEG + i)], cb);
+        T[p].write_point(cb);
+      }
+  }
+  mark(5);
+
+  // ---- evaluations (prover.rs:798-869, permutation/prover.rs:266-340)
+  std::vector<Fr> x(np), xn1(np), xl(np);
+  const Fr omega_last = fr_pow_u64(omega_inv, BF + 1);
+  {
+    std::vector<std::pair<const void*, size_t>> polys;
+    std::vector<Fr> pts;
+    for (size_t p = 0; p < np; p++) {
+      x[p] = T[p].squeeze();
+      xn1[p] = host::fr_mul(x[p], omega);
+      xl[p] = host::fr_mul(x[p], omega_last);
+      for (int j = 0; j < NADV; j++) polys.push_back({B[p].polys[j], n}), pts.push_back(x[p]);
+      for (int j = 0; j < NFIX; j++) polys.push_back({key->fixed_polys[j], n}), pts.push_back(x[p]);
+      for (int j = 0; j < NADV; j++) polys.push_back({key->sigma_polys[j], n}), pts.push_back(x[p]);
+      const void* z0 = B[p].polys[NADV];
+      const void* z1 = B[p].polys[NADV + 1];
+      polys.push_back({z0, n}), pts.push_back(x[p]);
+      polys.push_back({z0, n}), pts.push_back(xn1[p]);
+      polys.push_back({z0, n}), pts.push_back(xl[p]);
+      polys.push_back({z1, n}), pts.push_back(x[p]);
+      polys.push_back({z1, n}), pts.push_back(xn1[p]);
+    }
+    std::vector<Fr> ev;
+    PV_TRY(eval_batch(c, polys, pts, ev));
+    const size_t per_p = ev.size() / np;
+    for (size_t p = 0; p < np; p++)
+      for (size_t i = 0; i < per_p; i++) T[p].write_fr(ev[p * per_p + i]);
+  }
+  mark(6);
+
+  // ---- linearization and multi-open (linearization/prover.rs:42-86, kzg/mod.rs:110-289)
+  // query labels: advice 0..4, perm 10 + s, fixed 20 + j, sigma 30 + j, lin 40
+  std::vector<std::vector<Fr>> point_sets;
+  std::vector<int> labels, set_of;
+  std::vector<std::vector<std::pair<int, Fr>>> Qs(np);
+  for (size_t p = 0; p < np; p++) {
+    auto& q = Qs[p];
+    for (int j = 0; j < NADV; j++) q.push_back({j, x[p]});
+    q.push_back({10, x[p]});
+    q.push_back({10, xn1[p]});
+    q.push_back({11, x[p]});
+    q.push_back({11, xn1[p]});
+    q.push_back({10, xl[p]});
+    for (int j = 0; j < NFIX; j++) q.push_back({20 + j, x[p]});
+    for (int j = 0; j < NADV; j++) q.push_back({30 + j, x[p]});
+    q.push_back({40, x[p]});
+  }
+  std::vector<Fr> x3(np), x4(np);
+  for (size_t p = 0; p < np; p++) {
+    ProofBufs& b = B[p];
+    const Fr split = fr_pow_u64(x[p], n - 1);
+    const Fr xn = host::fr_mul(split, x[p]);
+    Fr sp = host::fr_sub(xn, one);
+    std::vector<std::pair<const void*, size_t>> lp;
+    std::vector<Fr> lc;
+    for (int i = 0; i < QDEG; i++) {
+      lp.push_back({b.limbs[i], n});
+      lc.push_back(fr_neg(sp));
+      sp = host::fr_mul(sp, split);
+    }
+    PV_TRY(lincomb(c, lp, lc, b.lin, n));
+  }
+  std::vector<Fr> x1(np), x2(np);
+  for (size_t p = 0; p < np; p++) {
+    x1[p] = T[p].squeeze();
+    x2[p] = T[p].squeeze();
+  }
+  for (size_t p = 0; p < np; p++) {
+    ProofBufs& b = B[p];
+    labels.clear();
+    set_of.clear();
+    point_sets.clear();
+    intermediate_sets(Qs[p], labels, set_of, point_sets);
+    auto poly_of = [&](int label) -> std::pair<const void*, size_t> {
+      if (label < NADV) return {b.polys[label], n};
+      if (label == 10) return {b.polys[NADV], n};
+      if (label == 11) return {b.polys[NADV + 1], n};
+      if (label >= 20 && label < 20 + NFIX) return {key->fixed_polys[label - 20], n};
+      if (label >= 30 && label < 30 + NADV) return {key->sigma_polys[label - 30], n};
+      return {b.lin, n};
+    };
+    std::vector<std::vector<std::pair<const void*, size_t>>> members(point_sets.size());
+    for (size_t i = 0; i < labels.size(); i++) members[set_of[i]].push_back(poly_of(labels[i]));
+    std::vector<int> ord(point_sets.size());
+    for (size_t i = 0; i < ord.size(); i++) ord[i] = (int)i;
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int a, int bb) { return point_sets[a].size() < point_sets[bb].size(); });
+    MZK_REQUIRE(ord.size() == 3, "mzk_plonk_prove: unexpected query structure");
+    // q polynomials: sum of the set's members with powers of x1 (into q0, k1, k2 by sorted slot)
+    uint32_t* qbuf[3] = {b.q0, b.k1, b.k2};
+    std::vector<std::pair<const void*, size_t>> qs(3);
+    for (int slot = 0; slot < 3; slot++) {
+      auto& m = members[ord[slot]];
+      std::vector<Fr> pw;
+      Fr t = one;
+      for (size_t i = 0; i < m.size(); i++) {
+        pw.push_back(t);
+        t = host::fr_mul(t, x1[p]);
+      }
+      PV_TRY(lincomb(c, m, pw, qbuf[slot], n));
+      qs[slot] = {qbuf[slot], n};
+    }
+    // f = sum_slot x2^slot q_slot / prod (X - point)
+    uint32_t* fout[3] = {b.fa, b.fb, b.fc};
+    std::vector<std::pair<const void*, size_t>> fparts;
+    for (int slot = 0; slot < 3; slot++) {
+      const auto& pts = point_sets[ord[slot]];
+      const void* cur = qs[slot].first;
+      size_t len = n;
+      uint32_t* tmp[2] = {b.fin, b.pi};   // free until the final phase
+      for (size_t i = 0; i < pts.size(); i++) {
+        uint32_t* dst = i + 1 == pts.size() ? fout[slot] : tmp[i & 1];
+        PV_TRY(mzk_fr_kate_div(c, cur, len, pts[i].l, dst));
+        cur = dst;
+        len -= 1;
+      }
+      fparts.push_back({fout[slot], len});
+    }
+    std::vector<Fr> pw = {one, x2[p], host::fr_mul(x2[p], x2[p])};
+    PV_TRY(lincomb(c, fparts, pw, b.f, n - 1));
+  }
+  {
+    std::vector<std::pair<const void*, size_t>> jobs;
+    for (size_t p = 0; p < np; p++) jobs.push_back({B[p].f, n - 1});
+    PV_TRY(msm_batch(c, key->g, jobs, jac));
+    for (size_t p = 0; p < np; p++) {
+      uint8_t cb[48];
+      compress(&jac[18 * p], cb);
+      T[p].write_point(cb);
+    }
+  }
+  {
+    std::vector<std::pair<const void*, size_t>> polys;
+    std::vector<Fr> pts;
+    for (size_t p = 0; p < np; p++) {
+      x3[p] = T[p].squeeze();
+      polys.push_back({B[p].q0, n}), pts.push_back(x3[p]);
+      polys.push_back({B[p].k1, n}), pts.push_back(x3[p]);
+      polys.push_back({B[p].k2, n}), pts.push_back(x3[p]);
+    }
+    std::vector<Fr> ev;
+    PV_TRY(eval_batch(c, polys, pts, ev));
+    for (size_t p = 0; p < np; p++)
+      for (int i = 0; i < 3; i++) T[p].write_fr(ev[3 * p + i]);
+  }
+  {
+    std::vector<std::pair<const void*, size_t>> polys;
+    std::vector<Fr> pts;
+    for (size_t p = 0; p < np; p++) {
+      ProofBufs& b = B[p];
+      x4[p] = T[p].squeeze();
+      Fr t = one;
+      std::vector<Fr> pw;
+      for (int i = 0; i < 4; i++) {
+        pw.push_back(t);
+        t = host::fr_mul(t, x4[p]);
+      }
+      std::vector<std::pair<const void*, size_t>> fp = {{b.q0, n}, {b.k1, n}, {b.k2, n}, {b.f, n - 1}};
+      PV_TRY(lincomb(c, fp, pw, b.fin, n));
+      polys.push_back({b.fin, n}), pts.push_back(x3[p]);
+    }
+    std::vector<Fr> v;
+    PV_TRY(eval_batch(c, polys, pts, v));
+    for (size_t p = 0; p < np; p++) {
+      ProofBufs& b = B[p];
+      PV_TRY(vscal(c, 1, b.fin, fr_neg(v[p]), b.fin, 1));   // final(X) - v
+      PV_TRY(mzk_fr_kate_div(c, b.fin, n, x3[p].l, b.pi));
+    }
+    std::vector<std::pair<const void*, size_t>> jobs;
+    for (size_t p = 0; p < np; p++) jobs.push_back({B[p].pi, n - 1});
+    PV_TRY(msm_batch(c, key->g, jobs, jac));
+    for (size_t p = 0; p < np; p++) {
+      uint8_t cb[48];
+      compress(&jac[18 * p], cb);
+      T[p].write_point(cb);
+    }
+  }
+  mark(7);
+  for (size_t p = 0; p < np; p++) {
+    MZK_REQUIRE(T[p].proof.size() == MZK_PLONK_PROOF_BYTES, "mzk_plonk_prove: proof of %zu bytes",
+                T[p].proof.size());
+    memcpy(proofs + MZK_PLONK_PROOF_BYTES * p, T[p].proof.data(), MZK_PLONK_PROOF_BYTES);
+  }
+  return MZK_OK;
+}
+
+}  // extern "C"

@@ -71,6 +71,19 @@ class PlonkNumerator(ctypes.Structure):
                 ("delta_beta", (ctypes.c_uint64 * 4) * 8), ("t_evals", (ctypes.c_uint64 * 4) * 16)]
 
 
+class PlonkKey(ctypes.Structure):
+    """mzk_plonk_key: the native prover's view of a proving key (device pointers)."""
+    _fields_ = [("k", ctypes.c_uint32), ("g", ctypes.c_void_p), ("g_lagrange", ctypes.c_void_p),
+                ("fixed_polys", ctypes.c_void_p * 5), ("fixed_cosets", ctypes.c_void_p * 5),
+                ("sigma_values", ctypes.c_void_p * 5), ("sigma_polys", ctypes.c_void_p * 5),
+                ("sigma_cosets", ctypes.c_void_p * 5),
+                ("l0", ctypes.c_void_p), ("l_last", ctypes.c_void_p), ("l_active", ctypes.c_void_p),
+                ("coset_x", ctypes.c_void_p), ("omega_powers", ctypes.c_void_p),
+                ("transcript_repr", ctypes.c_uint64 * 4)]
+
+
+PLONK_PROOF_BYTES = 1360
+
 _lib = None
 
 # (name, argtypes, restype); every int-returning function is a status code
@@ -130,6 +143,7 @@ SIGNATURES = {
     "mzk_fr_eval": ([_P, _P, _S, _P, _P], _I),
     "mzk_fr_eval_batch": ([_P, _P, _P, _P, _S, _P], _I),
     "mzk_plonk_numerator_dev": ([_P, _P, _P], _I),
+    "mzk_plonk_prove": ([_P, _P, _S, _P, _P, _P, _P], _I),
     "mzk_fr_lincomb_dev": ([_P, _P, _P, _P, _S, _P, _S], _I),
     "mzk_fr_kate_div": ([_P, _P, _S, _P, _P], _I),
     "mzk_fr_powers": ([_P, _P, _S, _P], _I),

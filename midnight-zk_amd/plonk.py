@@ -275,6 +275,7 @@ def keygen(be, params: Params, circuit: SyntheticCircuit) -> ProvingKey:
         lagrange_unit(list(range(n - BLINDING_FACTORS, n)))))
     pk.l_last = dom.coeff_to_extended(dom.lagrange_to_coeff(lagrange_unit([n - BLINDING_FACTORS - 1])))
     pk.l_active_row = be.add_scalar(be.neg(be.add(pk.l_last, l_blind)), 1)
+    pk.omega_pows = be.powers(omega_for(k), n)
     # the coset points zeta * omega_ext^i of the extended domain
     pk.coset_x = be.scale(be.powers(omega_for(dom.extended_k), pk.ext_n), ZETA)
     pk.transcript_repr = vk_transcript_repr(k, dom.extended_k, pk.fixed_commitments,
@@ -521,6 +522,49 @@ def create_proof(be, params: Params, pk: ProvingKey, advice_values, rng_seed: in
         except StopIteration as stop:
             return stop.value
         resp = serve_request(be, req)
+
+
+def create_proofs_native(be, params: Params, pk: ProvingKey, advices, rng_seeds, phase_ms: list | None = None):
+    """Proofs of one size in lockstep by the library's native prover (mzk_plonk_prove):
+    the same proofs create_proof makes, one batched MSM / NTT / evaluation call per
+    phase for the whole group.  advices: one five-column witness per proof."""
+    import ctypes
+    mzk = be.mzk
+    key = mzk.PlonkKey()
+    key.k = pk.k
+    key.g = params.g.h.value if hasattr(params.g.h, "value") else params.g.h
+    key.g_lagrange = params.g_lagrange.h.value if hasattr(params.g_lagrange.h, "value") else params.g_lagrange.h
+    p = be._p
+    for j in range(NUM_FIXED):
+        key.fixed_polys[j] = p(pk.fixed_polys[j])
+        key.fixed_cosets[j] = p(pk.fixed_cosets[j])
+    for j in range(NUM_ADVICE):
+        key.sigma_values[j] = p(pk.sigma_values[j])
+        key.sigma_polys[j] = p(pk.sigma_polys[j])
+        key.sigma_cosets[j] = p(pk.sigma_cosets[j])
+    key.l0, key.l_last, key.l_active = p(pk.l0), p(pk.l_last), p(pk.l_active_row)
+    key.coset_x, key.omega_powers = p(pk.coset_x), p(pk.omega_pows)
+    key.transcript_repr[:] = [int(w) for w in mont_limbs(pk.transcript_repr)]
+    advices = list(advices)
+    m = len(advices)
+    cols = []
+    for adv in advices:
+        if len(adv) != NUM_ADVICE:
+            raise ValueError("each witness has five columns")
+        cols += [p(v) for v in adv]
+    arr = (ctypes.c_void_p * max(1, len(cols)))(*cols)
+    seeds = np.array([int(s) & M64 for s in rng_seeds], dtype=np.uint64)
+    if len(seeds) != m:
+        raise ValueError("one rng seed per proof")
+    out = np.zeros(max(1, m) * mzk.PLONK_PROOF_BYTES, dtype=np.uint8)
+    ph = np.zeros(8, dtype=np.float64)
+    be.sync()
+    mzk._check(be._L.mzk_plonk_prove(be.ctx.h, ctypes.addressof(key), m, ctypes.cast(arr, ctypes.c_void_p),
+                                     seeds.ctypes.data, out.ctypes.data,
+                                     ph.ctypes.data if phase_ms is not None else None))
+    if phase_ms is not None:
+        phase_ms[:] = list(ph)
+    return [bytes(out[i * mzk.PLONK_PROOF_BYTES:(i + 1) * mzk.PLONK_PROOF_BYTES]) for i in range(m)]
 
 
 def construct_intermediate_sets(queries):
