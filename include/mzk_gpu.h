@@ -86,6 +86,26 @@ int mzk_srs_to_lagrange(mzk_ctx *ctx, const mzk_srs *g, uint32_t k, mzk_srs **ou
 /* suffix_sum (proofs/src/poly/kzg/params.rs:47-57): out_i = sum_(j >= i) in_j, the
  * g_lagrange_delta bases (and, applied to those, g_lagrange_double_delta). */
 int mzk_srs_suffix_sum(mzk_ctx *ctx, const mzk_srs *in, mzk_srs **out);
+/* G1 point encodings of the SRS file (proofs/src/utils/helpers.rs:16-28 SerdeFormat,
+ * same order). */
+enum {
+  MZK_SERDE_PROCESSED = 0,          /* 48 B compressed (ZCash flags 0x80/0x40/0x20) */
+  MZK_SERDE_RAW_BYTES = 1,          /* 96 B uncompressed big-endian x || y */
+  MZK_SERDE_RAW_BYTES_UNCHECKED = 2 /* the same bytes; on-curve check only */
+};
+/* The point arrays of ParamsKZG::read_custom (params.rs:286-336): count encoded G1
+ * points (48 or 96 bytes each, host memory) decoded on the device into a new SRS.
+ * Processed decompresses (from_compressed, g1.rs:367-370) and RawBytes parses
+ * (from_uncompressed, read_raw g1.rs:472-484); both check the curve equation and
+ * membership of the prime-order subgroup; RawBytesUnchecked (read_raw_unchecked,
+ * g1.rs:465-470) checks the curve equation only.  Any failing point fails the call
+ * with MZK_ERR_INVALID naming the lowest failing index (the reference returns an
+ * io::Error or panics).  The file header (k) and the G2 points are the caller's. */
+int mzk_srs_read_points(mzk_ctx *ctx, const uint8_t *bytes, size_t count, int format, mzk_srs **out);
+/* The point arrays of ParamsKZG::write_custom (params.rs:259-281): bases[offset,
+ * offset+count) encoded to host memory (48 B per point for Processed, else 96 B). */
+int mzk_srs_write_points(mzk_ctx *ctx, const mzk_srs *srs, size_t offset, size_t count, int format,
+                         uint8_t *bytes);
 size_t mzk_srs_len(const mzk_srs *srs);
 /* Device pointer to the n x 12 u64 affine bases. */
 const void *mzk_srs_bases_dev(const mzk_srs *srs);

@@ -166,9 +166,6 @@ __global__ void __launch_bounds__(64) k_suffix_final(const uint32_t* __restrict_
   }
 }
 
-int srs_alloc_bases(mzk_ctx* c, size_t n, mzk_srs** out);
-int srs_finish_bases(mzk_ctx* c, mzk_srs* s);
-
 static void fr_canonical(const host::Fr& a, uint32_t out[8]) {
   host::Fr one_raw = host::fr_zero();
   one_raw.l[0] = 1;

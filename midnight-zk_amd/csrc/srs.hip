@@ -228,8 +228,6 @@ static int srs_alloc(mzk_ctx* c, size_t n, mzk_srs** out) {
 }
 
 namespace mzk {
-// For the other SRS builders (g1fft.hip): a new SRS of n bases (device set by the
-// caller), and its shifted tables once the bases are written.
 int srs_alloc_bases(mzk_ctx* c, size_t n, mzk_srs** out) { return srs_alloc(c, n, out); }
 int srs_finish_bases(mzk_ctx* c, mzk_srs* s) { return srs_finish(c, s); }
 }  // namespace mzk

@@ -136,6 +136,8 @@ SIGNATURES = {
     "mzk_srs_from_scalars_dev": ([_P, _P, _S, ctypes.POINTER(_P)], _I),
     "mzk_srs_to_lagrange": ([_P, _P, _U32, ctypes.POINTER(_P)], _I),
     "mzk_srs_suffix_sum": ([_P, _P, ctypes.POINTER(_P)], _I),
+    "mzk_srs_read_points": ([_P, _P, _S, _I, ctypes.POINTER(_P)], _I),
+    "mzk_srs_write_points": ([_P, _P, _S, _S, _I, _P], _I),
     "mzk_fr_vec_op": ([_P, _I, _P, _P, _P, _S], _I),
     "mzk_fr_vec_scalar": ([_P, _I, _P, _P, _P, _S], _I),
     "mzk_fr_batch_invert": ([_P, _P, _S], _I),
@@ -182,6 +184,12 @@ def _check(rc: int):
 def _np_ptr(a: np.ndarray):
     if a.dtype != np.uint64 or not a.flags["C_CONTIGUOUS"]:
         raise MzkError("expected a C-contiguous uint64 array")
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _byte_ptr(a: np.ndarray):
+    if a.dtype != np.uint8 or not a.flags["C_CONTIGUOUS"]:
+        raise MzkError("expected a C-contiguous uint8 array")
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
@@ -400,6 +408,27 @@ class ResidentSrs:
         _check(load_library().mzk_srs_suffix_sum(self.ctx.h, self.h, ctypes.byref(h)))
         return ResidentSrs(self.ctx, h)
 
+    @classmethod
+    def read_points(cls, ctx: Context, data, count: int, fmt: int) -> "ResidentSrs":
+        """count encoded G1 points (48 B compressed for SERDE_PROCESSED, else 96 B) decoded
+        and checked on the device (ParamsKZG::read_custom's point arrays,
+        kzg/params.rs:286-336)."""
+        buf = np.frombuffer(data, dtype=np.uint8)
+        need = count * point_bytes(fmt)
+        if buf.size < need:
+            raise ValueError(f"{need} bytes of points needed, {buf.size} given")
+        buf = np.ascontiguousarray(buf[:need])
+        h = ctypes.c_void_p()
+        _check(load_library().mzk_srs_read_points(ctx.h, _byte_ptr(buf), count, fmt, ctypes.byref(h)))
+        return cls(ctx, h)
+
+    def write_points(self, fmt: int, offset: int = 0, count: int | None = None) -> bytes:
+        """The bases encoded as in ParamsKZG::write_custom (kzg/params.rs:259-281)."""
+        count = self.n - offset if count is None else count
+        out = np.zeros(count * point_bytes(fmt), dtype=np.uint8)
+        _check(load_library().mzk_srs_write_points(self.ctx.h, self.h, offset, count, fmt, _byte_ptr(out)))
+        return out.tobytes()
+
     def bases_ptr(self) -> int:
         return load_library().mzk_srs_bases_dev(self.h)
 
@@ -437,6 +466,62 @@ class ResidentSrs:
             self.close()
         except Exception:
             pass
+
+
+# SerdeFormat (proofs/src/utils/helpers.rs:16-28), in the reference's order
+SERDE_PROCESSED, SERDE_RAW_BYTES, SERDE_RAW_BYTES_UNCHECKED = 0, 1, 2
+
+
+def point_bytes(fmt: int, g2: bool = False) -> int:
+    """Encoded size of one G1 (or G2) point: compressed for Processed, else uncompressed."""
+    if fmt not in (SERDE_PROCESSED, SERDE_RAW_BYTES, SERDE_RAW_BYTES_UNCHECKED):
+        raise ValueError(f"unknown SerdeFormat {fmt}")
+    base = 96 if g2 else 48
+    return base if fmt == SERDE_PROCESSED else 2 * base
+
+
+class KzgParams:
+    """ParamsKZG (proofs/src/poly/kzg/params.rs:22-37) with its four G1 bases resident in
+    HBM: g, g_lagrange, and the delta bases suffix_sum(g_lagrange) and
+    suffix_sum(g_lagrange_delta) that read_custom derives (params.rs:338-340).  The G2
+    points are kept as the bytes read (the prover's MSMs never use them)."""
+
+    def __init__(self, k: int, g: ResidentSrs, g_lagrange: ResidentSrs, g2: bytes, s_g2: bytes, g2_format: int):
+        self.k = k
+        self.g = g
+        self.g_lagrange = g_lagrange
+        self.g_lagrange_delta = g_lagrange.suffix_sum()
+        self.g_lagrange_double_delta = self.g_lagrange_delta.suffix_sum()
+        self.g2, self.s_g2, self.g2_format = g2, s_g2, g2_format
+
+    @classmethod
+    def read_custom(cls, ctx: Context, data, fmt: int) -> "KzgParams":
+        """ParamsKZG::read_custom (params.rs:284-349): k as u32 LE, 2^k points g, 2^k points
+        g_lagrange, then g2 and s_g2.  Raises MzkError on a point that fails its checks
+        and ValueError on a short buffer."""
+        mv = memoryview(data).cast("B")
+        if len(mv) < 4:
+            raise ValueError("truncated parameters: no k")
+        k = int.from_bytes(mv[:4], "little")
+        if k > 30:
+            raise ValueError(f"k = {k} out of range")
+        n = 1 << k
+        pb, gb = point_bytes(fmt), point_bytes(fmt, g2=True)
+        need = 4 + 2 * n * pb + 2 * gb
+        if len(mv) < need:
+            raise ValueError(f"truncated parameters: {need} bytes needed, {len(mv)} given")
+        g = ResidentSrs.read_points(ctx, mv[4:4 + n * pb], n, fmt)
+        gl = ResidentSrs.read_points(ctx, mv[4 + n * pb:4 + 2 * n * pb], n, fmt)
+        o = 4 + 2 * n * pb
+        return cls(k, g, gl, bytes(mv[o:o + gb]), bytes(mv[o + gb:o + 2 * gb]), fmt)
+
+    def write_custom(self, fmt: int) -> bytes:
+        """ParamsKZG::write_custom (params.rs:259-281).  The G2 points are written as read,
+        so fmt must select the same G2 encoding (compressed or not) they came in."""
+        if (fmt == SERDE_PROCESSED) != (self.g2_format == SERDE_PROCESSED):
+            raise NotImplementedError("re-encoding G2 points between compressed and uncompressed forms")
+        return (self.k.to_bytes(4, "little") + self.g.write_points(fmt) + self.g_lagrange.write_points(fmt)
+                + self.g2 + self.s_g2)
 
 
 def g1_to_affine(jac: np.ndarray) -> np.ndarray:

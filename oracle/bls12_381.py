@@ -284,6 +284,69 @@ def g1_to_uncompressed(P) -> bytes:
     return P[0].to_bytes(48, "big") + P[1].to_bytes(48, "big")
 
 
+class PointDecodeError(ValueError):
+    """reason: 'encoding', 'curve' or 'subgroup'."""
+
+    def __init__(self, reason: str):
+        super().__init__(reason)
+        self.reason = reason
+
+
+def g1_in_subgroup(P) -> bool:
+    """[r] P = O: the test that defines G1 (r prime, cofactor prime to r)."""
+    return P is None or g1_mul(P, R_MOD) is None
+
+
+def g1_from_compressed(b: bytes, check_subgroup: bool = True):
+    """Inverse of g1_to_compressed per the ZCash serialisation (from_compressed,
+    g1.rs:367-384 via blst): flags checked, x < p, y = sqrt(x^3 + 4) chosen by the
+    0x20 flag; raises PointDecodeError."""
+    if len(b) != 48:
+        raise PointDecodeError("encoding")
+    flags = b[0] >> 5
+    x = int.from_bytes(bytes([b[0] & 0x1F]) + b[1:], "big")
+    if not flags & 4:
+        raise PointDecodeError("encoding")
+    if flags & 2:
+        if flags & 1 or x:
+            raise PointDecodeError("encoding")
+        return None
+    if x >= P_MOD:
+        raise PointDecodeError("encoding")
+    rhs = (x * x * x + G1_B) % P_MOD
+    y = pow(rhs, (P_MOD + 1) // 4, P_MOD)   # p = 3 mod 4
+    if y * y % P_MOD != rhs:
+        raise PointDecodeError("curve")
+    if (y > (P_MOD - 1) // 2) != bool(flags & 1):
+        y = P_MOD - y
+    if check_subgroup and not g1_in_subgroup((x, y)):
+        raise PointDecodeError("subgroup")
+    return (x, y)
+
+
+def g1_from_uncompressed(b: bytes, check_subgroup: bool = True):
+    """Inverse of g1_to_uncompressed (from_uncompressed / _unchecked, g1.rs:337-365):
+    flags, x, y < p, the curve equation, and membership of G1 when asked."""
+    if len(b) != 96:
+        raise PointDecodeError("encoding")
+    flags = b[0] >> 5
+    x = int.from_bytes(bytes([b[0] & 0x1F]) + b[1:48], "big")
+    y = int.from_bytes(b[48:], "big")
+    if flags & 4 or flags & 1:
+        raise PointDecodeError("encoding")
+    if flags & 2:
+        if x or y:
+            raise PointDecodeError("encoding")
+        return None
+    if x >= P_MOD or y >= P_MOD:
+        raise PointDecodeError("encoding")
+    if not g1_is_on_curve((x, y)):
+        raise PointDecodeError("curve")
+    if check_subgroup and not g1_in_subgroup((x, y)):
+        raise PointDecodeError("subgroup")
+    return (x, y)
+
+
 def g1_affine_mont_limbs(P) -> list[int]:
     """blst_p1_affine layout: x[6] || y[6] u64 LE Montgomery; identity = zeros
     (g1.rs:32-34)."""
