@@ -285,6 +285,30 @@ int mzk_prof_read(mzk_ctx *ctx, char *buf, size_t len);
  * sets, references per accumulation thread K, non-zero window digits M, n}. */
 int mzk_msm_last_plan(mzk_ctx *ctx, uint64_t out[6]);
 
+/* ------------------------------------------------------------------ autotuner
+ * No reference counterpart (the reference fixes c = ceil(ln n), msm.rs:464-470, and
+ * its FFT recursion depth by thread count, fft.rs:49-101).  The planners' choices --
+ * resident-SRS MSM path (per-window or shared buckets), window bits c, accumulation
+ * threads per CU (references per thread), NTT four-step split and workgroup target --
+ * come from cost models unless a process-wide table, filled by the tuner or imported,
+ * says otherwise.  Every plan returns the same point / DFT values; the tuner checks
+ * that for each candidate it times. */
+/* Times every MSM candidate (c in {4, 8, 16} x {256, 512, 1024} threads/CU, and the
+ * shared path where srs has shifted tables) on seeded scalars against srs at n = 2^lo
+ * .. 2^hi (best of reps) and keeps the fastest per size. */
+int mzk_tune_msm(mzk_ctx *ctx, const mzk_srs *srs, uint32_t log_lo, uint32_t log_hi, uint32_t reps);
+/* The same for forward NTTs of 2^lo .. 2^hi (sizes above 2^11: splits within two of
+ * the even split x workgroup targets {256, 1024, 4096}). */
+int mzk_tune_ntt(mzk_ctx *ctx, uint32_t log_lo, uint32_t log_hi, uint32_t reps);
+/* The table as text lines "msm <log_n> <path> <c> <tpc_window> <tpc_shared>" and
+ * "ntt <log_n> <l1> <groups>" (zero / -1 = model); *needed = bytes incl. the NUL.
+ * len = 0 only reports the size. */
+int mzk_tune_export(char *buf, size_t len, size_t *needed);
+/* Merges such text into the table ('#' starts a comment line); all or nothing. */
+int mzk_tune_import(const char *text);
+/* Back to the models. */
+void mzk_tune_reset(void);
+
 /* ------------------------------------------------------------------ diagnostics
  * Element-wise field / curve operations for parity tests (device pointers, n elements):
  *   field 0 = fr (8 u32 / element), 1 = fp (12 u32 / element);

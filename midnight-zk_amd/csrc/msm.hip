@@ -37,6 +37,7 @@
 #include "g1_28.hpp"
 #include "host_ec.hpp"
 #include "msm.hpp"
+#include "tune.hpp"
 
 namespace mzk {
 
@@ -491,7 +492,8 @@ static int best_c(size_t n, int per_window) {
 
 MsmPlan msm_plan(size_t n, int per_window, int num_cus, int c_override) {
   MsmPlan p;
-  int c = c_override > 0 ? c_override : best_c(n, per_window);
+  const MsmTune t = tune_msm_get(n);
+  int c = c_override > 0 ? c_override : (per_window && t.c > 0 ? t.c : best_c(n, per_window));
   p.c = c;
   p.W = 255 / c + 1;
   p.B = 1u << (c - 1);
@@ -499,7 +501,8 @@ MsmPlan msm_plan(size_t n, int per_window, int num_cus, int c_override) {
   p.sets = per_window ? p.W : 1;
   p.NB = (uint32_t)p.sets * p.B;
   size_t M = (size_t)p.W * n;
-  size_t target_threads = (size_t)std::max(num_cus, 1) * 512;
+  const uint32_t tpc = per_window ? (t.tpc_window ? t.tpc_window : 512) : (t.tpc_shared ? t.tpc_shared : 512);
+  size_t target_threads = (size_t)std::max(num_cus, 1) * tpc;
   size_t K = M / target_threads;
   p.K = (uint32_t)std::max<size_t>(2, std::min<size_t>(256, K));
   return p;

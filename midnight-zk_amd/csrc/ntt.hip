@@ -20,6 +20,7 @@
 #include "field.hpp"
 #include "host_ec.hpp"
 #include "ntt.hpp"
+#include "tune.hpp"
 
 namespace mzk {
 
@@ -192,7 +193,9 @@ NttPlan::~NttPlan() {
 int NttPlan::init(hipStream_t st, uint32_t log_n_, const host::Fr& omega) {
   log_n = log_n_;
   l1 = (log_n + 1) / 2;
-  l2 = log_n / 2;
+  const NttTune t = tune_ntt_get(log_n);
+  if (t.l1 && t.l1 <= (uint32_t)NTT_MAX_LOG_L && log_n > t.l1 && log_n - t.l1 <= (uint32_t)NTT_MAX_LOG_L) l1 = t.l1;
+  l2 = log_n - l1;
   const uint32_t N1 = 1u << l1, N2 = 1u << l2;
   const uint32_t lo_cnt = 1u << std::min<uint32_t>(TWIST_LO_BITS, log_n);
   const uint32_t hi_cnt = (uint32_t)std::max<uint64_t>(1, ((uint64_t)1 << log_n) >> TWIST_LO_BITS);
@@ -219,9 +222,9 @@ int NttPlan::init(hipStream_t st, uint32_t log_n_, const host::Fr& omega) {
   return MZK_OK;
 }
 
-static uint32_t choose_c(uint32_t L, uint64_t nbatch) {
+static uint32_t choose_c(uint32_t L, uint64_t nbatch, uint32_t groups) {
   uint32_t cmax = NTT_LDS_ELEMS / L;
-  uint64_t want = nbatch / 1024;   // keep >= ~1024 workgroups when the size allows
+  uint64_t want = nbatch / (groups ? groups : 1024);   // keep >= ~1024 workgroups when the size allows
   uint32_t c = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cmax, want));
   while (nbatch % c) c >>= 1;
   return std::max(c, 1u);
@@ -281,7 +284,8 @@ static int build_passes(const NttPlan& plan, const uint32_t* src, uint32_t* dst,
   a.out_bstride = 1;
   a.out_kstride = N1;
   a.log_l = plan.l2;
-  a.C = choose_c((uint32_t)N2, N1);
+  const uint32_t groups_target = tune_ntt_get(log_n).groups;
+  a.C = choose_c((uint32_t)N2, N1, groups_target);
   a.out_mode = 0;
   pa[0] = a;
   groups[0] = N1 / a.C;
@@ -298,7 +302,7 @@ static int build_passes(const NttPlan& plan, const uint32_t* src, uint32_t* dst,
   b.out_bstride = 1;
   b.out_kstride = N2;
   b.log_l = plan.l1;
-  b.C = choose_c((uint32_t)N1, N2);
+  b.C = choose_c((uint32_t)N1, N2, groups_target);
   b.out_mode = out_mode;
   pa[1] = b;
   groups[1] = N2 / b.C;
@@ -323,7 +327,8 @@ int ntt_run(mzk_ctx* c, const NttPlan& plan, const uint32_t* src, uint32_t* dst,
 
 // Cached plan for (log_n, inverse) on a context: omega = ROOT^(2^(32 - log_n)) or its inverse.
 const NttPlan* ctx_plan(mzk_ctx* c, uint32_t log_n, int inverse) {
-  const int key = (int)log_n * 2 + (inverse ? 1 : 0);
+  // keyed by the split too: a plan made before a tuning run keeps its own tables
+  const int key = ((int)tune_ntt_get(log_n).l1 << 8) | ((int)log_n * 2 + (inverse ? 1 : 0));
   auto it = c->ntt_plans.find(key);
   if (it != c->ntt_plans.end()) return it->second;
   host::Fr w = host::fr_root_of_unity(log_n);

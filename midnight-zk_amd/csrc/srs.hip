@@ -12,6 +12,7 @@
 #include "host_ec.hpp"
 #include "msm.hpp"
 #include "srs.hpp"
+#include "tune.hpp"
 
 namespace mzk {
 
@@ -360,7 +361,8 @@ int mzk_msm_srs_dev(mzk_ctx* c, const mzk_srs* s, size_t offset, const void* d_c
   MZK_REQUIRE(d_coeffs != nullptr, "mzk_msm_srs_dev: null coefficients");
   std::lock_guard<std::mutex> g(c->mu);
   MZK_HIP(hipSetDevice(c->device));
-  if (s->shifted && n >= SHARED_MIN_TERMS) {
+  const MsmTune t = tune_msm_get(n);
+  if (s->shifted && (t.path >= 0 ? t.path == 1 : n >= SHARED_MIN_TERMS)) {
     MsmPlan p = msm_plan(n, 0, c->num_cus, s->shifted_c);
     return msm_run(c, p, (const uint32_t*)d_coeffs, n, (const uint32_t*)s->shifted, (uint32_t)s->n, (uint32_t)offset,
                    out);

@@ -136,6 +136,11 @@ SIGNATURES = {
     "mzk_srs_from_scalars_dev": ([_P, _P, _S, ctypes.POINTER(_P)], _I),
     "mzk_srs_to_lagrange": ([_P, _P, _U32, ctypes.POINTER(_P)], _I),
     "mzk_srs_suffix_sum": ([_P, _P, ctypes.POINTER(_P)], _I),
+    "mzk_tune_msm": ([_P, _P, _U32, _U32, _U32], _I),
+    "mzk_tune_ntt": ([_P, _U32, _U32, _U32], _I),
+    "mzk_tune_export": ([_P, _S, ctypes.POINTER(_S)], _I),
+    "mzk_tune_import": ([ctypes.c_char_p], _I),
+    "mzk_tune_reset": ([], None),
     "mzk_srs_read_points": ([_P, _P, _S, _I, ctypes.POINTER(_P)], _I),
     "mzk_srs_write_points": ([_P, _P, _S, _S, _I, _P], _I),
     "mzk_fr_vec_op": ([_P, _I, _P, _P, _P, _S], _I),
@@ -466,6 +471,34 @@ class ResidentSrs:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------- autotuner
+def tune_msm(ctx: Context, srs: "ResidentSrs", log_lo: int, log_hi: int, reps: int = 3) -> None:
+    """Time the MSM plan candidates at 2^log_lo..2^log_hi against srs; keep the fastest."""
+    _check(load_library().mzk_tune_msm(ctx.h, srs.h, log_lo, log_hi, reps))
+
+
+def tune_ntt(ctx: Context, log_lo: int, log_hi: int, reps: int = 3) -> None:
+    """Time the NTT split / workgroup candidates at 2^log_lo..2^log_hi; keep the fastest."""
+    _check(load_library().mzk_tune_ntt(ctx.h, log_lo, log_hi, reps))
+
+
+def tune_export() -> str:
+    L = load_library()
+    need = ctypes.c_size_t(0)
+    _check(L.mzk_tune_export(None, 0, ctypes.byref(need)))
+    buf = ctypes.create_string_buffer(need.value)
+    _check(L.mzk_tune_export(buf, need.value, ctypes.byref(need)))
+    return buf.value.decode()
+
+
+def tune_import(text: str) -> None:
+    _check(load_library().mzk_tune_import(text.encode()))
+
+
+def tune_reset() -> None:
+    load_library().mzk_tune_reset()
 
 
 # SerdeFormat (proofs/src/utils/helpers.rs:16-28), in the reference's order
