@@ -7,13 +7,16 @@ HBM when the timed region starts.  The line printed reports whole-job points/s,
 the roofline of the dominant kernel (bucket accumulation) from HIP-event times
 measured in this run, the scalar-field NTT at 2^22 beside it, 96 uneven commitments
 and NTTs batched against per-call invocations, full k = 18 proofs of the synthetic
-PLONK circuit ("prove": median ms per proof and its phase split), and the oracle's C
-restatement of the reference CPU msm_best timed on this host on the same inputs
-(which also checks the GPU result bit for bit), with the same prover on the CPU at
-k = 16 beside the GPU's and the two proofs compared byte for byte.
+PLONK circuit ("prove": median ms per proof by the library's native prover, its phase
+split, lockstep groups, and the Python-host prover beside it), Poisson-arrival serving
+of mixed sizes, and the oracle's C restatement of the reference CPU msm_best timed on
+this host on the same inputs (which also checks the GPU result bit for bit), with the
+same prover on the CPU at k = 16 beside the GPU's and the two proofs compared byte for
+byte.  Before anything is timed the autotuner picks the MSM and NTT plans on this
+device (--no-tune keeps the models; --tune-file imports a saved table).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--log-n 20] [--no-cpu-baseline]
-                  [--cpu-prove-k 16]
+                  [--cpu-prove-k 16] [--no-tune | --tune-file F] [--msm-only | --prove-only]
 """
 from __future__ import annotations
 
@@ -154,11 +157,42 @@ def bench_prove(pkg, ctx, k: int = 18, reps: int = 5) -> dict:
     for name, t in phases.items():
         split[name] = round((t - prev) * 1e3, 3)
         prev = t
+    # the library's native prover (C++ host code over the same C ABI): one proof, then a
+    # lockstep group of three different witnesses; its proofs must equal the ones above
+    native = plonk.create_proofs_native(be, params, pk, [wit], [1])[0]
+    ntimes = []
+    for _ in range(reps):
+        a = time.perf_counter()
+        again = plonk.create_proofs_native(be, params, pk, [wit], [1])[0]
+        ntimes.append(time.perf_counter() - a)
+        assert again == native
+    nph = [0.0] * 8
+    plonk.create_proofs_native(be, params, pk, [wit], [1], nph)
+    names = ["advice_commit", "permutation", "intt", "coset_ntt", "numerator", "quotient_commit", "evaluations",
+             "multiopen"]
+    nsplit = {nm: round(nph[i] - (nph[i - 1] if i else 0.0), 3) for i, nm in enumerate(names)}
+    wits = [wit] + [circ.witness(be, ws) for ws in (101, 102)]
+    seeds = [1, 2, 3]
+    group = plonk.create_proofs_native(be, params, pk, wits, seeds)
+    gtimes = []
+    for _ in range(max(1, reps // 2)):
+        a = time.perf_counter()
+        plonk.create_proofs_native(be, params, pk, wits, seeds)
+        gtimes.append(time.perf_counter() - a)
+    group_same = group[0] == proof and all(
+        g == plonk.create_proof(be, params, pk, w, rng_seed=sd) for g, w, sd in zip(group[1:], wits[1:], seeds[1:]))
     return {
-        "k": k, "prove_ms": round(statistics.median(times) * 1e3, 3),
-        "prove_ms_min": round(min(times) * 1e3, 3), "proofs": reps,
+        "k": k, "prove_ms": round(statistics.median(ntimes) * 1e3, 3),
+        "prove_ms_min": round(min(ntimes) * 1e3, 3), "proofs": reps,
+        "prover": "native (mzk_plonk_prove)",
+        "native_equals_python_host_prover": native == proof,
+        "phase_ms": nsplit,
+        "group3_ms_per_proof": round(statistics.median(gtimes) * 1e3 / 3, 3),
+        "group3_equal_to_single_proofs": group_same,
+        "python_host_prover": {"prove_ms": round(statistics.median(times) * 1e3, 3),
+                               "prove_ms_min": round(min(times) * 1e3, 3), "phase_ms": split},
         "proof_bytes": len(proof), "msms_per_proof": 13,
-        "phase_ms": split, "setup_ms": round((t1 - t0) * 1e3, 1), "keygen_ms": round((t2 - t1) * 1e3, 1),
+        "setup_ms": round((t1 - t0) * 1e3, 1), "keygen_ms": round((t2 - t1) * 1e3, 1),
         "witness_ms": round((t3 - t2) * 1e3, 2),
         "_proof": proof, "_params": params, "_circ": circ,
     }
@@ -192,15 +226,18 @@ def bench_serve(pkg, ctx, nreq: int, rates, ks=(12, 14, 16, 18, 20), seed: int =
     same = True
     for rate in rates:
         trace = serve.poisson_trace(nreq, rate, ks, seed)
+        res_n, st_n = server.run(trace, workers=workers, slots=slots, engine="native")
         res_b, st_b = server.run(trace, workers=workers, slots=slots)
         res_s, st_s = server.run(trace, workers=1, slots=1)
         out["loads"].append({
             "offered_per_s": rate,
+            "served_native": {**serve.latency_stats(res_n), "groups": st_n["rounds"],
+                              "max_inflight": st_n["max_inflight"]},
             "served": {**serve.latency_stats(res_b), "rounds": st_b["rounds"], "max_inflight": st_b["max_inflight"]},
             "one_at_a_time": serve.latency_stats(res_s),
         })
         byid = {r.rid: r for r in res_s}
-        same = same and all(r.proof == byid[r.rid].proof for r in res_b)
+        same = same and all(r.proof == byid[r.rid].proof for r in res_b + res_n)
     sample = serve.poisson_trace(nreq, rates[0], ks, seed)[::max(1, nreq // 6)]
     out["served_equals_one_at_a_time"] = same
     out["served_equals_alone_sample"] = all(
@@ -242,6 +279,12 @@ def main() -> int:
                     help="only run a 256 MiB elementwise copy and 3 MSM commits (for rocprofv3 --pmc runs)")
     ap.add_argument("--prove-only", action="store_true",
                     help="only run keygen and 3 proofs at k=18 (for rocprofv3 kernel traces of the prover)")
+    ap.add_argument("--msm-only", action="store_true",
+                    help="only the MSM steps and their kernel times (for rocprofv3 kernel traces of the MSM)")
+    ap.add_argument("--no-tune", action="store_true", help="keep the planners' built-in models")
+    ap.add_argument("--tune-file", default=None,
+                    help="import this plan table instead of tuning (the text mzk.tune_export writes)")
+    ap.add_argument("--tune-out", default=None, help="write the plan table used to this file")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the MSM/NTT path has no multi-GPU form in this build "
@@ -273,6 +316,22 @@ def main() -> int:
     ctx.fr_from_seed_dev(SEED, 1, n, sc)
     ctx.synchronize()
     torch.cuda.synchronize()
+
+    # ---- autotuner: MSM plans against this SRS and NTT splits, before anything is timed
+    tune = {"mode": "model"}
+    if args.tune_file:
+        with open(args.tune_file) as f:
+            mzk.tune_import(f.read())
+        tune = {"mode": "imported", "file": args.tune_file}
+    elif not args.no_tune and not args.traffic_probe:
+        t0 = time.perf_counter()
+        mzk.tune_msm(ctx, srs, 10, args.log_n, reps=2)
+        mzk.tune_ntt(ctx, 12, 22, reps=2)
+        tune = {"mode": "tuned", "seconds": round(time.perf_counter() - t0, 2)}
+    tune["table"] = mzk.tune_export().splitlines()
+    if args.tune_out:
+        with open(args.tune_out, "w") as f:
+            f.write(mzk.tune_export())
 
     if args.traffic_probe:
         # calibration: one elementwise pass that reads and writes exactly COPY_BYTES each
@@ -310,6 +369,13 @@ def main() -> int:
     alg_mads = digits * FP_MUL_PER_MADD * MAD_PER_FP_MUL
     peak = measure_mad_peak(mzk, ctx)
     achieved = alg_mads / (acc_ms * 1e-3)
+
+    if args.msm_only:
+        print(json.dumps({"msm_only": True, "n": n, "ms_per_step": round(dt * 1e3, 4),
+                          "value": round(n / dt, 1), "plan": plan, "tune": tune,
+                          "kernels_ms_per_step": {k: round(v[0] / v[1], 4) for k, v in sorted(prof.items())}}),
+              flush=True)
+        return 0
 
     # ---- NTT at 2^22 beside it (forward, resident input)
     ntt = {}
@@ -424,6 +490,7 @@ def main() -> int:
         "prove": prove,
         "serve": served,
         "setup": {"srs_generate_ms": round(srs_ms, 2)},
+        "tune": tune,
     }
     print(json.dumps(line), flush=True)
     srs.close()

@@ -14,7 +14,10 @@ batched operation returns exactly what the per-call operation returns.
 
 Several workers, each with its own library context (its own HIP stream), serve at
 once: their kernels run concurrently on the device, and one worker's host-side work and
-waits overlap the others' device work.
+waits overlap the others' device work.  With the native engine a worker proves the
+oldest arrived request together with the arrived requests of the same size as one
+lockstep group in the library's C++ prover (mzk_plonk_prove), outside the interpreter
+lock.
 
 The reference has no serving layer; this is the request loop around its prover
 (create_proof, proofs/src/plonk/prover.rs:432) for many concurrent callers.
@@ -91,11 +94,19 @@ class ProofServer:
         wit = self.circuits[k].witness(be, req.witness_seed)
         return plonk.create_proof(be, self.params[k], self.pks[k], wit, req.rng_seed)
 
-    def run(self, requests, realtime: bool = True, workers: int | None = None, slots: int | None = None):
+    def run(self, requests, realtime: bool = True, workers: int | None = None, slots: int | None = None,
+            engine: str = "steps"):
         """Serves the requests (arrival times relative to the start of the run; with
         realtime=False every request counts as arrived at once) on the first `workers`
-        backends with `slots` proofs per worker.  Returns ProofResults in completion
-        order and the run's statistics."""
+        backends with `slots` proofs per worker.  engine "steps": each worker drives its
+        in-flight proofs as prove_steps generators in batched rounds (requests join at
+        the next round); engine "native": each worker takes the oldest arrived request
+        together with the arrived requests of the same size (up to `slots`) and proves
+        them as one lockstep group in the library's native prover, which runs without
+        the interpreter lock, so the workers' host work overlaps too.  Returns
+        ProofResults in completion order and the run's statistics."""
+        if engine not in ("steps", "native"):
+            raise ValueError(f"unknown engine {engine!r}")
         import threading
         for r in requests:
             if r.k not in self.pks:
@@ -123,6 +134,60 @@ class ProofServer:
                     out.append(req)
                 nxt = queue[0].arrival if queue else None
             return out, nxt
+
+        def admit_group():
+            """Native engine: the oldest arrived request and the arrived ones of its size."""
+            with lock:
+                now = clock()
+                if not queue or queue[0].arrival > now:
+                    return [], (queue[0].arrival if queue else None)
+                k = queue[0].k
+                picked = []
+                for r in queue:
+                    if r.arrival > now or len(picked) == slots:
+                        break
+                    if r.k != k:
+                        continue
+                    if (state["inflight"] + len(picked) > 0
+                            and state["used"] + footprint_bytes(k) * (len(picked) + 1) > self.budget):
+                        break
+                    picked.append(r)
+                if not picked:
+                    return [], (queue[0].arrival if queue else None)
+                for r in picked:
+                    queue.remove(r)
+                state["used"] += footprint_bytes(k) * len(picked)
+                state["inflight"] += len(picked)
+                state["max_inflight"] = max(state["max_inflight"], state["inflight"])
+                state["rounds"] += 1
+                return picked, None
+
+        def native_worker(be):
+            try:
+                while True:
+                    group, nxt = admit_group()
+                    if not group:
+                        with lock:
+                            if not queue:
+                                return
+                        wait = 0.0005 if nxt is None else max(0.0002, min(nxt - clock(), 0.002))
+                        time.sleep(wait)
+                        continue
+                    start = time.perf_counter() - t0
+                    k = group[0].k
+                    wits = [self.circuits[k].witness(be, r.witness_seed) for r in group]
+                    proofs = plonk.create_proofs_native(be, self.params[k], self.pks[k], wits,
+                                                        [r.rng_seed for r in group])
+                    t = time.perf_counter() - t0
+                    with lock:
+                        for r, pr in zip(group, proofs):
+                            done.append(ProofResult(r.rid, r.k, r.arrival, start, t, pr))
+                        state["used"] -= footprint_bytes(k) * len(group)
+                        state["inflight"] -= len(group)
+            except BaseException as e:  # surfaced to the caller after the join
+                errors.append(e)
+                with lock:
+                    queue.clear()
 
         def worker(be):
             inflight = []
@@ -174,7 +239,8 @@ class ProofServer:
                 with lock:
                     queue.clear()
 
-        threads = [threading.Thread(target=worker, args=(be,), daemon=True) for be in bes]
+        target = native_worker if engine == "native" else worker
+        threads = [threading.Thread(target=target, args=(be,), daemon=True) for be in bes]
         for th in threads:
             th.start()
         for th in threads:
@@ -183,7 +249,7 @@ class ProofServer:
             raise errors[0]
         wall = time.perf_counter() - t0
         return done, {"rounds": state["rounds"], "max_inflight": state["max_inflight"], "wall_s": wall,
-                      "workers": len(bes), "slots": slots}
+                      "workers": len(bes), "slots": slots, "engine": engine}
 
 
 def poisson_trace(n: int, rate: float, ks, seed: int):

@@ -39,3 +39,23 @@ def test_poisson_arrivals(server):
     print(f"  {st} rounds={stats['rounds']} max_inflight={stats['max_inflight']}", flush=True)
     for r in results[:6]:
         assert r.proof == server.prove_alone(reqs[r.rid])
+
+
+def test_native_engine_groups_equal_alone(server):
+    print("\n[serve native engine, all arrived]", flush=True)
+    reqs = [serve.ProofRequest(rid=i, k=(8, 10, 12)[i % 3], witness_seed=700 + i, rng_seed=5 + i) for i in range(12)]
+    results, stats = server.run(reqs, realtime=False, workers=2, slots=3, engine="native")
+    assert len(results) == 12 and stats["engine"] == "native"
+    assert stats["rounds"] <= 6            # same-size requests share lockstep groups
+    for r in results:
+        assert r.proof == server.prove_alone(reqs[r.rid]), r.rid
+
+
+def test_native_engine_poisson(server):
+    print("\n[serve native engine, poisson 24 requests]", flush=True)
+    reqs = serve.poisson_trace(24, 80.0, (8, 10, 12), seed=13)
+    results, stats = server.run(reqs, workers=3, slots=2, engine="native")
+    assert sorted(r.rid for r in results) == list(range(24))
+    print(f"  {serve.latency_stats(results)} groups={stats['rounds']}", flush=True)
+    for r in results[::4]:
+        assert r.proof == server.prove_alone(reqs[r.rid])
