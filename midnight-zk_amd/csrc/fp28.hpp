@@ -14,6 +14,9 @@
 //    small enough that a column sum stays below 2^64: both operands' limbs < 2^30,
 //    or one operand's limbs < 2^28 and the other's < 2^31.
 //  * mul outputs are normalised (limbs 0..12 < 2^28) and < 2p.
+//  * mul_sum(a, b, c, d) = (a b + c d) 2^-392 needs a b + c d < 2520 p^2 and the
+//    summed columns below 2^64 (both pairs' limbs < 2^29.6, or one pair's < 2^28 and
+//    2^29); it saves one reduction where a formula subtracts two products.
 //  * sub(a, b) adds a multiple k p of p written with lower limbs raised by
 //    m 2^28 (and the next limb lowered by m), so every limb difference is >= 0
 //    when b's lower limbs are <= m (2^28 - 1) and b's top limb is <= that of k p
@@ -39,6 +42,7 @@ namespace f28c {
 constexpr int N = 14;
 constexpr uint32_t M28 = (1u << 28) - 1;
 constexpr uint32_t PINV = 0xfffcfffdu;   // -p^-1 mod 2^32 (its low 28 bits: -p^-1 mod 2^28)
+constexpr uint32_t P0INV = 0x30003u;     // p^-1 mod 2^28
 
 // p in radix 2^28
 MZK_HD constexpr uint32_t P(int i) {
@@ -141,6 +145,46 @@ MZK_HD f28 f28_mul(const f28& a, const f28& b) {
   return r;
 }
 
+// (a b + c d) 2^-392 mod p with one reduction: the two products share the column
+// accumulators.  Needs a b + c d < 2520 p^2 and column sums below 2^64.
+MZK_HD f28 f28_mul_sum(const f28& a, const f28& b, const f28& c, const f28& d) {
+  using namespace f28c;
+  uint32_t m[N];
+  f28 r;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    uint64_t red = 0;
+#pragma unroll
+    for (int j = 0; j < k; j++) {
+      acc += (uint64_t)a.l[j] * b.l[k - j];
+      red += (uint64_t)c.l[j] * d.l[k - j];
+      red += (uint64_t)m[j] * P(k - j);
+    }
+    acc += (uint64_t)a.l[k] * b.l[0];
+    red += (uint64_t)c.l[k] * d.l[0];
+    acc += red;
+    m[k] = ((uint32_t)acc * PINV) & M28;
+    acc += (uint64_t)m[k] * P(0);
+    acc >>= 28;
+  }
+#pragma unroll
+  for (int k = N; k < 2 * N - 1; k++) {
+    uint64_t red = 0;
+#pragma unroll
+    for (int j = k - N + 1; j < N; j++) {
+      acc += (uint64_t)a.l[j] * b.l[k - j];
+      red += (uint64_t)c.l[j] * d.l[k - j];
+      red += (uint64_t)m[j] * P(k - j);
+    }
+    acc += red;
+    r.l[k - N] = (uint32_t)acc & M28;
+    acc >>= 28;
+  }
+  r.l[N - 1] = (uint32_t)acc;
+  return r;
+}
+
 // a^2: the cross products a_i a_j (i < j) once, doubled, plus the squares; the
 // reduction terms in a second accumulator as in f28_mul
 MZK_HD f28 f28_sqr(const f28& a) {
@@ -223,11 +267,10 @@ MZK_HD f28 f28_norm(const f28& a) {
 // compared digit by digit.
 template <int KMAX>
 MZK_HD bool f28_is_zero_mod_p(const f28& a) {
+  // the low digit of k p is k p_0 mod 2^28 (p_0 odd): k = lo p_0^-1 mod 2^28
   const uint32_t lo = a.l[0] & f28c::M28;
-  bool hit = false;
-#pragma unroll
-  for (int k = 1; k < KMAX; k++) hit |= lo == f28c::KP_DIGIT(k, 0);
-  if (!hit) return false;
+  const uint32_t kk = (lo * f28c::P0INV) & f28c::M28;
+  if (kk - 1u >= (uint32_t)(KMAX - 1)) return false;
   const f28 n = f28_norm(a);
   for (int k = 1; k < KMAX; k++) {
     uint32_t diff = 0;

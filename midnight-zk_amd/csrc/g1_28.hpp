@@ -4,8 +4,8 @@
 // Same formulas as g1.hpp (madd-2008-s with the exceptional cases handled exactly,
 // mdbl-2008-s-1 for the doubling case), restated on f28 with the value bounds the
 // form needs.  Invariants of an accumulator entering or leaving g1_28_madd:
-//   X < 10p, Y < 6p (normalised), ZZ, ZZZ < 2p (product outputs), or the identity
-//   (ZZ literally zero).
+//   X < 10p, Y < 6p (normalised; the formulas below leave Y < 2p), ZZ, ZZZ < 2p
+//   (product outputs), or the identity (ZZ literally zero).
 // Affine inputs come straight from HBM in the blst layout; from_std leaves them
 // below 2^389 < 256 p.
 #pragma once
@@ -59,7 +59,8 @@ MZK_HD g1x28 g1x28_dbl_affine(const f28& x, const f28& y) {
   // X3 = M^2 + 8p - 2S: lower limbs of 8p raised by 2 2^28 cover 2S's (< 2^29)
   r.X = f28_norm(f28_sub<8, 2>(f28_sqr(M), S2));   // < 10p
   const f28 T = f28_sub<16, 1>(S, r.X);            // S + 16p - X3 < 18p
-  r.Y = f28_norm(f28_sub<4, 1>(f28_mul(M, T), f28_mul(W, y)));   // < 6p
+  // Y3 = M T + W (8p - y): 108 p^2 + 16 p^2, one reduction -> < 2p
+  r.Y = f28_mul_sum(M, T, W, f28_sub<8, 1>(f28_zero(), y));
   r.ZZ = V;
   r.ZZZ = W;
   return r;
@@ -89,9 +90,9 @@ MZK_HD g1x28 g1x28_madd(const g1x28& p, const f28& x, const f28& y, bool neg) {
   // X3 = R^2 + 8p - PPP - 2Q: lower limbs of 8p raised by 3 2^28 cover PPP + 2Q
   const f28 Q2 = f28_add(Q, Q);
   r.X = f28_norm(f28_sub<8, 3>(f28_sqr(R), f28_add(PPP, Q2)));   // < 10p
-  // Y3 = R (Q + 16p - X3) - Y PPP + 4p
+  // Y3 = R (Q + 16p - X3) + Y (4p - PPP): 288 p^2 + 24 p^2, one reduction -> < 2p
   const f28 T = f28_sub<16, 1>(Q, r.X);            // < 18p
-  r.Y = f28_norm(f28_sub<4, 1>(f28_mul(R, T), f28_mul(p.Y, PPP)));   // < 6p
+  r.Y = f28_mul_sum(R, T, p.Y, f28_sub<4, 1>(f28_zero(), PPP));
   r.ZZ = f28_mul(p.ZZ, PP);
   r.ZZZ = f28_mul(p.ZZZ, PPP);
   return r;
@@ -108,7 +109,8 @@ MZK_HD g1x28 g1x28_dbl(const g1x28& p) {
   const f28 M = f28_add(f28_add(x2, x2), x2);      // < 6p, limbs < 3 2^28
   r.X = f28_norm(f28_sub<8, 2>(f28_sqr(M), f28_add(S, S)));      // < 10p
   const f28 T = f28_sub<16, 1>(S, r.X);            // < 18p
-  r.Y = f28_norm(f28_sub<4, 1>(f28_mul(M, T), f28_mul(W, p.Y)));   // < 6p
+  // Y3 = M T + W (8p - Y): 108 p^2 + 16 p^2, one reduction -> < 2p
+  r.Y = f28_mul_sum(M, T, W, f28_sub<8, 1>(f28_zero(), p.Y));
   r.ZZ = f28_mul(V, p.ZZ);
   r.ZZZ = f28_mul(W, p.ZZZ);
   return r;
@@ -134,7 +136,8 @@ MZK_HD g1x28 g1x28_add(const g1x28& p, const g1x28& q) {
   g1x28 r;
   r.X = f28_norm(f28_sub<8, 3>(f28_sqr(R), f28_add(PPP, f28_add(Q, Q))));   // < 10p
   const f28 T = f28_sub<16, 1>(Q, r.X);            // < 18p
-  r.Y = f28_norm(f28_sub<4, 1>(f28_mul(R, T), f28_mul(S1, PPP)));   // < 6p
+  // Y3 = R T + S1 (4p - PPP): 108 p^2 + 8 p^2, one reduction -> < 2p
+  r.Y = f28_mul_sum(R, T, S1, f28_sub<4, 1>(f28_zero(), PPP));
   r.ZZ = f28_mul(f28_mul(p.ZZ, q.ZZ), PP);
   r.ZZZ = f28_mul(f28_mul(p.ZZZ, q.ZZZ), PPP);
   return r;
