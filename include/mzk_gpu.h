@@ -238,6 +238,24 @@ typedef struct {
   uint64_t t_evals[16][4];     /* 1 / ((zeta omega_ext^i)^n - 1), i < t_count */
 } mzk_plonk_numerator;
 int mzk_plonk_numerator_dev(mzk_ctx *ctx, const mzk_plonk_numerator *args, void *d_out);
+/* The permutation argument's per-set products (proofs/src/plonk/permutation/prover.rs:
+ * 124-164), numerator and denominator in one pass, for many sets at once:
+ *   den[i] = prod_j (values_j[i] + beta sigma_j[i] + gamma),
+ *   num[i] = prod_j (values_j[i] + delta_beta_j omega^i + gamma),
+ * with delta_beta_j = beta delta^(set chunk + j) and omega_powers[i] = omega^i, i < n.
+ * The reference then inverts den (batch_invert) and multiplies num into it; z is the
+ * running product of num / den. */
+#define MZK_PERM_MAX_CHUNK 4
+typedef struct {
+  const void *values[MZK_PERM_MAX_CHUNK];   /* the set's columns (n Lagrange values each) */
+  const void *sigma[MZK_PERM_MAX_CHUNK];    /* their permutation polynomials' values */
+  uint64_t delta_beta[MZK_PERM_MAX_CHUNK][4];
+  uint64_t beta[4], gamma[4];
+  uint32_t ncols;                           /* 1 .. MZK_PERM_MAX_CHUNK */
+  void *num, *den;                          /* outputs, n each */
+} mzk_perm_set;
+int mzk_plonk_perm_products_dev(mzk_ctx *ctx, const void *omega_powers, size_t n, const mzk_perm_set *sets,
+                                size_t nsets);
 /* sum_j s_j p_j (zero-extended to the longest): poly_inner_product (kzg/mod.rs:125-147)
  * in one launch for up to 64 polynomials; out has max_j len_j elements. */
 int mzk_fr_lincomb_dev(mzk_ctx *ctx, const void *const *d_polys, const size_t *lens, const uint64_t *scalars,

@@ -353,6 +353,35 @@ static Pows make_pows(const host::Fr& x) {
   return P;
 }
 
+// ---------------------------------------------------------------- permutation products
+struct PermArgs {
+  const uint32_t* val[MZK_PERM_MAX_CHUNK];
+  const uint32_t* sig[MZK_PERM_MAX_CHUNK];
+  FrArg delta_beta[MZK_PERM_MAX_CHUNK];
+  FrArg beta, gamma;
+  const uint32_t* omega;
+  uint32_t* num;
+  uint32_t* den;
+  uint64_t n;
+  uint32_t ncols;
+};
+
+// One element per thread: both products of the set at row i (permutation/prover.rs:129-160).
+__global__ void __launch_bounds__(256) k_perm_products(PermArgs A) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  const fr_t beta = arg_fr(A.beta), gamma = arg_fr(A.gamma);
+  const fr_t w = load<FrP>(A.omega + 8 * i);
+  fr_t nu = fr_t::one(), de = fr_t::one();
+  for (uint32_t j = 0; j < A.ncols; j++) {
+    const fr_t v = add(load<FrP>(A.val[j] + 8 * i), gamma);
+    de = mul(de, add(v, mul(beta, load<FrP>(A.sig[j] + 8 * i))));
+    nu = mul(nu, add(v, mul(arg_fr(A.delta_beta[j]), w)));
+  }
+  store<FrP>(A.den + 8 * i, de);
+  store<FrP>(A.num + 8 * i, nu);
+}
+
 // ---------------------------------------------------------------- quotient numerator
 struct NumArgs {
   uint64_t N;
@@ -634,6 +663,42 @@ int mzk_fr_kate_div(mzk_ctx* c, const void* a, size_t n, const uint64_t b[4], vo
   MZK_LAUNCH(c, "k_kate_carries", k_kate_carries, dim3(1), dim3(256), 0, loc, C, to_arg(gL), carry);
   MZK_LAUNCH(c, "k_kate_final", k_kate_final, dim3(grid_for(C, 64)), dim3(64), 0, (const uint32_t*)a, m, KL,
              to_arg(b), carry, (uint32_t*)q);
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
+}
+
+int mzk_plonk_perm_products_dev(mzk_ctx* c, const void* omega_powers, size_t n, const mzk_perm_set* sets,
+                                size_t nsets) {
+  MZK_REQUIRE(c != nullptr && (sets != nullptr || nsets == 0), "mzk_plonk_perm_products_dev: null argument");
+  if (n == 0 || nsets == 0) return MZK_OK;
+  MZK_REQUIRE(omega_powers != nullptr, "mzk_plonk_perm_products_dev: null omega powers");
+  for (size_t s = 0; s < nsets; s++) {
+    const mzk_perm_set& e = sets[s];
+    MZK_REQUIRE(e.ncols >= 1 && e.ncols <= MZK_PERM_MAX_CHUNK && e.num && e.den,
+                "mzk_plonk_perm_products_dev: set %zu: %u columns / null output", s, e.ncols);
+    for (uint32_t j = 0; j < e.ncols; j++)
+      MZK_REQUIRE(e.values[j] && e.sigma[j], "mzk_plonk_perm_products_dev: set %zu column %u missing", s, j);
+  }
+  MZK_TRY(vec_ready(c));
+  std::lock_guard<std::mutex> g(c->mu);
+  for (size_t s = 0; s < nsets; s++) {
+    const mzk_perm_set& e = sets[s];
+    PermArgs A;
+    memset(&A, 0, sizeof A);
+    for (uint32_t j = 0; j < e.ncols; j++) {
+      A.val[j] = (const uint32_t*)e.values[j];
+      A.sig[j] = (const uint32_t*)e.sigma[j];
+      A.delta_beta[j] = to_arg(e.delta_beta[j]);
+    }
+    A.beta = to_arg(e.beta);
+    A.gamma = to_arg(e.gamma);
+    A.omega = (const uint32_t*)omega_powers;
+    A.num = (uint32_t*)e.num;
+    A.den = (uint32_t*)e.den;
+    A.n = n;
+    A.ncols = e.ncols;
+    MZK_LAUNCH(c, "k_perm_products", k_perm_products, dim3(grid_for(n, 256)), dim3(256), 0, A);
+  }
   MZK_HIP(hipGetLastError());
   return MZK_OK;
 }

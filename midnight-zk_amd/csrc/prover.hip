@@ -139,6 +139,30 @@ using namespace mzk::prover;
 
 namespace {
 
+// z[0] = 1, the blinding rows from the draws, and z[u] (the set's last usable value,
+// before any correction) copied to *last (permutation/prover.rs:186-201).  Elements
+// move as two 16-byte words.
+struct ElemArg {
+  uint32_t v[8];
+};
+__device__ __forceinline__ void copy_elem(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src) {
+  const uint4* s4 = reinterpret_cast<const uint4*>(src);
+  uint4* d4 = reinterpret_cast<uint4*>(dst);
+  d4[0] = s4[0];
+  d4[1] = s4[1];
+}
+__global__ void k_perm_z_finish(uint32_t* __restrict__ z, const uint32_t* __restrict__ blind, uint32_t bf,
+                                uint64_t n, uint64_t u, ElemArg one, uint32_t* __restrict__ last) {
+  const uint32_t t = threadIdx.x;
+  if (t == 0) {
+    copy_elem(last, z + 8 * u);
+    uint4* d4 = reinterpret_cast<uint4*>(z);
+    d4[0] = make_uint4(one.v[0], one.v[1], one.v[2], one.v[3]);
+    d4[1] = make_uint4(one.v[4], one.v[5], one.v[6], one.v[7]);
+  }
+  if (t < bf) copy_elem(z + 8 * (n - bf + t), blind + 8 * t);
+}
+
 struct ProofBufs {
   uint32_t* adv[NADV];
   uint32_t* polys[NADV + NSETS];   // advice then z, coefficient form
@@ -157,8 +181,6 @@ struct ProofBufs {
   uint32_t* f;
   uint32_t* fin;
   uint32_t* pi;
-  uint32_t* mv;
-  uint32_t* t;
   uint32_t* draws;
 };
 
@@ -279,12 +301,14 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
     ph[i] = now_ms() - t0;
   };
 
-  // ---- device arena: per proof 69 n scalars plus the draws
-  const size_t per = (size_t)(NADV + (NADV + NSETS) + NSETS + 4 * (NADV + NSETS) + 4 + 4 + QDEG + 12) * n + 64;
+  // ---- device arena: per proof 64 n scalars plus the draws
+  const size_t per = (size_t)(NADV + (NADV + NSETS) + NSETS + 4 * (NADV + NSETS) + 4 + 4 + QDEG + 10) * n + 64;
   {
     std::lock_guard<std::mutex> g(c->mu);
     MZK_HIP(hipSetDevice(c->device));
-    PV_TRY(c->scratch[SCR_PROVER].reserve(per * 32 * np + 256));
+    // plus, for the group: numerators and denominators of every (proof, set), and the
+    // sets' last values
+    PV_TRY(c->scratch[SCR_PROVER].reserve(per * 32 * np + (2 * n + 1) * 32 * np * NSETS + 256));
   }
   uint32_t* arena = (uint32_t*)c->scratch[SCR_PROVER].ptr;
   std::vector<ProofBufs> B(np);
@@ -313,8 +337,6 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
     b.f = take(n);
     b.fin = take(n);
     b.pi = take(n);
-    b.mv = take(n);
-    b.t = take(n);
     b.draws = take(64);
   }
   hipStream_t st = c->stream;
@@ -354,38 +376,62 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
     gamma[p] = T[p].squeeze();
   }
 
-  // ---- permutation grand products (permutation/prover.rs:81-255)
-  for (size_t p = 0; p < np; p++) {
-    ProofBufs& b = B[p];
-    Fr corr = one;
-    for (int s = 0; s < NSETS; s++) {
-      const int j0 = s * CHUNK, j1 = std::min(NADV, j0 + CHUNK);
-      for (int j = j0; j < j1; j++) {   // prod (beta sigma_j + adv_j + gamma)
-        uint32_t* dst = j == j0 ? b.mv : b.t;
-        PV_TRY(vscal(c, 0, key->sigma_values[j], beta[p], dst, n));
-        PV_TRY(mzk_fr_vec_op(c, 0, dst, b.adv[j], dst, n));
-        PV_TRY(vscal(c, 1, dst, gamma[p], dst, n));
-        if (j != j0) PV_TRY(mzk_fr_vec_op(c, 2, b.mv, b.t, b.mv, n));
+  // ---- permutation grand products (permutation/prover.rs:81-255): the products of every
+  // (proof, set) in one pass, one batch inversion and one quotient for all of them, the
+  // running products, then one read-back of the sets' last values for the corrections
+  {
+    const size_t J = np * NSETS;
+    uint32_t* num_all = arena + 8 * per * np;
+    uint32_t* den_all = num_all + 8 * J * n;
+    uint32_t* lasts = den_all + 8 * J * n;
+    std::vector<mzk_perm_set> sets(J);
+    for (size_t p = 0; p < np; p++)
+      for (int s = 0; s < NSETS; s++) {
+        mzk_perm_set& e = sets[p * NSETS + s];
+        memset(&e, 0, sizeof e);
+        const int j0 = s * CHUNK, j1 = std::min(NADV, j0 + CHUNK);
+        e.ncols = (uint32_t)(j1 - j0);
+        for (int j = j0; j < j1; j++) {
+          e.values[j - j0] = B[p].adv[j];
+          e.sigma[j - j0] = key->sigma_values[j];
+          const Fr bd = host::fr_mul(beta[p], fr_pow_u64(delta, (uint64_t)j));
+          memcpy(e.delta_beta[j - j0], bd.l, 32);
+        }
+        memcpy(e.beta, beta[p].l, 32);
+        memcpy(e.gamma, gamma[p].l, 32);
+        e.num = num_all + 8 * (p * NSETS + s) * n;
+        e.den = den_all + 8 * (p * NSETS + s) * n;
       }
-      PV_TRY(mzk_fr_batch_invert(c, b.mv, n));
-      for (int j = j0; j < j1; j++) {   // times prod (beta delta^j omega^i + adv_j + gamma)
-        const Fr bd = host::fr_mul(beta[p], fr_pow_u64(delta, (uint64_t)j));
-        PV_TRY(vscal(c, 0, key->omega_powers, bd, b.t, n));
-        PV_TRY(mzk_fr_vec_op(c, 0, b.t, b.adv[j], b.t, n));
-        PV_TRY(vscal(c, 1, b.t, gamma[p], b.t, n));
-        PV_TRY(mzk_fr_vec_op(c, 2, b.mv, b.t, b.mv, n));
+    PV_TRY(mzk_plonk_perm_products_dev(c, key->omega_powers, n, sets.data(), J));
+    PV_TRY(mzk_fr_batch_invert(c, den_all, J * n));
+    PV_TRY(mzk_fr_vec_op(c, 2, num_all, den_all, num_all, J * n));
+    for (size_t p = 0; p < np; p++)
+      for (int s = 0; s < NSETS; s++) {
+        const size_t js = p * NSETS + s;
+        uint32_t* z = B[p].z[s];
+        // z_0 = 1, z_i = z_(i-1) mv_(i-1); blinding rows from the draws
+        PV_TRY(mzk_fr_prefix_product(c, num_all + 8 * js * n, z + 8, n - 1));
+        const uint32_t* blind = B[p].draws + 8 * (size_t)(NADV * (BF + 1) + s * BF);
+        std::lock_guard<std::mutex> g(c->mu);
+        ElemArg one_w;
+        memcpy(one_w.v, one.l, 32);
+        hipLaunchKernelGGL(k_perm_z_finish, dim3(1), dim3(64), 0, st, z, blind, (uint32_t)BF, (uint64_t)n,
+                           (uint64_t)u, one_w, lasts + 8 * js);
+        MZK_HIP(hipGetLastError());
       }
-      // z_0 = 1, z_i = z_(i-1) mv_(i-1); blinding rows; set s > 0 scaled by the earlier sets' last values
-      uint32_t* z = b.z[s];
-      MZK_HIP(hipMemcpyAsync(z, one.l, 32, hipMemcpyHostToDevice, st));
-      PV_TRY(mzk_fr_prefix_product(c, b.mv, z + 8, n - 1));
-      MZK_HIP(hipMemcpyAsync(z + 8 * (n - BF), b.draws + 8 * (size_t)(NADV * (BF + 1) + s * BF), BF * 32,
-                             hipMemcpyDeviceToDevice, st));
-      Fr last;
-      MZK_HIP(hipMemcpyAsync(last.l, z + 8 * u, 32, hipMemcpyDeviceToHost, st));
+    std::vector<Fr> last(J);
+    {
+      std::lock_guard<std::mutex> g(c->mu);
+      MZK_HIP(hipMemcpyAsync(last.data(), lasts, J * 32, hipMemcpyDeviceToHost, st));
       MZK_HIP(hipStreamSynchronize(st));
-      if (s > 0) PV_TRY(vscal(c, 0, z, corr, z, n));
-      corr = host::fr_mul(corr, last);
+    }
+    // set s > 0 continues from the earlier sets' last values: scale by their product
+    for (size_t p = 0; p < np; p++) {
+      Fr corr = one;
+      for (int s = 0; s < NSETS; s++) {
+        if (s > 0) PV_TRY(vscal(c, 0, B[p].z[s], corr, B[p].z[s], n));
+        corr = host::fr_mul(corr, last[p * NSETS + s]);
+      }
     }
   }
   {

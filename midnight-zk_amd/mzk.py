@@ -71,6 +71,17 @@ class PlonkNumerator(ctypes.Structure):
                 ("delta_beta", (ctypes.c_uint64 * 4) * 8), ("t_evals", (ctypes.c_uint64 * 4) * 16)]
 
 
+PERM_MAX_CHUNK = 4
+
+
+class PermSet(ctypes.Structure):
+    """mzk_perm_set: one permutation set's columns and challenges, and its outputs."""
+    _fields_ = [("values", ctypes.c_void_p * PERM_MAX_CHUNK), ("sigma", ctypes.c_void_p * PERM_MAX_CHUNK),
+                ("delta_beta", (ctypes.c_uint64 * 4) * PERM_MAX_CHUNK), ("beta", ctypes.c_uint64 * 4),
+                ("gamma", ctypes.c_uint64 * 4), ("ncols", ctypes.c_uint32), ("num", ctypes.c_void_p),
+                ("den", ctypes.c_void_p)]
+
+
 class PlonkKey(ctypes.Structure):
     """mzk_plonk_key: the native prover's view of a proving key (device pointers)."""
     _fields_ = [("k", ctypes.c_uint32), ("g", ctypes.c_void_p), ("g_lagrange", ctypes.c_void_p),
@@ -136,6 +147,7 @@ SIGNATURES = {
     "mzk_srs_from_scalars_dev": ([_P, _P, _S, ctypes.POINTER(_P)], _I),
     "mzk_srs_to_lagrange": ([_P, _P, _U32, ctypes.POINTER(_P)], _I),
     "mzk_srs_suffix_sum": ([_P, _P, ctypes.POINTER(_P)], _I),
+    "mzk_plonk_perm_products_dev": ([_P, _P, _S, _P, _S], _I),
     "mzk_tune_msm": ([_P, _P, _U32, _U32, _U32], _I),
     "mzk_tune_ntt": ([_P, _U32, _U32, _U32], _I),
     "mzk_tune_export": ([_P, _S, ctypes.POINTER(_S)], _I),

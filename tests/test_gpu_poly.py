@@ -151,3 +151,48 @@ def test_eval_batch_mixed_sizes_and_points(be):
     want = [plonk.from_mont_limbs(C.poly_eval(p, plonk.mont_limbs(x))) if p.shape[0] else 0
             for p, x in zip(polys, xs)]
     assert got == want
+
+
+@pytest.mark.parametrize("n,ncols", [(1, 1), (100, 3), (4097, 2), ((1 << 18) + 3, 4)])
+def test_perm_products(be, n, ncols):
+    """Both products of permutation/prover.rs:129-160 for two sets in one call, against
+    the same products composed from the C restatement's vector operations."""
+    import ctypes
+    L = mzk.load_library()
+    R = plonk.R_MOD
+    w = C.fr_powers(plonk.mont_limbs(pow(7, (R - 1) >> 20, R)), n)
+    beta, gamma = 0xBE7A % R, 0x6A33A % R
+    sets, want, outs, keep = [], [], [], []
+    for s in range(2):
+        vals = [C.fr_from_seed(40 + 10 * s + j, n, n) for j in range(ncols)]
+        sig = [C.fr_from_seed(41 + 10 * s + j, n, n) for j in range(ncols)]
+        dbs = [beta * pow(O.FR_DELTA, 3 * s + j, R) % R for j in range(ncols)]
+        ones = np.tile(plonk.mont_limbs(1), (n, 1))
+        den, num = ones.copy(), ones.copy()
+        g = np.tile(plonk.mont_limbs(gamma), (n, 1))
+        for j in range(ncols):
+            bs = C.fr_vec_scalar(0, sig[j], plonk.mont_limbs(beta))
+            den = C.fr_vec_op(2, den, C.fr_vec_op(0, C.fr_vec_op(0, vals[j], g), bs))
+            dw = C.fr_vec_scalar(0, w, plonk.mont_limbs(dbs[j]))
+            num = C.fr_vec_op(2, num, C.fr_vec_op(0, C.fr_vec_op(0, vals[j], g), dw))
+        want.append((num, den))
+        e = mzk.PermSet()
+        dv = [to_dev(v) for v in vals] + [to_dev(v) for v in sig]
+        keep += dv
+        for j in range(ncols):
+            e.values[j] = dv[j].data_ptr()
+            e.sigma[j] = dv[ncols + j].data_ptr()
+            e.delta_beta[j][:] = [int(x) for x in plonk.mont_limbs(dbs[j])]
+        e.beta[:] = [int(x) for x in plonk.mont_limbs(beta)]
+        e.gamma[:] = [int(x) for x in plonk.mont_limbs(gamma)]
+        e.ncols = ncols
+        o_num, o_den = to_dev(np.zeros((n, 4), dtype=np.uint64)), to_dev(np.zeros((n, 4), dtype=np.uint64))
+        outs.append((o_num, o_den))
+        e.num, e.den = o_num.data_ptr(), o_den.data_ptr()
+        sets.append(e)
+    arr = (mzk.PermSet * 2)(*sets)
+    dw_ = to_dev(w)
+    be.sync()
+    assert L.mzk_plonk_perm_products_dev(be.ctx.h, dw_.data_ptr(), n, ctypes.cast(arr, ctypes.c_void_p), 2) == 0
+    for (o_num, o_den), (num, den) in zip(outs, want):
+        assert np.array_equal(host(be, o_num), num) and np.array_equal(host(be, o_den), den)
