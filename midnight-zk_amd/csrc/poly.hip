@@ -251,13 +251,17 @@ __global__ void __launch_bounds__(256) k_eval_batch(const EvalJob* __restrict__ 
 
 // ---------------------------------------------------------------- Kate division
 // q_i = a_(i+1) + b q_(i+1), i < m = n - 1, with q_m = 0 (kate_division).  Chunk c holds
-// q indices [c L, c L + L): q_lo = loc_c + b^L q_hi, composed from the top chunk down.
-// L balances the chunk walks (L steps) against the carry pass (m / (256 L) steps).
-static uint32_t kate_chunk(size_t m) {
-  uint32_t L = 4;
-  while (L < 64 && (size_t)L * L * 256 < m) L <<= 1;
-  return L;
-}
+// q indices [c L, c L + L): its bottom value is f_c(x) = loc_c + g x for the value x at
+// the bottom of the chunk above (g = b^L).  Four passes: loc_c per chunk; per workgroup
+// of 256 chunks the suffix compositions f_c o ... o f_(last of the group) (uniform
+// multiplier g, so a Hillis-Steele scan of the loc values with g^(2^k)); one workgroup
+// carries the groups' totals down; each chunk recomputes its q values from its carry-in.
+static constexpr uint32_t KATE_L = 8;
+static constexpr uint32_t KATE_GROUP = 256;
+
+struct KatePows {
+  FrArg g2k[8];   // g^(2^k), g = b^L
+};
 
 // Phase 1: loc_c (the chunk's bottom value with a zero carry-in).
 __global__ void k_kate_local(const uint32_t* __restrict__ a, size_t m, uint32_t KL, FrArg b,
@@ -321,18 +325,53 @@ __global__ void __launch_bounds__(256) k_kate_carries(const uint32_t* __restrict
   }
 }
 
-// Phase 3: recompute each chunk with its true carry-in; write q.
-__global__ void k_kate_final(const uint32_t* __restrict__ a, size_t m, uint32_t KL, FrArg b,
-                             const uint32_t* __restrict__ carry, uint32_t* __restrict__ q) {
+// Group suffix compositions: suf[c] = sum_(j >= c, j in c's group) g^(j - c) loc_j (chunks
+// past C count as loc = 0); tot[group] = suf of the group's first chunk.
+__global__ void __launch_bounds__(KATE_GROUP) k_kate_group(const uint32_t* __restrict__ loc, size_t C, KatePows P,
+                                                           uint32_t* __restrict__ suf, uint32_t* __restrict__ tot) {
+  __shared__ uint32_t la[8 * KATE_GROUP];
+  const uint32_t t = threadIdx.x;
+  const size_t c = (size_t)blockIdx.x * KATE_GROUP + t;
+  fr_t A = c < C ? load<FrP>(loc + 8 * c) : fr_t::zero();
+  store<FrP>(la + 8 * t, A);
+  __syncthreads();
+#pragma unroll 1
+  for (int k = 0; k < 8; k++) {   // A_t <- A_t + g^(2^k) A_(t + 2^k)
+    const uint32_t d = 1u << k;
+    const fr_t nb = t + d < KATE_GROUP ? load<FrP>(la + 8 * (t + d)) : fr_t::zero();
+    __syncthreads();
+    A = add(A, mul(arg_fr(P.g2k[k]), nb));
+    store<FrP>(la + 8 * t, A);
+    __syncthreads();
+  }
+  store<FrP>(suf + 8 * c, A);   // whole groups: chunks past C hold 0, as the final pass reads
+  if (t == 0) store<FrP>(tot + 8 * (size_t)blockIdx.x, A);
+}
+
+// Each chunk with its true carry-in (suf of the next chunk carried by the group's
+// carry-in gcarry, or gcarry itself for a group's last chunk); writes q.
+__global__ void k_kate_final(const uint32_t* __restrict__ a, size_t m, FrArg b, KatePows P,
+                             const uint32_t* __restrict__ suf, const uint32_t* __restrict__ gcarry,
+                             uint32_t* __restrict__ q) {
   const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t lo = c * KL;
+  const size_t lo = c * KATE_L;
   if (lo >= m) return;
-  const size_t hi = lo + KL < m ? lo + KL : m;
+  const size_t hi = lo + KATE_L < m ? lo + KATE_L : m;
+  const uint32_t i = (uint32_t)(c % KATE_GROUP);
+  const fr_t gc = load<FrP>(gcarry + 8 * (c / KATE_GROUP));
+  fr_t v = gc;
+  if (i + 1 < KATE_GROUP) {   // suf[c + 1] + g^(255 - i) gcarry (suf exists: the group is allocated whole)
+    const uint32_t e = KATE_GROUP - 1 - i;
+    fr_t ge = fr_t::one();
+#pragma unroll 1
+    for (int k = 0; k < 8; k++)
+      if ((e >> k) & 1u) ge = mul(ge, arg_fr(P.g2k[k]));
+    v = add(load<FrP>(suf + 8 * (c + 1)), mul(ge, gc));
+  }
   const fr_t bb = arg_fr(b);
-  fr_t v = load<FrP>(carry + 8 * c);
-  for (size_t i = hi; i-- > lo;) {
-    v = add(load<FrP>(a + 8 * (i + 1)), mul(bb, v));
-    store<FrP>(q + 8 * i, v);
+  for (size_t j = hi; j-- > lo;) {
+    v = add(load<FrP>(a + 8 * (j + 1)), mul(bb, v));
+    store<FrP>(q + 8 * j, v);
   }
 }
 
@@ -648,21 +687,31 @@ int mzk_fr_kate_div(mzk_ctx* c, const void* a, size_t n, const uint64_t b[4], vo
   if (n == 1) return MZK_OK;   // the quotient has no coefficients
   MZK_REQUIRE(a && q && a != q, "mzk_fr_kate_div: null or aliased data");
   const size_t m = n - 1;
-  const uint32_t KL = kate_chunk(m);
-  const size_t C = (m + KL - 1) / KL;
+  const size_t C = (m + KATE_L - 1) / KATE_L;
+  const size_t G = (C + KATE_GROUP - 1) / KATE_GROUP;
   MZK_TRY(vec_ready(c));
   std::lock_guard<std::mutex> g(c->mu);
-  MZK_TRY(c->scratch[SCR_MSM_D].reserve(2 * C * 32 + 64));
+  // scratch: loc (C), suf (G whole groups), group totals (G), group carries (G)
+  MZK_TRY(c->scratch[SCR_MSM_D].reserve((C + G * KATE_GROUP + 2 * G) * 32 + 64));
   uint32_t* loc = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
-  uint32_t* carry = loc + 8 * C;
+  uint32_t* suf = loc + 8 * C;
+  uint32_t* tot = suf + 8 * G * KATE_GROUP;
+  uint32_t* gcarry = tot + 8 * G;
   host::Fr hb;
   memcpy(hb.l, b, 32);
-  const host::Fr gL = host::fr_pow(hb, KL);
-  MZK_LAUNCH(c, "k_kate_local", k_kate_local, dim3(grid_for(C, 64)), dim3(64), 0, (const uint32_t*)a, m, KL,
+  KatePows P;
+  host::Fr gk = host::fr_pow(hb, KATE_L);
+  for (int k = 0; k < 8; k++) {
+    P.g2k[k] = to_arg(gk);
+    gk = host::fr_mul(gk, gk);
+  }
+  const host::Fr g256 = gk;   // g^(2^8): one group's multiplier
+  MZK_LAUNCH(c, "k_kate_local", k_kate_local, dim3(grid_for(C, 64)), dim3(64), 0, (const uint32_t*)a, m, KATE_L,
              to_arg(b), loc);
-  MZK_LAUNCH(c, "k_kate_carries", k_kate_carries, dim3(1), dim3(256), 0, loc, C, to_arg(gL), carry);
-  MZK_LAUNCH(c, "k_kate_final", k_kate_final, dim3(grid_for(C, 64)), dim3(64), 0, (const uint32_t*)a, m, KL,
-             to_arg(b), carry, (uint32_t*)q);
+  MZK_LAUNCH(c, "k_kate_group", k_kate_group, dim3((unsigned)G), dim3(KATE_GROUP), 0, loc, C, P, suf, tot);
+  MZK_LAUNCH(c, "k_kate_carries", k_kate_carries, dim3(1), dim3(256), 0, tot, G, to_arg(g256), gcarry);
+  MZK_LAUNCH(c, "k_kate_final", k_kate_final, dim3(grid_for(C, 64)), dim3(64), 0, (const uint32_t*)a, m, to_arg(b),
+             P, suf, gcarry, (uint32_t*)q);
   MZK_HIP(hipGetLastError());
   return MZK_OK;
 }
