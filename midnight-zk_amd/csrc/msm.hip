@@ -401,6 +401,58 @@ __global__ void __launch_bounds__(64) k_bucket_seg(const uint32_t* __restrict__ 
   store_x28_raw(R + MZK_X28_WORDS * (size_t)t, run);
 }
 
+// Small bucket sets (per-window sets of B <= 256 buckets, B a power of two): one
+// workgroup reduces 256 / B whole sets.  Thread i of a set merges bucket i's chunk
+// partials into S_i, a suffix scan in LDS gives U_i = sum_(i' >= i) S_i', and a tree sum
+// gives sum_i U_i = sum_i (i + 1) S_i, the set's weighted bucket sum, written in the
+// blst layout for the host's window combination.
+__global__ void __launch_bounds__(256) k_set_reduce(const uint32_t* __restrict__ off, uint32_t NB, uint32_t K,
+                                                    uint32_t B, uint32_t nsets,
+                                                    const uint32_t* __restrict__ partials,
+                                                    uint32_t* __restrict__ out) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t t = threadIdx.x;
+  const uint32_t i = t % B;
+  const uint32_t set = blockIdx.x * (blockDim.x / B) + t / B;
+  const bool live = set < nsets;
+  g1x28 x = g1x28_inf();
+  if (live) {
+    const uint32_t b = set * B + i;
+    const uint32_t lo = off[b], hi = off[b + 1];
+    if (hi > lo) {
+      const uint32_t k0 = lo / K, k1 = (hi - 1) / K;
+      for (uint32_t k = k0; k <= k1; k++) x = g1x28_add(x, load_x28(partials + MZK_X28_WORDS * (size_t)(k + b)));
+    }
+  }
+  uint32_t* mine = lds + MZK_X28_WORDS * t;
+  store_x28_raw(mine, x);
+  __syncthreads();
+  for (uint32_t d = 1; d < B; d <<= 1) {   // suffix sums within the set's B threads
+    g1x28 nb = g1x28_inf();
+    if (i + d < B) nb = load_x28(lds + MZK_X28_WORDS * (t + d));
+    __syncthreads();
+    if (i + d < B) {
+      x = g1x28_add(x, nb);
+      store_x28_raw(mine, x);
+    }
+    __syncthreads();
+  }
+  for (uint32_t d = B / 2; d >= 1; d >>= 1) {   // tree sum of the suffix sums
+    if (i < d) {
+      x = g1x28_add(x, load_x28(lds + MZK_X28_WORDS * (t + d)));
+      store_x28_raw(mine, x);
+    }
+    __syncthreads();
+  }
+  if (live && i == 0) {
+    uint32_t o[48];
+    g1x28_to_std(x, o);
+    uint4* qv = reinterpret_cast<uint4*>(out + 48 * (size_t)set);
+#pragma unroll
+    for (int w = 0; w < 12; w++) qv[w] = make_uint4(o[4 * w], o[4 * w + 1], o[4 * w + 2], o[4 * w + 3]);
+  }
+}
+
 // Block-wide sum of one point per thread (LDS tree); thread 0 returns it.
 __device__ __forceinline__ g1x28 block_sum(g1x28 x, uint32_t* lds) {
   const uint32_t t = threadIdx.x;
@@ -546,6 +598,16 @@ static void job_tail(const uint32_t* h, int sets_per_job, int c, uint32_t nch, u
   host::xyzz_to_jacobian(acc, out);
 }
 
+// Host Horner over one job's per-window set sums: sum_w 2^(c w) T_w.
+static void job_tail_sets(const uint32_t* h, int windows, int c, uint64_t out[18]) {
+  host::Xyzz acc = host::Xyzz::inf();
+  for (int w = windows - 1; w >= 0; w--) {
+    for (int d = 0; d < c; d++) acc = host::xyzz_dbl(acc);
+    acc = host::xyzz_add(acc, to_host(h + 48 * (size_t)w));
+  }
+  host::xyzz_to_jacobian(acc, out);
+}
+
 int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, const uint32_t* d_bases,
                  uint32_t ref_stride, uint64_t* outs) {
   const size_t nj = jobs.size();
@@ -636,8 +698,9 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   uint32_t* partials = (uint32_t*)c->scratch[SCR_MSM_C].ptr;
   uint32_t* segA = partials + (nchunks + NB) * MZK_X28_WORDS;
   uint32_t* segR = segA + nseg * MZK_X28_WORDS;
-  // scratch D: channel partials and channel sums
-  size_t d_words = nchan * parts * MZK_X28_WORDS + nchan * 48 + 64;
+  // scratch D: channel partials and channel sums (small sets: one sum per set)
+  const bool small_sets = p.per_window && p.B <= 256;
+  size_t d_words = nchan * parts * MZK_X28_WORDS + std::max(nchan, sets) * 48 + 64;
   MZK_TRY(c->scratch[SCR_MSM_D].reserve(d_words * 4));
   uint32_t* chan_parts = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
   uint32_t* chan = chan_parts + nchan * parts * MZK_X28_WORDS;
@@ -652,6 +715,34 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
              tmp_ref, refs, off);
   MZK_LAUNCH(c, "k_msm_accum", k_msm_accum, dim3(grid_for(nchunks, 64)), dim3(64), 0, refs, off, NB, K, d_bases,
              partials);
+  if (small_sets) {
+    // per-set weighted sums in LDS, then the window combination on the host
+    uint32_t* tot = chan;
+    const uint32_t spb = 256 / p.B;
+    MZK_LAUNCH(c, "k_set_reduce", k_set_reduce, dim3((unsigned)((sets + spb - 1) / spb)), dim3(256),
+               256 * MZK_X28_WORDS * 4, off, NB, K, p.B, (uint32_t)sets, partials, tot);
+    MZK_HIP(hipGetLastError());
+    std::vector<uint32_t> h(sets * 48);
+    uint32_t M = 0;
+    double h0 = host_ms();
+    MZK_HIP(hipMemcpyAsync(h.data(), tot, sets * 48 * 4, hipMemcpyDeviceToHost, c->stream));
+    MZK_HIP(hipMemcpyAsync(&M, off + NB, 4, hipMemcpyDeviceToHost, c->stream));
+    MZK_HIP(hipStreamSynchronize(c->stream));
+    double h1 = host_ms();
+    c->prof.host("host_wait", h1 - h0);
+    c->last_msm[0] = (uint64_t)p.c;
+    c->last_msm[1] = (uint64_t)p.W;
+    c->last_msm[2] = (uint64_t)sets;
+    c->last_msm[3] = K;
+    c->last_msm[4] = M;
+    c->last_msm[5] = ntot;
+    parallel_jobs(nj, [&](size_t j) {
+      if (jobs[j].n == 0) memset(outs + 18 * j, 0, 18 * 8);
+      else job_tail_sets(h.data() + j * (size_t)spj * 48, spj, p.c, outs + 18 * j);
+    });
+    c->prof.host("host_horner", host_ms() - h1);
+    return MZK_OK;
+  }
   MZK_LAUNCH(c, "k_bucket_seg", k_bucket_seg, dim3(grid_for(nseg, 64)), dim3(64), 0, off, NB, K, L, partials, segA,
              segR);
   MZK_LAUNCH(c, "k_chan_sum", k_chan_sum, dim3((unsigned)(nchan * parts)), dim3(CHAN_BLOCK), CHAN_BLOCK * MZK_X28_WORDS * 4, segA,

@@ -205,6 +205,7 @@ using namespace mzk;
 // Commitments of at least this many terms use the window-shifted bases (one shared
 // bucket set); smaller ones use per-window buckets on the plain bases.
 static constexpr size_t SHARED_MIN_TERMS = 1u << 14;
+static constexpr size_t SHARED_MIN_BATCH_AVG = 1u << 15;
 static constexpr int SHARED_C = 16;
 // Shifted tables cost (255/16 + 1) x 96 B per base; built for SRS of at least this size.
 static constexpr size_t SHIFTED_MIN_SRS = 1u << 14;
@@ -390,7 +391,9 @@ int mzk_msm_srs_batch_dev(mzk_ctx* c, const mzk_srs* s, const mzk_msm_job* jobs,
   }
   std::lock_guard<std::mutex> g(c->mu);
   MZK_HIP(hipSetDevice(c->device));
-  if (s->shifted) {
+  // the shared set costs 2^15 buckets of reduction per job: worth it once jobs average
+  // 2^15 terms; smaller jobs take per-window sets (small sets reduce in one workgroup each)
+  if (s->shifted && total / njobs >= SHARED_MIN_BATCH_AVG) {
     MsmPlan p = msm_plan(total, 0, c->num_cus, s->shifted_c);
     return msm_run_jobs(c, p, v, (const uint32_t*)s->shifted, (uint32_t)s->n, outs);
   }
