@@ -264,6 +264,32 @@ def cpu_prove_sample(pkg, ctx, k: int, cores: int) -> dict:
             "identical_proof_bytes": proof == g["_proof"]}
 
 
+def cpu_ntt_sample(mzk, ctx, C, log_n: int) -> dict:
+    """One forward NTT of 2^log_n seeded coefficients by the oracle's C restatement of
+    best_fft (bit reverse + DIT, threads as set), beside the same transform on the GPU;
+    the two outputs must be identical."""
+    import numpy as np
+    import torch
+    n = 1 << log_n
+    a = torch.zeros((n, 4), dtype=torch.int64, device="cuda")
+    ctx.fr_from_seed_dev(SEED, 3, n, a)
+    out = torch.empty_like(a)
+    job = mzk.NttJob.make(a, out, log_n)
+    ctx.ntt(job)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.ntt(job)
+    ctx.synchronize()
+    gpu_ms = (time.perf_counter() - t0) * 1e3
+    a_h = a.cpu().numpy().view(np.uint64).copy()
+    t1 = time.perf_counter()
+    ref = C.ntt(a_h, log_n)
+    cpu_s = time.perf_counter() - t1
+    same = bool(np.array_equal(ref, out.cpu().numpy().view(np.uint64)))
+    return {"log_n": log_n, "cpu_ms": round(cpu_s * 1e3, 1), "gpu_ms": round(gpu_ms, 3),
+            "cpu_elements_per_s": round(n / cpu_s, 1), "bit_exact": same}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -451,6 +477,10 @@ def main() -> int:
             cpu["prove_sample"] = cpu_prove_sample(pkg, ctx, args.cpu_prove_k, cores)
         except Exception as e:
             cpu["prove_sample"] = {"error": repr(e)}
+        try:
+            cpu["ntt_sample"] = cpu_ntt_sample(mzk, ctx, C, 22)
+        except Exception as e:
+            cpu["ntt_sample"] = {"error": repr(e)}
 
     line = {
         "metric": METRIC,

@@ -100,3 +100,21 @@ def test_imported_ntt_split(ctx, l1, groups):
     ctx.ntt(mzk.NttJob.make(d_in, d_out, log_n))
     ctx.synchronize()
     assert np.array_equal(d_out.cpu().numpy().view(np.uint64), C.ntt(a, log_n))
+
+
+@pytest.mark.parametrize("log_n,l1,groups", [(12, 4, 256), (14, 9, 4096), (16, 6, 1024)])
+def test_imported_split_inverse_and_coset(ctx, log_n, l1, groups):
+    """A tuned split is used by every transform of that size: inverse (lagrange_to_coeff)
+    and coset extension (coeff_to_extended 2^(log_n-2) -> 2^log_n) stay equal to the oracle."""
+    mzk.tune_import(f"ntt {log_n} {l1} {groups}\n")
+    n = 1 << log_n
+    a = C.fr_from_seed(21, 2, n)
+    d_in, d_out = dev(a), torch.empty((n, 4), dtype=torch.int64, device="cuda")
+    ctx.ntt(mzk.NttJob.make(d_in, d_out, log_n, mzk.NTT_INVERSE))
+    ctx.synchronize()
+    assert np.array_equal(d_out.cpu().numpy().view(np.uint64), C.intt(a, log_n))
+    small = C.fr_from_seed(22, 2, n // 4)
+    d_small = dev(small)
+    ctx.ntt(mzk.NttJob.make(d_small, d_out, log_n, mzk.NTT_COSET, n_real=n // 4))
+    ctx.synchronize()
+    assert np.array_equal(d_out.cpu().numpy().view(np.uint64), C.coeff_to_extended(small, log_n - 2, log_n))
