@@ -400,20 +400,18 @@ def prove_steps(be, params: Params, pk: ProvingKey, advice_values, rng_seed: int
     gamma = tr.squeeze()
     perm_blind = [draws(bf) for _ in range(2)]
 
-    # permutation grand products (permutation/prover.rs:81-255)
+    # permutation grand products (permutation/prover.rs:81-255): each set's numerator and
+    # denominator products (:124-164), one inversion of all denominators, their quotients
     w = be.powers(omega_for(k), n)
     sets = [list(range(0, CHUNK_LEN)), list(range(CHUNK_LEN, NUM_ADVICE))]
+    specs = [[(advice[j], pk.sigma_values[j], pow(DELTA, j, R_MOD) * beta % R_MOD) for j in cols] for cols in sets]
+    fused = getattr(be, "perm_products", None)
+    num_den = fused(w, beta, gamma, specs) if fused else perm_products_composed(be, w, beta, gamma, specs)
+    den_inv = be.batch_invert(be.cat([d for _, d in num_den]))
     zs = []
     last_uncorrected = []
-    for s, cols in enumerate(sets):
-        mv = None
-        for j in cols:
-            t = be.add_scalar(be.add(be.scale(pk.sigma_values[j], beta), advice[j]), gamma)
-            mv = t if mv is None else be.mul(mv, t)
-        mv = be.batch_invert(mv)
-        for j in cols:
-            dw = be.scale(w, pow(DELTA, j, R_MOD) * beta % R_MOD)
-            mv = be.mul(mv, be.add_scalar(be.add(dw, advice[j]), gamma))
+    for s, (num, _) in enumerate(num_den):
+        mv = be.mul(num, be.slice(den_inv, s * n, (s + 1) * n))
         # z_0 = 1, z_i = z_(i-1) * mv_(i-1): exclusive prefix products
         z = be.cat([be.const(1, 1), be.prefix_product(be.slice(mv, 0, n - 1))])
         z = be.cat([be.slice(z, 0, n - bf), perm_blind[s]])
@@ -565,6 +563,22 @@ def create_proofs_native(be, params: Params, pk: ProvingKey, advices, rng_seeds,
     if phase_ms is not None:
         phase_ms[:] = list(ph)
     return [bytes(out[i * mzk.PLONK_PROOF_BYTES:(i + 1) * mzk.PLONK_PROOF_BYTES]) for i in range(m)]
+
+
+def perm_products_composed(be, w, beta: int, gamma: int, specs):
+    """Per set: (prod_j (value_j + delta_beta_j w + gamma), prod_j (value_j + beta sigma_j + gamma))
+    from the backend's vector operations (permutation/prover.rs:129-160); specs holds, per
+    set, the (value, sigma, delta_beta) of its columns."""
+    out = []
+    for cols in specs:
+        num = den = None
+        for value, sigma, db in cols:
+            t = be.add_scalar(be.add(be.scale(sigma, beta), value), gamma)
+            den = t if den is None else be.mul(den, t)
+            u = be.add_scalar(be.add(be.scale(w, db), value), gamma)
+            num = u if num is None else be.mul(num, u)
+        out.append((num, den))
+    return out
 
 
 def construct_intermediate_sets(queries):
@@ -856,6 +870,29 @@ class GpuBackend:
         out = a.clone()
         self._scalar(1, a[:1], (-v) % R_MOD, out[:1])
         return out
+
+    def perm_products(self, w, beta: int, gamma: int, specs):
+        """perm_products_composed in one fused pass per set (mzk_plonk_perm_products_dev)."""
+        import ctypes
+        n = w.shape[0]
+        sets, outs = [], []
+        for cols in specs:
+            e = self.mzk.PermSet()
+            for j, (value, sigma, db) in enumerate(cols):
+                e.values[j] = self._p(value)
+                e.sigma[j] = self._p(sigma)
+                e.delta_beta[j][:] = [int(x) for x in mont_limbs(db)]
+            e.beta[:] = [int(x) for x in mont_limbs(beta)]
+            e.gamma[:] = [int(x) for x in mont_limbs(gamma)]
+            e.ncols = len(cols)
+            num, den = self._empty(n), self._empty(n)
+            e.num, e.den = self._p(num), self._p(den)
+            sets.append(e)
+            outs.append((num, den))
+        arr = (self.mzk.PermSet * len(sets))(*sets)
+        self.mzk._check(self._L.mzk_plonk_perm_products_dev(self.ctx.h, self._p(w), n, ctypes.cast(arr, ctypes.c_void_p),
+                                                            len(sets)))
+        return outs
 
     def batch_invert(self, a):
         out = a.clone()
