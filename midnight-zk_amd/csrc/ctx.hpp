@@ -76,6 +76,7 @@ struct mzk_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::mutex mu;
+  std::mutex prove_mu;   // held for a whole mzk_plonk_prove call (its arena spans many locked calls)
   int num_cus = 0;
   size_t lds_per_block = 0;
   mzk::Scratch scratch[mzk::SCR_COUNT];

@@ -282,6 +282,9 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
                     const uint64_t* rng_seeds, uint8_t* proofs, double* phase_ms) {
   MZK_REQUIRE(c && key && (np == 0 || (advice && rng_seeds && proofs)), "mzk_plonk_prove: null argument");
   if (np == 0) return MZK_OK;
+  // the arena and the per-phase buffers stay in use across the library calls below, each
+  // of which takes the context lock on its own: concurrent provers on one context wait here
+  std::lock_guard<std::mutex> whole(c->prove_mu);
   const uint32_t k = key->k;
   MZK_REQUIRE(k >= 4 && k <= 24, "mzk_plonk_prove: k = %u out of range", k);
   const size_t n = (size_t)1 << k, N = 4 * n, u = n - BF - 1;

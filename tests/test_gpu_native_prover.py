@@ -51,3 +51,26 @@ def test_native_rejects_bad_arguments(gpu):
     rc = gpu._L.mzk_plonk_prove(gpu.ctx.h, ctypes.addressof(key), 1, ctypes.cast(cols, ctypes.c_void_p),
                                 ctypes.cast(seeds, ctypes.c_void_p), ctypes.cast(out, ctypes.c_void_p), None)
     assert rc == 1
+
+
+def test_native_concurrent_calls_on_one_context(gpu):
+    """Two threads proving on one context at once: each call holds the context's prover
+    lock for its whole run, so both get the proofs a lone call gives."""
+    import threading
+    k = 8
+    params = plonk.Params(gpu, k, 0xC0C0 + k)
+    circ = plonk.SyntheticCircuit(k, seed=3 + k)
+    pk = plonk.keygen(gpu, params, circ)
+    wits = [circ.witness(gpu, 2000 + i) for i in range(4)]
+    alone = [plonk.create_proofs_native(gpu, params, pk, [w], [9 + i])[0] for i, w in enumerate(wits)]
+    got = [None, None]
+
+    def run(t):
+        got[t] = plonk.create_proofs_native(gpu, params, pk, wits[2 * t:2 * t + 2], [9 + 2 * t, 10 + 2 * t])
+
+    th = [threading.Thread(target=run, args=(t,)) for t in range(2)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    assert got[0] + got[1] == alone
