@@ -27,14 +27,15 @@ struct MsmJob {
 };
 
 // Runs all jobs through one pipeline (one sort, one accumulation, one reduction):
-// outs[18 j ..] receives job j's Jacobian result.
+// outs[18 j ..] receives job j's Jacobian result.  records: d_bases holds 128-byte
+// accumulation records (a resident SRS's shifted tables) instead of blst affine points.
 int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, const uint32_t* d_bases,
-                 uint32_t ref_stride, uint64_t* outs);
+                 uint32_t ref_stride, uint64_t* outs, bool records = false);
 
 // Runs the pipeline on device scalars (n x 8 u32 Montgomery) and device bases.
 // Reference r of scalar i in window w is ref_offset + i (+ w * ref_stride when
 // per_window == 0); out is the Jacobian result.
 int msm_run(mzk_ctx* c, const MsmPlan& p, const uint32_t* d_scalars, size_t n, const uint32_t* d_bases,
-            uint32_t ref_stride, uint32_t ref_offset, uint64_t out[18]);
+            uint32_t ref_stride, uint32_t ref_offset, bool records, uint64_t out[18]);
 
 }  // namespace mzk

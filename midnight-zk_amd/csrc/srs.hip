@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "ctx.hpp"
+#include "fp28.hpp"
 #include "g1.hpp"
 #include "host_ec.hpp"
 #include "msm.hpp"
@@ -171,29 +172,77 @@ __global__ void __launch_bounds__(64) k_shift_window(const uint32_t* __restrict_
   store_xyzz(out + 48 * i, p);
 }
 
+// Affine points (blst layout) -> 128-byte records for the bucket accumulation: x and y
+// in the radix-2^28 form (words 0..13, 14..27; the values from_std gives), word 28 = 1
+// for the identity, words 29..31 zero.  One record is one 128-byte line.
+__global__ void __launch_bounds__(64) k_to_records(const uint32_t* __restrict__ aff, size_t m,
+                                                   uint32_t* __restrict__ rec) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t* a = aff + 24 * i;
+  uint32_t any = 0;
+  for (int k = 0; k < 24; k++) any |= a[k];
+  uint32_t w[MZK_SRS_REC_WORDS];
+#pragma unroll
+  for (int k = 0; k < MZK_SRS_REC_WORDS; k++) w[k] = 0;
+  if (any) {
+    const f28 x = f28_from_std(a), y = f28_from_std(a + 12);
+#pragma unroll
+    for (int k = 0; k < 14; k++) {
+      w[k] = x.l[k];
+      w[14 + k] = y.l[k];
+    }
+  } else {
+    w[28] = 1;
+  }
+  uint4* q = reinterpret_cast<uint4*>(rec + MZK_SRS_REC_WORDS * i);
+#pragma unroll
+  for (int k = 0; k < MZK_SRS_REC_WORDS / 4; k++) q[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+}
+
 // Window-shifted copies tab[w n + i] = 2^(c w) g_i for w < W, so that all windows of a
-// commitment can share one bucket set.  Built window by window (n XYZZ temporaries).
+// commitment can share one bucket set.  Built window by window in the blst layout (n
+// XYZZ temporaries), then stored as accumulation records.
 int srs_build_shifted(mzk_ctx* c, mzk_srs* s, int cbits) {
   const int W = 255 / cbits + 1;
   const size_t n = s->n;
   const size_t bytes = (size_t)W * n * 96;
-  hipError_t e = hipMalloc(&s->shifted, bytes);
+  void* std_tab = nullptr;
+  hipError_t e = hipMalloc(&std_tab, bytes);
+  if (e == hipSuccess) e = hipMalloc(&s->shifted, (size_t)W * n * MZK_SRS_REC_WORDS * 4);
   if (e != hipSuccess) {
+    if (std_tab) (void)hipFree(std_tab);
     s->shifted = nullptr;
-    set_error("srs: hipMalloc(%zu) for shifted bases failed: %s", bytes, hipGetErrorString(e));
+    set_error("srs: hipMalloc for shifted bases failed: %s", hipGetErrorString(e));
     return MZK_ERR_NOMEM;
   }
-  MZK_TRY(c->scratch[SCR_MSM_C].reserve(n * 192));
-  uint32_t* tmp = (uint32_t*)c->scratch[SCR_MSM_C].ptr;
-  uint32_t* tab = (uint32_t*)s->shifted;
-  MZK_HIP(hipMemcpyAsync(tab, s->bases, n * 96, hipMemcpyDeviceToDevice, c->stream));
-  for (int w = 1; w < W; w++) {
-    MZK_LAUNCH(c, "k_shift_window", k_shift_window, dim3(grid_for(n, 64)), dim3(64), 0, tab + 24 * (size_t)(w - 1) * n,
-               n, cbits, tmp);
-    MZK_TRY(batch_normalize(c, tmp, n, tab + 24 * (size_t)w * n));
+  int rc = MZK_OK;
+  do {
+    if ((rc = c->scratch[SCR_MSM_C].reserve(n * 192)) != MZK_OK) break;
+    uint32_t* tmp = (uint32_t*)c->scratch[SCR_MSM_C].ptr;
+    uint32_t* tab = (uint32_t*)std_tab;
+    if ((e = hipMemcpyAsync(tab, s->bases, n * 96, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) break;
+    for (int w = 1; w < W && rc == MZK_OK; w++) {
+      MZK_LAUNCH(c, "k_shift_window", k_shift_window, dim3(grid_for(n, 64)), dim3(64), 0,
+                 tab + 24 * (size_t)(w - 1) * n, n, cbits, tmp);
+      rc = batch_normalize(c, tmp, n, tab + 24 * (size_t)w * n);
+    }
+    if (rc != MZK_OK) break;
+    MZK_LAUNCH(c, "k_to_records", k_to_records, dim3(grid_for((size_t)W * n, 64)), dim3(64), 0, tab, (size_t)W * n,
+               (uint32_t*)s->shifted);
+    if ((e = hipGetLastError()) != hipSuccess) break;
+    e = hipStreamSynchronize(c->stream);
+  } while (0);
+  (void)hipFree(std_tab);
+  if (rc == MZK_OK && e != hipSuccess) {
+    set_error("srs: shifted bases: %s", hipGetErrorString(e));
+    rc = MZK_ERR_HIP;
   }
-  MZK_HIP(hipGetLastError());
-  MZK_HIP(hipStreamSynchronize(c->stream));
+  if (rc != MZK_OK) {
+    (void)hipFree(s->shifted);
+    s->shifted = nullptr;
+    return rc;
+  }
   s->shifted_c = cbits;
   return MZK_OK;
 }
@@ -365,11 +414,11 @@ int mzk_msm_srs_dev(mzk_ctx* c, const mzk_srs* s, size_t offset, const void* d_c
   const MsmTune t = tune_msm_get(n);
   if (s->shifted && (t.path >= 0 ? t.path == 1 : n >= SHARED_MIN_TERMS)) {
     MsmPlan p = msm_plan(n, 0, c->num_cus, s->shifted_c);
-    return msm_run(c, p, (const uint32_t*)d_coeffs, n, (const uint32_t*)s->shifted, (uint32_t)s->n, (uint32_t)offset,
+    return msm_run(c, p, (const uint32_t*)d_coeffs, n, (const uint32_t*)s->shifted, (uint32_t)s->n, (uint32_t)offset, true,
                    out);
   }
   MsmPlan p = msm_plan(n, 1, c->num_cus, 0);
-  return msm_run(c, p, (const uint32_t*)d_coeffs, n, (const uint32_t*)s->bases + 24 * offset, 0, 0, out);
+  return msm_run(c, p, (const uint32_t*)d_coeffs, n, (const uint32_t*)s->bases + 24 * offset, 0, 0, false, out);
 }
 
 int mzk_msm_srs_batch_dev(mzk_ctx* c, const mzk_srs* s, const mzk_msm_job* jobs, size_t njobs, uint64_t* outs) {
@@ -395,7 +444,7 @@ int mzk_msm_srs_batch_dev(mzk_ctx* c, const mzk_srs* s, const mzk_msm_job* jobs,
   // 2^15 terms; smaller jobs take per-window sets (small sets reduce in one workgroup each)
   if (s->shifted && total / njobs >= SHARED_MIN_BATCH_AVG) {
     MsmPlan p = msm_plan(total, 0, c->num_cus, s->shifted_c);
-    return msm_run_jobs(c, p, v, (const uint32_t*)s->shifted, (uint32_t)s->n, outs);
+    return msm_run_jobs(c, p, v, (const uint32_t*)s->shifted, (uint32_t)s->n, outs, true);
   }
   MsmPlan p = msm_plan(total / njobs + 1, 1, c->num_cus, 0);
   MsmPlan pt = msm_plan(total, 1, c->num_cus, p.c);   // K from the batch's total digits

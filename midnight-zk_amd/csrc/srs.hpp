@@ -2,11 +2,14 @@
 #pragma once
 #include "ctx.hpp"
 
+#define MZK_SRS_REC_WORDS 32
+
 struct mzk_srs {
   mzk_ctx* ctx = nullptr;
   size_t n = 0;
   void* bases = nullptr;     // n x 24 u32 affine (blst_p1_affine layout)
-  void* shifted = nullptr;   // optional window-shifted copies 2^(c w) g_i
+  void* shifted = nullptr;   // optional window-shifted copies 2^(c w) g_i, as accumulation
+                             // records of MZK_SRS_REC_WORDS words (x, y radix-2^28, identity flag)
   int shifted_c = 0;
 };
 
