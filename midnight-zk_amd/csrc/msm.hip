@@ -100,6 +100,31 @@ __device__ __forceinline__ uint32_t bucket_key(const SortArgs& a, const JobDesc&
   return set * B + (uint32_t)(d > 0 ? d : -d) - 1u;
 }
 
+// Exclusive prefix sum of one value per thread over a SORT_BLOCK workgroup: wave-level
+// shuffles, then the wave totals (wtot: SORT_BLOCK / 64 words of LDS).  *total gets the
+// sum of all values.  Every thread of the block must call it.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wtot, uint32_t* total) {
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) wtot[wid] = x;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < SORT_BLOCK / 64; w++) {
+    const uint32_t t = wtot[w];
+    pre += w < wid ? t : 0u;
+    tot += t;
+  }
+  __syncthreads();   // wtot may be rewritten by the next call
+  *total = tot;
+  return pre + x - v;
+}
+
 // 1. digits + per-tile histogram of key >> l2 (LDS atomics)
 __global__ void __launch_bounds__(SORT_BLOCK) k_sort_hist(SortArgs a, uint32_t* __restrict__ canon,
                                                           uint32_t* __restrict__ hist) {
@@ -166,21 +191,11 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __r
         if (d != 0) rank[tid * SCAT_WG + wl] = atomicAdd(&cnt[bucket_key(a, j, w, d) >> a.l2], 1u);
       }
       __syncthreads();
-      // exclusive scan of the counts: per-thread ranges, then the SORT_BLOCK partials
+      // exclusive scan of the counts: per-thread ranges, then a block scan of the partials
       uint32_t part = 0;
       for (uint32_t b = b0; b < b1; b++) part += cnt[b];
-      psum[tid] = part;
-      __syncthreads();
-      if (tid == 0) {
-        uint32_t run = 0;
-        for (uint32_t t = 0; t < SORT_BLOCK; t++) {
-          const uint32_t v = psum[t];
-          psum[t] = run;
-          run += v;
-        }
-      }
-      __syncthreads();
-      uint32_t run = psum[tid];
+      uint32_t all;
+      uint32_t run = block_excl_scan(part, psum, &all);
       for (uint32_t b = b0; b < b1; b++) {
         offs[b] = run;
         run += cnt[b];
@@ -234,14 +249,20 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_bins(const uint32_t* __rest
   __syncthreads();
   for (uint32_t e = lo + threadIdx.x; e < hi; e += SORT_BLOCK) atomicAdd(&c2[tmp_key[e]], 1u);
   __syncthreads();
-  if (threadIdx.x == 0) {   // R <= 8192 counters: a serial exclusive scan is short
-    uint32_t run = lo;
-    for (uint32_t d = 0; d < R; d++) {
-      uint32_t v = c2[d];
+  {   // exclusive scan of the R counters from lo: per-thread ranges, then a block scan
+    __shared__ uint32_t wtot[SORT_BLOCK / 64];
+    const uint32_t per = (R + SORT_BLOCK - 1) / SORT_BLOCK;
+    const uint32_t d0 = threadIdx.x * per, d1 = d0 + per < R ? d0 + per : R;
+    uint32_t part = 0;
+    for (uint32_t d = d0; d < d1; d++) part += c2[d];
+    uint32_t all;
+    uint32_t run = lo + block_excl_scan(part, wtot, &all);
+    for (uint32_t d = d0; d < d1; d++) {
+      const uint32_t v = c2[d];
       c2[d] = run;
       run += v;
     }
-    if (bin == 0) off[NB] = hoff[(size_t)a.D1 * a.G];
+    if (bin == 0 && threadIdx.x == 0) off[NB] = hoff[(size_t)a.D1 * a.G];
   }
   __syncthreads();
   for (uint32_t d = threadIdx.x; d < R; d += SORT_BLOCK) {
