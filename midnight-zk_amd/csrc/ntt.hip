@@ -14,10 +14,12 @@
 // the pass-1 load; the 1/N scaling and zeta^-(i mod 3) of the inverse transforms
 // into the pass-2 store.  Results are the unique DFT values, hence identical to the
 // reference's bit-reverse + DIT and pruned-DIF schedules.
+#include <mutex>
 #include <vector>
 
 #include "ctx.hpp"
 #include "field.hpp"
+#include "fr29.hpp"
 #include "host_ec.hpp"
 #include "ntt.hpp"
 #include "tune.hpp"
@@ -27,6 +29,8 @@ namespace mzk {
 static constexpr int NTT_MAX_LOG_L = 11;
 static constexpr int NTT_LDS_ELEMS = 1 << NTT_MAX_LOG_L;
 static constexpr int TWIST_LO_BITS = 11;
+static constexpr int TW29 = 12;   // words per radix-2^29 table record (9 used; 16-byte aligned)
+static constexpr int ELEM29 = 9;  // words per element staged in LDS
 
 struct FrWords {
   uint32_t v[8];
@@ -35,9 +39,9 @@ struct FrWords {
 struct PassArgs {
   const uint32_t* src;
   uint32_t* dst;
-  const uint32_t* tw;        // omega_L^i, i < L/2
-  const uint32_t* twist_lo;  // omega_N^i, i < 2^TWIST_LO_BITS        (pass 1 only)
-  const uint32_t* twist_hi;  // omega_N^(i 2^TWIST_LO_BITS)            (pass 1 only)
+  const uint32_t* tw;        // omega_L^i, i < L/2 (radix-2^29 records of TW29 words)
+  const uint32_t* twist_lo;  // omega_N^i, i < 2^TWIST_LO_BITS        (pass 1 only; records)
+  const uint32_t* twist_hi;  // omega_N^(i 2^TWIST_LO_BITS)            (pass 1 only; records)
   uint64_t in_bstride, in_jstride, out_bstride, out_kstride;
   uint64_t n_real;           // input indices >= n_real read as zero
   uint32_t log_l;            // sub-DFT size L = 2^log_l
@@ -63,7 +67,7 @@ __device__ __forceinline__ uint32_t bitrev(uint32_t j, uint32_t bits) {
 __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block, uint32_t* lds);
 
 __global__ void __launch_bounds__(256) k_ntt_pass(PassArgs a) {
-  extern __shared__ uint32_t lds[];   // C * L elements of 8 words
+  extern __shared__ uint32_t lds[];   // C * L elements of ELEM29 words
   ntt_pass_body(a, blockIdx.x, lds);
 }
 
@@ -80,6 +84,38 @@ __global__ void __launch_bounds__(256) k_ntt_pass_batch(const PassArgs* __restri
   ntt_pass_body(a, blockIdx.x - first[lo], lds);
 }
 
+// Element arithmetic of the passes is the radix-2^29 form (fr29.hpp): each butterfly's
+// product is one v_mad_u64_u32 per limb product with no carry instruction.  Values enter
+// a pass from canonical blst words (below 2 r after reduce_small), grow by at most 2 r
+// per stage (x + t and x - t + 2 r with t below 2 r: a product of a value below 25 r and
+// a table entry below r, or reduce_small of the operand where the twiddle is 1), so they
+// stay below 25 r (< 2^261) through 11 stages; the store multiplies by the twist / scale
+// factors where there are any and leaves in canonical blst words.
+__device__ __forceinline__ fr29 load29(const uint32_t* __restrict__ p) {
+  fr29 r;
+#pragma unroll
+  for (int i = 0; i < 9; i++) r.l[i] = p[i];
+  return r;
+}
+
+__device__ __forceinline__ void store29(uint32_t* __restrict__ p, const fr29& v) {
+#pragma unroll
+  for (int i = 0; i < 9; i++) p[i] = v.l[i];
+}
+
+// a table record (TW29 words, 16-byte aligned): the value below r
+__device__ __forceinline__ fr29 load_rec29(const uint32_t* __restrict__ p) {
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+  const uint4 a = q[0], b = q[1], c = q[2];
+  fr29 r;
+  r.l[0] = a.x; r.l[1] = a.y; r.l[2] = a.z; r.l[3] = a.w;
+  r.l[4] = b.x; r.l[5] = b.y; r.l[6] = b.z; r.l[7] = b.w;
+  r.l[8] = c.x;
+  return r;
+}
+
+__device__ __forceinline__ fr29 arg29(const FrWords& w) { return fr29_reduce_full(fr29_from_std(w.v)); }
+
 __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block, uint32_t* lds) {
   const uint32_t L = 1u << a.log_l;
   const uint32_t CL = a.C * L;
@@ -87,21 +123,30 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block,
   const uint32_t nt = blockDim.x;
 
   // load (bit-reversed within each sub-DFT)
+  fr29 z1, z2;
+  if (a.in_coset) {
+    z1 = arg29(a.zeta);
+    z2 = arg29(a.zeta2);
+  }
   for (uint32_t e = threadIdx.x; e < CL; e += nt) {
     const uint32_t c = e % a.C, j = e / a.C;
     const uint64_t gi = (b0 + c) * a.in_bstride + (uint64_t)j * a.in_jstride;
-    fr_t x;
+    fr29 x;
     if (gi < a.n_real) {
-      x = load<FrP>(a.src + 8 * gi);
+      const uint4* q = reinterpret_cast<const uint4*>(a.src + 8 * gi);
+      const uint4 u0 = q[0], u1 = q[1];
+      const uint32_t w[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+      x = fr29_reduce_small(fr29_from_std(w));
       if (a.in_coset) {
         const uint32_t m = (uint32_t)(gi % 3);
-        if (m == 1) x = mul(x, to_fr(a.zeta));
-        else if (m == 2) x = mul(x, to_fr(a.zeta2));
+        if (m == 1) x = fr29_mul(x, z1);
+        else if (m == 2) x = fr29_mul(x, z2);
       }
     } else {
-      x = fr_t::zero();
+#pragma unroll
+      for (int i = 0; i < 9; i++) x.l[i] = 0;
     }
-    store<FrP>(lds + 8 * (c * L + bitrev(j, a.log_l)), x);
+    store29(lds + 9 * (c * L + bitrev(j, a.log_l)), x);
   }
   __syncthreads();
 
@@ -113,38 +158,63 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block,
       const uint32_t c = u / halfL, v = u % halfL;
       const uint32_t pos = v & (half - 1);
       const uint32_t i = ((v >> s) << (s + 1)) + pos;
-      uint32_t* pa = lds + 8 * (c * L + i);
-      uint32_t* pb = pa + 8 * half;
-      fr_t x = load<FrP>(pa);
-      fr_t y = load<FrP>(pb);
-      if (pos != 0) y = mul(y, load<FrP>(a.tw + 8 * (pos << (a.log_l - 1 - s))));
-      store<FrP>(pa, add(x, y));
-      store<FrP>(pb, sub(x, y));
+      uint32_t* pa = lds + 9 * (c * L + i);
+      uint32_t* pb = pa + 9 * half;
+      const fr29 x = load29(pa);
+      const fr29 y = load29(pb);
+      const fr29 t = pos != 0 ? fr29_mul(y, load_rec29(a.tw + TW29 * (pos << (a.log_l - 1 - s))))
+                              : fr29_reduce_small(y);
+      store29(pa, fr29_add(x, t));
+      store29(pb, fr29_sub_kr<2>(x, t));
     }
     __syncthreads();
   }
 
-  // store (natural order), fused twist / scaling
+  // store (natural order), fused twist / scaling, back to canonical blst words
+  fr29 p0, p1, p2;
+  if (a.out_mode != 0) {
+    p0 = arg29(a.post[0]);
+    p1 = arg29(a.post[1]);
+    p2 = arg29(a.post[2]);
+  }
   for (uint32_t e = threadIdx.x; e < CL; e += nt) {
     const uint32_t c = e % a.C, k = e / a.C;
     const uint64_t b = b0 + c;
-    fr_t x = load<FrP>(lds + 8 * (c * L + k));
+    fr29 x = load29(lds + 9 * (c * L + k));
     if (a.do_twist) {
       const uint64_t ex = b * k;   // < N
       if (ex != 0) {
-        fr_t w = mul(load<FrP>(a.twist_lo + 8 * (ex & ((1u << TWIST_LO_BITS) - 1))),
-                     load<FrP>(a.twist_hi + 8 * (ex >> TWIST_LO_BITS)));
-        x = mul(x, w);
+        const fr29 w = fr29_mul(load_rec29(a.twist_lo + TW29 * (ex & ((1u << TWIST_LO_BITS) - 1))),
+                                load_rec29(a.twist_hi + TW29 * (ex >> TWIST_LO_BITS)));
+        x = fr29_mul(x, w);
       }
     }
     const uint64_t go = b * a.out_bstride + (uint64_t)k * a.out_kstride;
     if (a.out_mode == 1) {
-      x = mul(x, to_fr(a.post[0]));
+      x = fr29_mul(x, p0);
     } else if (a.out_mode == 2) {
-      x = mul(x, to_fr(a.post[go % 3]));
+      const uint32_t m = (uint32_t)(go % 3);
+      x = fr29_mul(x, m == 0 ? p0 : (m == 1 ? p1 : p2));
     }
-    store<FrP>(a.dst + 8 * go, x);
+    uint32_t o[8];
+    fr29_to_std(x, o);
+    uint4* q = reinterpret_cast<uint4*>(a.dst + 8 * go);
+    q[0] = make_uint4(o[0], o[1], o[2], o[3]);
+    q[1] = make_uint4(o[4], o[5], o[6], o[7]);
   }
+}
+
+// std blst values -> table records (TW29 words: the value below r in the radix-2^29
+// form, then padding)
+__global__ void k_to_rec29(const uint32_t* __restrict__ in, uint32_t count, uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const fr29 v = fr29_reduce_full(fr29_from_std(in + 8 * (size_t)i));
+  uint32_t* o = out + TW29 * (size_t)i;
+#pragma unroll
+  for (int k = 0; k < 9; k++) o[k] = v.l[k];
+#pragma unroll
+  for (int k = 9; k < TW29; k++) o[k] = 0;
 }
 
 // out[i] = prod_{j : bit j of i} pows[j]  (pows[j] = g^(2^j)), i.e. g^i.
@@ -193,33 +263,69 @@ NttPlan::~NttPlan() {
 int NttPlan::init(hipStream_t st, uint32_t log_n_, const host::Fr& omega) {
   log_n = log_n_;
   l1 = (log_n + 1) / 2;
-  const NttTune t = tune_ntt_get(log_n);
-  if (t.l1 && t.l1 <= (uint32_t)NTT_MAX_LOG_L && log_n > t.l1 && log_n - t.l1 <= (uint32_t)NTT_MAX_LOG_L) l1 = t.l1;
+  const NttTune tn = tune_ntt_get(log_n);
+  if (tn.l1 && tn.l1 <= (uint32_t)NTT_MAX_LOG_L && log_n > tn.l1 && log_n - tn.l1 <= (uint32_t)NTT_MAX_LOG_L)
+    l1 = tn.l1;
   l2 = log_n - l1;
   const uint32_t N1 = 1u << l1, N2 = 1u << l2;
   const uint32_t lo_cnt = 1u << std::min<uint32_t>(TWIST_LO_BITS, log_n);
   const uint32_t hi_cnt = (uint32_t)std::max<uint64_t>(1, ((uint64_t)1 << log_n) >> TWIST_LO_BITS);
-  const size_t words_needed = 8 * ((size_t)std::max(N1 / 2, 1u) + std::max(N2 / 2, 1u) + lo_cnt + hi_cnt);
-  MZK_HIP(hipMalloc(&mem, words_needed * 4));
+  const uint32_t c1 = std::max(N1 / 2, 1u), c2 = std::max(N2 / 2, 1u);
+  const size_t entries = (size_t)c1 + c2 + lo_cnt + hi_cnt;
+  // powers in blst words into a temporary, then the tables as radix-2^29 records
+  void* tmp = nullptr;
+  MZK_HIP(hipMalloc(&tmp, entries * 32));
+  if (hipMalloc(&mem, entries * TW29 * 4) != hipSuccess) {
+    mem = nullptr;
+    (void)hipFree(tmp);
+    set_error("ntt plan 2^%u: out of device memory for twiddle tables", log_n);
+    return MZK_ERR_NOMEM;
+  }
+  uint32_t* t = (uint32_t*)tmp;
   uint32_t* p = (uint32_t*)mem;
-  tw1 = p;
-  p += 8 * std::max(N1 / 2, 1u);
-  tw2 = p;
-  p += 8 * std::max(N2 / 2, 1u);
-  twist_lo = p;
-  p += 8 * lo_cnt;
-  twist_hi = p;
   // omega_N1 = omega^N2, omega_N2 = omega^N1
   host::Fr w1 = omega, w2 = omega;
   for (uint32_t i = 0; i < l2; i++) w1 = host::fr_mul(w1, w1);
   for (uint32_t i = 0; i < l1; i++) w2 = host::fr_mul(w2, w2);
-  MZK_TRY(pow_table(st, tw1, std::max(N1 / 2, 1u), w1));
-  MZK_TRY(pow_table(st, tw2, std::max(N2 / 2, 1u), w2));
-  MZK_TRY(pow_table(st, twist_lo, lo_cnt, omega));
   host::Fr whi = omega;
   for (int i = 0; i < TWIST_LO_BITS; i++) whi = host::fr_mul(whi, whi);
-  MZK_TRY(pow_table(st, twist_hi, hi_cnt, whi));
+  int rc = pow_table(st, t, c1, w1);
+  if (rc == MZK_OK) rc = pow_table(st, t + 8 * (size_t)c1, c2, w2);
+  if (rc == MZK_OK) rc = pow_table(st, t + 8 * (size_t)(c1 + c2), lo_cnt, omega);
+  if (rc == MZK_OK) rc = pow_table(st, t + 8 * (size_t)(c1 + c2 + lo_cnt), hi_cnt, whi);
+  if (rc == MZK_OK) {
+    hipLaunchKernelGGL(k_to_rec29, dim3(grid_for(entries, 256)), dim3(256), 0, st, t, (uint32_t)entries, p);
+    if (hipGetLastError() != hipSuccess) rc = MZK_ERR_HIP;
+  }
+  if (hipStreamSynchronize(st) != hipSuccess && rc == MZK_OK) rc = MZK_ERR_HIP;
+  (void)hipFree(tmp);
+  if (rc != MZK_OK) {
+    set_error("ntt plan 2^%u: twiddle tables failed", log_n);
+    return rc;
+  }
+  tw1 = p;
+  tw2 = p + TW29 * (size_t)c1;
+  twist_lo = p + TW29 * (size_t)(c1 + c2);
+  twist_hi = p + TW29 * (size_t)(c1 + c2 + lo_cnt);
   return MZK_OK;
+}
+
+// 2^11 staged elements of 9 words take 72 KiB of LDS, above the 64 KiB a kernel gets
+// without asking (gfx950 gives a workgroup up to 160 KiB)
+static int ntt_lds_attr() {
+  static std::once_flag once;
+  static int rc = MZK_OK;
+  std::call_once(once, [] {
+    const int bytes = NTT_LDS_ELEMS * ELEM29 * 4;
+    if (hipFuncSetAttribute((const void*)k_ntt_pass, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
+            hipSuccess ||
+        hipFuncSetAttribute((const void*)k_ntt_pass_batch, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
+            hipSuccess) {
+      set_error("ntt: cannot raise the LDS limit to %d bytes", bytes);
+      rc = MZK_ERR_HIP;
+    }
+  });
+  return rc;
 }
 
 static uint32_t choose_c(uint32_t L, uint64_t nbatch, uint32_t groups) {
@@ -233,7 +339,8 @@ static uint32_t choose_c(uint32_t L, uint64_t nbatch, uint32_t groups) {
 static int launch_pass(mzk_ctx* c, const char* name, PassArgs a, uint64_t nbatch) {
   const uint32_t L = 1u << a.log_l;
   const uint32_t threads = std::min<uint32_t>(256, std::max<uint32_t>(64, a.C * L / 2));
-  const size_t lds = (size_t)a.C * L * 32;
+  const size_t lds = (size_t)a.C * L * ELEM29 * 4;
+  MZK_TRY(ntt_lds_attr());
   MZK_LAUNCH(c, name, k_ntt_pass, dim3((unsigned)(nbatch / a.C)), dim3(threads), lds, a);
   MZK_HIP(hipGetLastError());
   return MZK_OK;
@@ -395,7 +502,8 @@ int ntt_run_batch(mzk_ctx* c, const mzk_ntt_job* jobs, size_t njobs) {
   uint32_t* d_f1 = (uint32_t*)put(f1.data(), f1.size() * 4);
   uint32_t* d_f2 = (uint32_t*)put(f2.data(), f2.size() * 4);
   MZK_HIP(hipMemcpyAsync(d, h.data(), o, hipMemcpyHostToDevice, c->stream));
-  const size_t lds = (size_t)NTT_LDS_ELEMS * 32;
+  const size_t lds = (size_t)NTT_LDS_ELEMS * ELEM29 * 4;
+  MZK_TRY(ntt_lds_attr());
   if (!t1.empty())
     MZK_LAUNCH(c, "k_ntt_pass_batch1", k_ntt_pass_batch, dim3(f1.back()), dim3(256), lds, d_t1, d_f1,
                (uint32_t)t1.size());

@@ -10,6 +10,7 @@ namespace mzk {
 struct NttPlan {
   uint32_t log_n = 0, l1 = 0, l2 = 0;
   void* mem = nullptr;
+  // tables as radix-2^29 records (values below r, 12 words each)
   uint32_t* tw1 = nullptr;       // omega_{N1}^i, i < N1/2   (pass 2)
   uint32_t* tw2 = nullptr;       // omega_{N2}^i, i < N2/2   (pass 1)
   uint32_t* twist_lo = nullptr;  // omega_N^i, i < 2^min(11, log_n)
