@@ -207,6 +207,18 @@ int mzk_fr_eval_batch(mzk_ctx *ctx, const void *const *d_polys, const size_t *ns
 /* Quotient of a(X) - a(b) by (X - b): n - 1 coefficients into d_q (kate_division,
  * utils/arithmetic.rs:106-125).  n >= 1; d_q must not alias d_a. */
 int mzk_fr_kate_div(mzk_ctx *ctx, const void *d_a, size_t n, const uint64_t b[4], void *d_q);
+/* Many independent Kate divisions in one launch of each pass, as the multi-open divides
+ * every point set's polynomial in parallel (kzg/mod.rs:226-236, each by kate_division,
+ * utils/arithmetic.rs:106-125).  Job j: d_q = quotient of d_a (n coefficients) by
+ * (X - b).  Results equal the per-job mzk_fr_kate_div; n = 1 jobs write nothing.  No
+ * output may overlap any job's input or another job's output (MZK_ERR_INVALID). */
+typedef struct mzk_kate_job {
+  const void *d_a;
+  size_t n;
+  uint64_t b[4];
+  void *d_q;
+} mzk_kate_job;
+int mzk_fr_kate_div_batch(mzk_ctx *ctx, const mzk_kate_job *jobs, size_t njobs);
 /* out_i = x^i for i < n (powers, kzg/mod.rs:151-167 and the domain's omega tables). */
 int mzk_fr_powers(mzk_ctx *ctx, const uint64_t x[4], size_t n, void *d_out);
 

@@ -29,6 +29,10 @@
 //  * one shared set (resident SRS with window-shifted copies 2^(cw) g_i built once):
 //    key = |d| - 1, reference = w * N + offset + i, a single set of weight 1.
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -612,22 +616,70 @@ MsmPlan msm_plan(size_t n, int per_window, int num_cus, int c_override) {
   return p;
 }
 
-// Runs fn(j) for j < n on up to 16 host threads (serially for small n).
-template <class F>
-static void parallel_jobs(size_t n, F fn) {
-  const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
-  const size_t nt = std::min<size_t>(std::min<size_t>(16, hw), n / 2);
-  if (nt <= 1) {
-    for (size_t j = 0; j < n; j++) fn(j);
-    return;
+// Host workers for the per-job tails: up to 15 threads started once and kept for the
+// process (a batch's tails took ~0.1 ms more when each call started its own threads).
+// One batch runs at a time; a caller that finds the pool busy runs its tails itself.
+class TailPool {
+ public:
+  static TailPool& get() {
+    static TailPool* p = new TailPool();   // never destroyed: its workers wait until exit
+    return *p;
   }
-  std::vector<std::thread> th;
-  for (size_t t = 0; t < nt; t++)
-    th.emplace_back([&, t]() {
-      for (size_t j = t; j < n; j += nt) fn(j);
-    });
-  for (auto& x : th) x.join();
-}
+  // fn(j) for j < n on the workers and the calling thread
+  void run(size_t n, const std::function<void(size_t)>& fn) {
+    std::unique_lock<std::mutex> busy(run_mu, std::try_to_lock);
+    if (!busy.owns_lock() || workers_.empty() || n < 2) {
+      for (size_t j = 0; j < n; j++) fn(j);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> g(mu);
+      task = &fn;
+      total = n;
+      next.store(0);
+      pending = workers_.size();
+      gen++;
+    }
+    cv.notify_all();
+    drain();
+    std::unique_lock<std::mutex> g(mu);
+    done.wait(g, [&] { return pending == 0; });
+    task = nullptr;
+  }
+
+ private:
+  TailPool() {
+    const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>(15, hw > 1 ? hw - 1 : 0);
+    for (size_t t = 0; t < nt; t++) workers_.emplace_back([this] { loop(); });
+    for (auto& w : workers_) w.detach();
+  }
+  void drain() {
+    for (size_t j; (j = next.fetch_add(1)) < total;) (*task)(j);
+  }
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> g(mu);
+        cv.wait(g, [&] { return gen != seen; });
+        seen = gen;
+      }
+      drain();
+      std::lock_guard<std::mutex> g(mu);
+      if (--pending == 0) done.notify_one();
+    }
+  }
+  std::mutex run_mu, mu;
+  std::condition_variable cv, done;
+  const std::function<void(size_t)>* task = nullptr;
+  size_t total = 0, pending = 0;
+  std::atomic<size_t> next{0};
+  uint64_t gen = 0;
+  std::vector<std::thread> workers_;
+};
+
+static void parallel_jobs(size_t n, const std::function<void(size_t)>& fn) { TailPool::get().run(n, fn); }
 
 // Host Horner over one job's channel positions.  Set s (of sets_per_job) weighs
 // 2^(c s); inside it, sum_t A_t has weight 1 and U_j (bit j of t) 2^(log2 L + j).
@@ -708,7 +760,16 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   const size_t HN = (size_t)sa.D1 * sa.G;   // histogram entries
   const size_t tiles = (HN + SCAN_TILE - 1) / SCAN_TILE;
   MZK_REQUIRE(tiles <= (size_t)SCAN_TILE, "msm: sort histogram too large (%zu)", HN);
-  const uint32_t K = p.K;
+  // Per-window sets: a bucket holds ~n / B digits of its job, merged from its chunk
+  // partials by one dependent chain in the reduction; chunks of at least n / (4 B)
+  // digits keep that chain within ~5 additions (small batches would otherwise cut
+  // their few digits into chunks of 2 or 3).
+  uint32_t K = p.K;
+  if (p.per_window) {
+    size_t nmax = 0;
+    for (auto& j : jobs) nmax = std::max(nmax, j.n);
+    K = (uint32_t)std::min<size_t>(256, std::max<size_t>(K, (nmax + 4 * (size_t)p.B - 1) / (4 * (size_t)p.B)));
+  }
   const size_t nchunks = (M_max + K - 1) / K;
   // reduction geometry: segments of L buckets, T per set; channels A + bits of t.
   // One MSM alone: a thread per bucket keeps chains short.  Many sets: segments of

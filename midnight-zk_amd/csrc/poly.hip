@@ -255,43 +255,74 @@ __global__ void __launch_bounds__(256) k_eval_batch(const EvalJob* __restrict__ 
 // the bottom of the chunk above (g = b^L).  Four passes: loc_c per chunk; per workgroup
 // of 256 chunks the suffix compositions f_c o ... o f_(last of the group) (uniform
 // multiplier g, so a Hillis-Steele scan of the loc values with g^(2^k)); one workgroup
-// carries the groups' totals down; each chunk recomputes its q values from its carry-in.
+// per division carries the groups' totals down; each chunk recomputes its q values from
+// its carry-in.  Up to KATE_MAXJ independent divisions share each pass's launch (the
+// multi-open divides every point set's polynomial at once, kzg/mod.rs:226-236): a
+// workgroup finds its division from the per-division first-workgroup tables.
 static constexpr uint32_t KATE_L = 8;
 static constexpr uint32_t KATE_GROUP = 256;
+static constexpr int KATE_MAXJ = 9;   // the kernel argument block stays below 4 KB
+static constexpr uint32_t KATE_CHUNK_BLOCK = 64;
 
 struct KatePows {
   FrArg g2k[8];   // g^(2^k), g = b^L
 };
 
-// Phase 1: loc_c (the chunk's bottom value with a zero carry-in).
-__global__ void k_kate_local(const uint32_t* __restrict__ a, size_t m, uint32_t KL, FrArg b,
-                             uint32_t* __restrict__ loc) {
-  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t lo = c * KL;
-  if (lo >= m) return;
-  const size_t hi = lo + KL < m ? lo + KL : m;
-  const fr_t bb = arg_fr(b);
-  fr_t q = fr_t::zero();
-  for (size_t i = hi; i-- > lo;) q = add(load<FrP>(a + 8 * (i + 1)), mul(bb, q));
-  store<FrP>(loc + 8 * c, q);
+struct KateJob {
+  const uint32_t* a;
+  uint32_t* q;
+  uint32_t *loc, *suf, *tot, *gcarry;   // scratch: C, G whole groups, G, G elements
+  uint64_t m, C, G;
+  FrArg b, g256;
+  KatePows P;
+};
+
+struct KateBatch {
+  uint32_t njobs;
+  uint32_t first_chunk_block[KATE_MAXJ + 1];   // k_kate_local / k_kate_final workgroups
+  uint32_t first_group[KATE_MAXJ + 1];         // k_kate_group workgroups
+  KateJob j[KATE_MAXJ];
+};
+static_assert(sizeof(KateBatch) <= 4000, "kernel argument block too large");
+
+// the division a workgroup belongs to (uniform across the workgroup)
+__device__ __forceinline__ uint32_t kate_job_of(const uint32_t* first, uint32_t njobs, uint32_t blk) {
+  uint32_t jb = 0;
+  while (jb + 1 < njobs && blk >= first[jb + 1]) jb++;
+  return __builtin_amdgcn_readfirstlane(jb);
 }
 
-// Phase 2 (one block): carry-in of every chunk, i.e. q at the bottom of the chunk above.
-// The maps f_c(x) = loc_c + g x (g = b^L) compose from the top: carry_c = f_(c+1)(carry_(c+1)).
-// Thread t owns chunks [t S, t S + S) (S = ceil(C / 256)); a suffix scan of the composed
+// Phase 1: loc_c (the chunk's bottom value with a zero carry-in).
+__global__ void __launch_bounds__(KATE_CHUNK_BLOCK) k_kate_local(const KateBatch B) {
+  const uint32_t jb = kate_job_of(B.first_chunk_block, B.njobs, blockIdx.x);
+  const KateJob& J = B.j[jb];
+  const size_t c = (size_t)(blockIdx.x - B.first_chunk_block[jb]) * KATE_CHUNK_BLOCK + threadIdx.x;
+  const size_t lo = c * KATE_L;
+  if (lo >= J.m) return;
+  const size_t hi = lo + KATE_L < J.m ? lo + KATE_L : J.m;
+  const fr_t bb = arg_fr(J.b);
+  fr_t q = fr_t::zero();
+  for (size_t i = hi; i-- > lo;) q = add(load<FrP>(J.a + 8 * (i + 1)), mul(bb, q));
+  store<FrP>(J.loc + 8 * c, q);
+}
+
+// Phase 3 (one workgroup per division): carry-in of every group, i.e. the value at the
+// bottom of the group above.  The group maps F_g(x) = tot_g + g256 x compose from the top;
+// thread t owns groups [t S, t S + S) (S = ceil(G / 256)); a suffix scan of the composed
 // per-thread maps gives each thread its incoming value.
-__global__ void __launch_bounds__(256) k_kate_carries(const uint32_t* __restrict__ loc, size_t C, FrArg g_L,
-                                                      uint32_t* __restrict__ carry) {
+__global__ void __launch_bounds__(256) k_kate_carries(const KateBatch B) {
   __shared__ uint32_t la[8 * 256], lm[8 * 256];
+  const KateJob& J = B.j[blockIdx.x];
   const int t = threadIdx.x;
+  const size_t C = J.G;
   const size_t S = (C + 255) / 256;
   const size_t c0 = t * S;
   const size_t c1 = c0 + S < C ? c0 + S : C;
-  const fr_t g = arg_fr(g_L);
+  const fr_t g = arg_fr(J.g256);
   // thread map F_t = f_c0 o f_(c0+1) o ... o f_(c1-1): F(x) = A + M x
   fr_t A = fr_t::zero(), M = fr_t::one();
-  for (size_t c = c1; c-- > c0;) {   // apply the lower chunk last: F <- f_c o F
-    A = add(load<FrP>(loc + 8 * c), mul(g, A));
+  for (size_t c = c1; c-- > c0;) {   // apply the lower group last: F <- f_c o F
+    A = add(load<FrP>(J.tot + 8 * c), mul(g, A));
     M = mul(g, M);
   }
   if (c0 >= C) {
@@ -320,19 +351,21 @@ __global__ void __launch_bounds__(256) k_kate_carries(const uint32_t* __restrict
   // value entering thread t from above = (F_(t+1) o ...)(0) = suffix A of t + 1
   fr_t in = t + 1 < 256 ? load<FrP>(la + 8 * (t + 1)) : fr_t::zero();
   for (size_t c = c1; c-- > c0;) {
-    store<FrP>(carry + 8 * c, in);
-    in = add(load<FrP>(loc + 8 * c), mul(g, in));
+    store<FrP>(J.gcarry + 8 * c, in);
+    in = add(load<FrP>(J.tot + 8 * c), mul(g, in));
   }
 }
 
-// Group suffix compositions: suf[c] = sum_(j >= c, j in c's group) g^(j - c) loc_j (chunks
-// past C count as loc = 0); tot[group] = suf of the group's first chunk.
-__global__ void __launch_bounds__(KATE_GROUP) k_kate_group(const uint32_t* __restrict__ loc, size_t C, KatePows P,
-                                                           uint32_t* __restrict__ suf, uint32_t* __restrict__ tot) {
+// Phase 2, group suffix compositions: suf[c] = sum_(j >= c, j in c's group) g^(j - c) loc_j
+// (chunks past C count as loc = 0); tot[group] = suf of the group's first chunk.
+__global__ void __launch_bounds__(KATE_GROUP) k_kate_group(const KateBatch B) {
   __shared__ uint32_t la[8 * KATE_GROUP];
+  const uint32_t jb = kate_job_of(B.first_group, B.njobs, blockIdx.x);
+  const KateJob& J = B.j[jb];
+  const uint32_t grp = blockIdx.x - B.first_group[jb];
   const uint32_t t = threadIdx.x;
-  const size_t c = (size_t)blockIdx.x * KATE_GROUP + t;
-  fr_t A = c < C ? load<FrP>(loc + 8 * c) : fr_t::zero();
+  const size_t c = (size_t)grp * KATE_GROUP + t;
+  fr_t A = c < J.C ? load<FrP>(J.loc + 8 * c) : fr_t::zero();
   store<FrP>(la + 8 * t, A);
   __syncthreads();
 #pragma unroll 1
@@ -340,38 +373,38 @@ __global__ void __launch_bounds__(KATE_GROUP) k_kate_group(const uint32_t* __res
     const uint32_t d = 1u << k;
     const fr_t nb = t + d < KATE_GROUP ? load<FrP>(la + 8 * (t + d)) : fr_t::zero();
     __syncthreads();
-    A = add(A, mul(arg_fr(P.g2k[k]), nb));
+    A = add(A, mul(arg_fr(J.P.g2k[k]), nb));
     store<FrP>(la + 8 * t, A);
     __syncthreads();
   }
-  store<FrP>(suf + 8 * c, A);   // whole groups: chunks past C hold 0, as the final pass reads
-  if (t == 0) store<FrP>(tot + 8 * (size_t)blockIdx.x, A);
+  store<FrP>(J.suf + 8 * c, A);   // whole groups: chunks past C hold 0, as the final pass reads
+  if (t == 0) store<FrP>(J.tot + 8 * (size_t)grp, A);
 }
 
-// Each chunk with its true carry-in (suf of the next chunk carried by the group's
+// Phase 4: each chunk with its true carry-in (suf of the next chunk carried by the group's
 // carry-in gcarry, or gcarry itself for a group's last chunk); writes q.
-__global__ void k_kate_final(const uint32_t* __restrict__ a, size_t m, FrArg b, KatePows P,
-                             const uint32_t* __restrict__ suf, const uint32_t* __restrict__ gcarry,
-                             uint32_t* __restrict__ q) {
-  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(KATE_CHUNK_BLOCK) k_kate_final(const KateBatch B) {
+  const uint32_t jb = kate_job_of(B.first_chunk_block, B.njobs, blockIdx.x);
+  const KateJob& J = B.j[jb];
+  const size_t c = (size_t)(blockIdx.x - B.first_chunk_block[jb]) * KATE_CHUNK_BLOCK + threadIdx.x;
   const size_t lo = c * KATE_L;
-  if (lo >= m) return;
-  const size_t hi = lo + KATE_L < m ? lo + KATE_L : m;
+  if (lo >= J.m) return;
+  const size_t hi = lo + KATE_L < J.m ? lo + KATE_L : J.m;
   const uint32_t i = (uint32_t)(c % KATE_GROUP);
-  const fr_t gc = load<FrP>(gcarry + 8 * (c / KATE_GROUP));
+  const fr_t gc = load<FrP>(J.gcarry + 8 * (c / KATE_GROUP));
   fr_t v = gc;
   if (i + 1 < KATE_GROUP) {   // suf[c + 1] + g^(255 - i) gcarry (suf exists: the group is allocated whole)
     const uint32_t e = KATE_GROUP - 1 - i;
     fr_t ge = fr_t::one();
 #pragma unroll 1
     for (int k = 0; k < 8; k++)
-      if ((e >> k) & 1u) ge = mul(ge, arg_fr(P.g2k[k]));
-    v = add(load<FrP>(suf + 8 * (c + 1)), mul(ge, gc));
+      if ((e >> k) & 1u) ge = mul(ge, arg_fr(J.P.g2k[k]));
+    v = add(load<FrP>(J.suf + 8 * (c + 1)), mul(ge, gc));
   }
-  const fr_t bb = arg_fr(b);
+  const fr_t bb = arg_fr(J.b);
   for (size_t j = hi; j-- > lo;) {
-    v = add(load<FrP>(a + 8 * (j + 1)), mul(bb, v));
-    store<FrP>(q + 8 * j, v);
+    v = add(load<FrP>(J.a + 8 * (j + 1)), mul(bb, v));
+    store<FrP>(J.q + 8 * j, v);
   }
 }
 
@@ -681,39 +714,98 @@ int mzk_fr_eval_batch(mzk_ctx* c, const void* const* polys, const size_t* ns, co
   return MZK_OK;
 }
 
+// One launch of each pass for up to KATE_MAXJ divisions (every job has n >= 2).
+static int kate_launch(mzk_ctx* c, const mzk_kate_job* jobs, size_t nj) {
+  KateBatch B;
+  memset(&B, 0, sizeof B);
+  B.njobs = (uint32_t)nj;
+  size_t words = 0;
+  for (size_t i = 0; i < nj; i++) {
+    const size_t m = jobs[i].n - 1, C = (m + KATE_L - 1) / KATE_L, G = (C + KATE_GROUP - 1) / KATE_GROUP;
+    words += 8 * (C + G * KATE_GROUP + 2 * G);
+  }
+  MZK_TRY(c->scratch[SCR_MSM_D].reserve(words * 4 + 64));
+  uint32_t* w = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
+  uint32_t cb = 0, gb = 0;
+  for (size_t i = 0; i < nj; i++) {
+    KateJob& J = B.j[i];
+    J.a = (const uint32_t*)jobs[i].d_a;
+    J.q = (uint32_t*)jobs[i].d_q;
+    J.m = jobs[i].n - 1;
+    J.C = (J.m + KATE_L - 1) / KATE_L;
+    J.G = (J.C + KATE_GROUP - 1) / KATE_GROUP;
+    J.loc = w;
+    J.suf = J.loc + 8 * J.C;
+    J.tot = J.suf + 8 * J.G * KATE_GROUP;
+    J.gcarry = J.tot + 8 * J.G;
+    w = J.gcarry + 8 * J.G;
+    host::Fr hb;
+    memcpy(hb.l, jobs[i].b, 32);
+    J.b = to_arg(hb);
+    host::Fr gk = host::fr_pow(hb, KATE_L);
+    for (int k = 0; k < 8; k++) {
+      J.P.g2k[k] = to_arg(gk);
+      gk = host::fr_mul(gk, gk);
+    }
+    J.g256 = to_arg(gk);   // g^(2^8): one group's multiplier
+    B.first_chunk_block[i] = cb;
+    B.first_group[i] = gb;
+    cb += (uint32_t)((J.C + KATE_CHUNK_BLOCK - 1) / KATE_CHUNK_BLOCK);
+    gb += (uint32_t)J.G;
+  }
+  B.first_chunk_block[nj] = cb;
+  B.first_group[nj] = gb;
+  MZK_LAUNCH(c, "k_kate_local", k_kate_local, dim3(cb), dim3(KATE_CHUNK_BLOCK), 0, B);
+  MZK_LAUNCH(c, "k_kate_group", k_kate_group, dim3(gb), dim3(KATE_GROUP), 0, B);
+  MZK_LAUNCH(c, "k_kate_carries", k_kate_carries, dim3((unsigned)nj), dim3(256), 0, B);
+  MZK_LAUNCH(c, "k_kate_final", k_kate_final, dim3(cb), dim3(KATE_CHUNK_BLOCK), 0, B);
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
+}
+
+static bool spans_overlap(const void* p, size_t pb, const void* q, size_t qb) {
+  const uintptr_t a = (uintptr_t)p, b = (uintptr_t)q;
+  return a < b + qb && b < a + pb;
+}
+
+int mzk_fr_kate_div_batch(mzk_ctx* c, const mzk_kate_job* jobs, size_t njobs) {
+  MZK_REQUIRE(c != nullptr && (jobs != nullptr || njobs == 0), "mzk_fr_kate_div_batch: null argument");
+  std::vector<mzk_kate_job> live;
+  for (size_t i = 0; i < njobs; i++) {
+    const mzk_kate_job& j = jobs[i];
+    MZK_REQUIRE(j.n >= 1, "mzk_fr_kate_div_batch: job %zu: empty polynomial", i);
+    MZK_REQUIRE(j.n <= ((size_t)1 << 28), "mzk_fr_kate_div_batch: job %zu: n = %zu above 2^28", i, j.n);
+    if (j.n == 1) continue;   // the quotient has no coefficients
+    MZK_REQUIRE(j.d_a && j.d_q, "mzk_fr_kate_div_batch: job %zu: null data", i);
+    live.push_back(j);
+  }
+  // every job runs at once: no output may overlap any job's input or another output
+  for (size_t i = 0; i < live.size(); i++)
+    for (size_t k = 0; k < live.size(); k++) {
+      MZK_REQUIRE(!spans_overlap(live[i].d_q, (live[i].n - 1) * 32, live[k].d_a, live[k].n * 32),
+                  "mzk_fr_kate_div_batch: an output overlaps an input");
+      MZK_REQUIRE(i == k || !spans_overlap(live[i].d_q, (live[i].n - 1) * 32, live[k].d_q, (live[k].n - 1) * 32),
+                  "mzk_fr_kate_div_batch: two outputs overlap");
+    }
+  if (live.empty()) return MZK_OK;
+  MZK_TRY(vec_ready(c));
+  std::lock_guard<std::mutex> g(c->mu);
+  for (size_t i = 0; i < live.size(); i += KATE_MAXJ)
+    MZK_TRY(kate_launch(c, live.data() + i, std::min<size_t>(KATE_MAXJ, live.size() - i)));
+  return MZK_OK;
+}
+
 int mzk_fr_kate_div(mzk_ctx* c, const void* a, size_t n, const uint64_t b[4], void* q) {
   MZK_REQUIRE(c != nullptr && b != nullptr, "mzk_fr_kate_div: null argument");
   MZK_REQUIRE(n >= 1, "mzk_fr_kate_div: empty polynomial");
   if (n == 1) return MZK_OK;   // the quotient has no coefficients
   MZK_REQUIRE(a && q && a != q, "mzk_fr_kate_div: null or aliased data");
-  const size_t m = n - 1;
-  const size_t C = (m + KATE_L - 1) / KATE_L;
-  const size_t G = (C + KATE_GROUP - 1) / KATE_GROUP;
-  MZK_TRY(vec_ready(c));
-  std::lock_guard<std::mutex> g(c->mu);
-  // scratch: loc (C), suf (G whole groups), group totals (G), group carries (G)
-  MZK_TRY(c->scratch[SCR_MSM_D].reserve((C + G * KATE_GROUP + 2 * G) * 32 + 64));
-  uint32_t* loc = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
-  uint32_t* suf = loc + 8 * C;
-  uint32_t* tot = suf + 8 * G * KATE_GROUP;
-  uint32_t* gcarry = tot + 8 * G;
-  host::Fr hb;
-  memcpy(hb.l, b, 32);
-  KatePows P;
-  host::Fr gk = host::fr_pow(hb, KATE_L);
-  for (int k = 0; k < 8; k++) {
-    P.g2k[k] = to_arg(gk);
-    gk = host::fr_mul(gk, gk);
-  }
-  const host::Fr g256 = gk;   // g^(2^8): one group's multiplier
-  MZK_LAUNCH(c, "k_kate_local", k_kate_local, dim3(grid_for(C, 64)), dim3(64), 0, (const uint32_t*)a, m, KATE_L,
-             to_arg(b), loc);
-  MZK_LAUNCH(c, "k_kate_group", k_kate_group, dim3((unsigned)G), dim3(KATE_GROUP), 0, loc, C, P, suf, tot);
-  MZK_LAUNCH(c, "k_kate_carries", k_kate_carries, dim3(1), dim3(256), 0, tot, G, to_arg(g256), gcarry);
-  MZK_LAUNCH(c, "k_kate_final", k_kate_final, dim3(grid_for(C, 64)), dim3(64), 0, (const uint32_t*)a, m, to_arg(b),
-             P, suf, gcarry, (uint32_t*)q);
-  MZK_HIP(hipGetLastError());
-  return MZK_OK;
+  mzk_kate_job j;
+  j.d_a = a;
+  j.n = n;
+  memcpy(j.b, b, 32);
+  j.d_q = q;
+  return mzk_fr_kate_div_batch(c, &j, 1);
 }
 
 int mzk_plonk_perm_products_dev(mzk_ctx* c, const void* omega_powers, size_t n, const mzk_perm_set* sets,

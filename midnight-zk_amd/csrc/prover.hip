@@ -636,6 +636,17 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
     x1[p] = T[p].squeeze();
     x2[p] = T[p].squeeze();
   }
+  // q polynomials per proof, then every proof's point-set divisions stage by stage: stage i
+  // divides each set with more than i points once, all sets of all proofs in one batch
+  // (the reference divides the sets in parallel, kzg/mod.rs:226-236)
+  struct Chain {
+    const void* cur;
+    size_t len;
+    std::vector<Fr> pts;
+    uint32_t* out;
+    uint32_t* tmp[2];
+  };
+  std::vector<Chain> chains;
   for (size_t p = 0; p < np; p++) {
     ProofBufs& b = B[p];
     labels.clear();
@@ -659,7 +670,6 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
     MZK_REQUIRE(ord.size() == 3, "mzk_plonk_prove: unexpected query structure");
     // q polynomials: sum of the set's members with powers of x1 (into q0, k1, k2 by sorted slot)
     uint32_t* qbuf[3] = {b.q0, b.k1, b.k2};
-    std::vector<std::pair<const void*, size_t>> qs(3);
     for (int slot = 0; slot < 3; slot++) {
       auto& m = members[ord[slot]];
       std::vector<Fr> pw;
@@ -669,24 +679,48 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
         t = host::fr_mul(t, x1[p]);
       }
       PV_TRY(lincomb(c, m, pw, qbuf[slot], n));
-      qs[slot] = {qbuf[slot], n};
     }
-    // f = sum_slot x2^slot q_slot / prod (X - point)
+    // intermediate quotients ping-pong through fin, pi and lin (free from here until the
+    // final phase; lin only fed the q polynomials): a set of L points takes min(L - 1, 2)
+    uint32_t* spare[3] = {b.fin, b.pi, b.lin};
+    int used = 0;
     uint32_t* fout[3] = {b.fa, b.fb, b.fc};
-    std::vector<std::pair<const void*, size_t>> fparts;
     for (int slot = 0; slot < 3; slot++) {
-      const auto& pts = point_sets[ord[slot]];
-      const void* cur = qs[slot].first;
-      size_t len = n;
-      uint32_t* tmp[2] = {b.fin, b.pi};   // free until the final phase
-      for (size_t i = 0; i < pts.size(); i++) {
-        uint32_t* dst = i + 1 == pts.size() ? fout[slot] : tmp[i & 1];
-        PV_TRY(mzk_fr_kate_div(c, cur, len, pts[i].l, dst));
-        cur = dst;
-        len -= 1;
-      }
-      fparts.push_back({fout[slot], len});
+      Chain ch;
+      ch.cur = qbuf[slot];
+      ch.len = n;
+      ch.pts = point_sets[ord[slot]];
+      ch.out = fout[slot];
+      const int need = (int)std::min<size_t>(ch.pts.size() > 0 ? ch.pts.size() - 1 : 0, 2);
+      MZK_REQUIRE(used + need <= 3, "mzk_plonk_prove: point sets need more intermediate buffers");
+      ch.tmp[0] = need > 0 ? spare[used] : nullptr;
+      ch.tmp[1] = need > 1 ? spare[used + 1] : nullptr;
+      used += need;
+      chains.push_back(ch);
     }
+  }
+  for (size_t stage = 0;; stage++) {
+    std::vector<mzk_kate_job> kj;
+    for (Chain& ch : chains) {
+      if (stage >= ch.pts.size()) continue;
+      uint32_t* dst = stage + 1 == ch.pts.size() ? ch.out : ch.tmp[stage & 1];
+      mzk_kate_job j;
+      j.d_a = ch.cur;
+      j.n = ch.len;
+      memcpy(j.b, ch.pts[stage].l, 32);
+      j.d_q = dst;
+      kj.push_back(j);
+      ch.cur = dst;
+      ch.len -= 1;
+    }
+    if (kj.empty()) break;
+    PV_TRY(mzk_fr_kate_div_batch(c, kj.data(), kj.size()));
+  }
+  for (size_t p = 0; p < np; p++) {
+    ProofBufs& b = B[p];
+    // f = sum_slot x2^slot q_slot / prod (X - point)
+    std::vector<std::pair<const void*, size_t>> fparts;
+    for (int slot = 0; slot < 3; slot++) fparts.push_back({chains[3 * p + slot].out, chains[3 * p + slot].len});
     std::vector<Fr> pw = {one, x2[p], host::fr_mul(x2[p], x2[p])};
     PV_TRY(lincomb(c, fparts, pw, b.f, n - 1));
   }
@@ -732,11 +766,16 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
     }
     std::vector<Fr> v;
     PV_TRY(eval_batch(c, polys, pts, v));
+    std::vector<mzk_kate_job> kj(np);
     for (size_t p = 0; p < np; p++) {
       ProofBufs& b = B[p];
       PV_TRY(vscal(c, 1, b.fin, fr_neg(v[p]), b.fin, 1));   // final(X) - v
-      PV_TRY(mzk_fr_kate_div(c, b.fin, n, x3[p].l, b.pi));
+      kj[p].d_a = b.fin;
+      kj[p].n = n;
+      memcpy(kj[p].b, x3[p].l, 32);
+      kj[p].d_q = b.pi;
     }
+    PV_TRY(mzk_fr_kate_div_batch(c, kj.data(), np));
     std::vector<std::pair<const void*, size_t>> jobs;
     for (size_t p = 0; p < np; p++) jobs.push_back({B[p].pi, n - 1});
     PV_TRY(msm_batch(c, key->g, jobs, jac));

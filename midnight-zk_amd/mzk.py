@@ -82,6 +82,12 @@ class PermSet(ctypes.Structure):
                 ("den", ctypes.c_void_p)]
 
 
+class KateJob(ctypes.Structure):
+    """mzk_kate_job: d_q = quotient of d_a (n coefficients) by (X - b)."""
+    _fields_ = [("d_a", ctypes.c_void_p), ("n", ctypes.c_size_t), ("b", ctypes.c_uint64 * 4),
+                ("d_q", ctypes.c_void_p)]
+
+
 class PlonkKey(ctypes.Structure):
     """mzk_plonk_key: the native prover's view of a proving key (device pointers)."""
     _fields_ = [("k", ctypes.c_uint32), ("g", ctypes.c_void_p), ("g_lagrange", ctypes.c_void_p),
@@ -165,6 +171,7 @@ SIGNATURES = {
     "mzk_plonk_prove": ([_P, _P, _S, _P, _P, _P, _P], _I),
     "mzk_fr_lincomb_dev": ([_P, _P, _P, _P, _S, _P, _S], _I),
     "mzk_fr_kate_div": ([_P, _P, _S, _P, _P], _I),
+    "mzk_fr_kate_div_batch": ([_P, _P, _S], _I),
     "mzk_fr_powers": ([_P, _P, _S, _P], _I),
 }
 

@@ -933,6 +933,19 @@ class GpuBackend:
                                                 self._p(q) if q.shape[0] else None))
         return q
 
+    def kate_div_many(self, items):
+        """[(a, b)] -> the quotients of a(X) - a(b) by (X - b), one batched call."""
+        import ctypes
+        qs = [self._empty(max(0, a.shape[0] - 1)) for a, _ in items]
+        jobs = (self.mzk.KateJob * max(1, len(items)))()
+        for j, ((a, b), q) in enumerate(zip(items, qs)):
+            jobs[j].d_a = self._p(a)
+            jobs[j].n = a.shape[0]
+            jobs[j].b[:] = [int(w) for w in mont_limbs(b)]
+            jobs[j].d_q = self._p(q) if q.shape[0] else None
+        self.mzk._check(self._L.mzk_fr_kate_div_batch(self.ctx.h, ctypes.addressof(jobs), len(items)))
+        return qs
+
     # -- domains and commitments
     def domain(self, j: int, k: int):
         return GpuDomain(self, j, k)

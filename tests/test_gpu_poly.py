@@ -2,6 +2,8 @@
 products, Horner evaluation, Kate division, powers, bases from scalars) on the device
 against the oracle's C restatement, bit-exact, at sizes that cross every tile and
 chunk boundary of the kernels (GPU)."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -115,6 +117,35 @@ def test_kate_division(be, n):
         ab = plonk.from_mont_limbs(C.poly_eval(a, plonk.mont_limbs(b)))
         az = plonk.from_mont_limbs(C.poly_eval(a, plonk.mont_limbs(z)))
         assert (qz * (z - b) + ab) % O.R_MOD == az
+
+
+def test_kate_division_batch_equals_per_call(be):
+    # more jobs than one launch takes, mixed sizes (n = 1 writes nothing), distinct points
+    sizes = [2, 3, 65, 1, 4096, 4097, 64 * 256 * 3 + 17, 129, (1 << 16) + 1, 7, 300, 64 * 256 + 1]
+    polys = [C.fr_from_seed(40 + i, n, n) for i, n in enumerate(sizes)]
+    pts = [(0x1234567 * (i + 3) + 11) % O.R_MOD for i in range(len(sizes))]
+    print(f"\n[kate batch] {len(sizes)} jobs", flush=True)
+    got = be.kate_div_many([(to_dev(a), b) for a, b in zip(polys, pts)])
+    for a, b, q in zip(polys, pts, got):
+        want = C.fr_kate_div(a, plonk.mont_limbs(b))
+        assert np.array_equal(host(be, q) if q.shape[0] else np.zeros((0, 4), dtype=np.uint64), want)
+
+
+def test_kate_division_batch_rejects_overlap(be):
+    L = mzk.load_library()
+    d = to_dev(C.fr_from_seed(2, 1, 128))          # every span below stays inside it
+    jobs = (mzk.KateJob * 2)()
+    for j in jobs:
+        j.d_a = d.data_ptr()
+        j.n = 32
+        j.b[:] = [int(w) for w in plonk.mont_limbs(5)]
+    jobs[0].d_q = d.data_ptr() + 32 * 32          # disjoint from both inputs
+    jobs[1].d_q = d.data_ptr() + 32 * 16          # inside the inputs
+    assert L.mzk_fr_kate_div_batch(be.ctx.h, ctypes.addressof(jobs), 2) == 1
+    jobs[1].d_q = d.data_ptr() + 32 * 40          # overlaps job 0's output
+    assert L.mzk_fr_kate_div_batch(be.ctx.h, ctypes.addressof(jobs), 2) == 1
+    jobs[1].n = 0
+    assert L.mzk_fr_kate_div_batch(be.ctx.h, ctypes.addressof(jobs), 2) == 1
 
 
 @pytest.mark.parametrize("n", [1, 7, 4096, (1 << 20) + 5])
