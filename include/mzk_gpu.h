@@ -197,6 +197,16 @@ int mzk_fr_batch_invert(mzk_ctx *ctx, void *d_a, size_t n);
  * permutation/prover.rs:190-197 is the exclusive form, out shifted by one).  n <= 2^24.
  * d_out must not alias d_a. */
 int mzk_fr_prefix_product(mzk_ctx *ctx, const void *d_a, void *d_out, size_t n);
+/* Many independent inclusive product scans in one launch of each phase: the running
+ * products of every permutation set (permutation/prover.rs:190-197, one per set and
+ * proof).  Job j: d_out = prefix products of d_a (n <= 2^24).  Results equal the per-job
+ * mzk_fr_prefix_product.  No output may overlap any job's input or another output. */
+typedef struct mzk_prefix_job {
+  const void *d_a;
+  void *d_out;
+  size_t n;
+} mzk_prefix_job;
+int mzk_fr_prefix_product_batch(mzk_ctx *ctx, const mzk_prefix_job *jobs, size_t njobs);
 /* sum_i a_i x^i to host memory (eval_polynomial, utils/arithmetic.rs:59-87). */
 int mzk_fr_eval(mzk_ctx *ctx, const void *d_a, size_t n, const uint64_t x[4], uint64_t out[4]);
 /* njobs evaluations in one launch and one wait: outs[4j..4j+4) = sum_i polys[j][i] x_j^i,

@@ -121,48 +121,86 @@ __device__ fr_t block_excl_prod(fr_t v, uint32_t* lds, fr_t& total) {
   return excl;
 }
 
+// Several independent scans share each phase's launch (the permutation argument's running
+// products of every set and proof, permutation/prover.rs:190-197): a workgroup finds its
+// scan from the first-tile table.
+static constexpr int PROD_MAXJ = 48;
+static constexpr uint32_t PROD_TILE = SCAN_T * SCAN_E;
+
+struct ProdJob {
+  const uint32_t* a;
+  uint32_t* out;
+  uint32_t* totals;   // one per tile
+  uint64_t n;
+};
+
+struct ProdBatch {
+  uint32_t njobs;
+  uint32_t first_tile[PROD_MAXJ + 1];
+  ProdJob j[PROD_MAXJ];
+};
+static_assert(sizeof(ProdBatch) <= 4000, "kernel argument block too large");
+
+__device__ __forceinline__ uint32_t prod_job_of(const ProdBatch& B, uint32_t tile) {
+  uint32_t lo = 0, hi = B.njobs;   // first_tile[lo] <= tile < first_tile[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (B.first_tile[mid] <= tile) lo = mid; else hi = mid;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
 // Phase 1: inclusive products within each tile of SCAN_T * SCAN_E; tile totals.
-__global__ void __launch_bounds__(SCAN_T) k_prod_tiles(const uint32_t* __restrict__ a, uint32_t* __restrict__ out,
-                                                       size_t n, uint32_t* __restrict__ totals) {
+__global__ void __launch_bounds__(SCAN_T) k_prod_tiles(const ProdBatch B) {
   __shared__ uint32_t lds[8 * SCAN_T];
-  const size_t base = ((size_t)blockIdx.x * SCAN_T + threadIdx.x) * SCAN_E;
+  const uint32_t jb = prod_job_of(B, blockIdx.x);
+  const ProdJob& J = B.j[jb];
+  const uint32_t tile = blockIdx.x - B.first_tile[jb];
+  const size_t base = ((size_t)tile * SCAN_T + threadIdx.x) * SCAN_E;
   fr_t p = fr_t::one();
   for (int k = 0; k < SCAN_E; k++)
-    if (base + k < n) p = mul(p, load<FrP>(a + 8 * (base + k)));
+    if (base + k < J.n) p = mul(p, load<FrP>(J.a + 8 * (base + k)));
   fr_t total;
   fr_t run = block_excl_prod(p, lds, total);
   for (int k = 0; k < SCAN_E; k++) {
-    if (base + k >= n) break;
-    run = mul(run, load<FrP>(a + 8 * (base + k)));
-    store<FrP>(out + 8 * (base + k), run);
+    if (base + k >= J.n) break;
+    run = mul(run, load<FrP>(J.a + 8 * (base + k)));
+    store<FrP>(J.out + 8 * (base + k), run);
   }
-  if (threadIdx.x == 0) store<FrP>(totals + 8 * blockIdx.x, total);
+  if (threadIdx.x == 0) store<FrP>(J.totals + 8 * tile, total);
 }
 
-// Phase 2: one block turns the tile totals into exclusive prefixes (tiles <= SCAN_T * SCAN_E).
-__global__ void __launch_bounds__(SCAN_T) k_prod_totals(uint32_t* __restrict__ totals, size_t tiles) {
+// Phase 2: one workgroup per scan turns its tile totals into exclusive prefixes
+// (tiles <= SCAN_T * SCAN_E).
+__global__ void __launch_bounds__(SCAN_T) k_prod_totals(const ProdBatch B) {
   __shared__ uint32_t lds[8 * SCAN_T];
+  const ProdJob& J = B.j[blockIdx.x];
+  const size_t tiles = B.first_tile[blockIdx.x + 1] - B.first_tile[blockIdx.x];
   const size_t base = (size_t)threadIdx.x * SCAN_E;
   fr_t p = fr_t::one();
   for (int k = 0; k < SCAN_E; k++)
-    if (base + k < tiles) p = mul(p, load<FrP>(totals + 8 * (base + k)));
+    if (base + k < tiles) p = mul(p, load<FrP>(J.totals + 8 * (base + k)));
   fr_t total;
   fr_t run = block_excl_prod(p, lds, total);
   for (int k = 0; k < SCAN_E; k++) {
     if (base + k >= tiles) break;
-    fr_t v = load<FrP>(totals + 8 * (base + k));
-    store<FrP>(totals + 8 * (base + k), run);
+    fr_t v = load<FrP>(J.totals + 8 * (base + k));
+    store<FrP>(J.totals + 8 * (base + k), run);
     run = mul(run, v);
   }
 }
 
-// Phase 3: out[i] *= exclusive prefix of its tile.
-__global__ void k_prod_apply(uint32_t* __restrict__ out, size_t n, const uint32_t* __restrict__ totals) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const size_t tile = i / (SCAN_T * SCAN_E);
+// Phase 3: out[i] *= exclusive prefix of its tile; PROD_TILE / 256 workgroups per tile.
+__global__ void __launch_bounds__(256) k_prod_apply(const ProdBatch B) {
+  constexpr uint32_t per_tile = PROD_TILE / 256;
+  const uint32_t gtile = blockIdx.x / per_tile;
+  const uint32_t jb = prod_job_of(B, gtile);
+  const ProdJob& J = B.j[jb];
+  const uint32_t tile = gtile - B.first_tile[jb];
   if (tile == 0) return;
-  store<FrP>(out + 8 * i, mul(load<FrP>(out + 8 * i), load<FrP>(totals + 8 * tile)));
+  const size_t i = (size_t)tile * PROD_TILE + (blockIdx.x % per_tile) * 256 + threadIdx.x;
+  if (i >= J.n) return;
+  store<FrP>(J.out + 8 * i, mul(load<FrP>(J.out + 8 * i), load<FrP>(J.totals + 8 * tile)));
 }
 
 // ---------------------------------------------------------------- evaluation
@@ -609,23 +647,72 @@ int mzk_fr_batch_invert(mzk_ctx* c, void* a, size_t n) {
   return MZK_OK;
 }
 
+static bool spans_overlap(const void* p, size_t pb, const void* q, size_t qb) {
+  const uintptr_t a = (uintptr_t)p, b = (uintptr_t)q;
+  return a < b + qb && b < a + pb;
+}
+
+int mzk_fr_prefix_product_batch(mzk_ctx* c, const mzk_prefix_job* jobs, size_t njobs) {
+  MZK_REQUIRE(c != nullptr && (jobs != nullptr || njobs == 0), "mzk_fr_prefix_product_batch: null argument");
+  std::vector<mzk_prefix_job> live;
+  for (size_t i = 0; i < njobs; i++) {
+    const mzk_prefix_job& j = jobs[i];
+    if (j.n == 0) continue;
+    MZK_REQUIRE(j.d_a && j.d_out, "mzk_fr_prefix_product_batch: job %zu: null data", i);
+    MZK_REQUIRE(j.n <= (size_t)PROD_TILE * PROD_TILE, "mzk_fr_prefix_product_batch: job %zu: n above %zu", i,
+                (size_t)PROD_TILE * PROD_TILE);
+    live.push_back(j);
+  }
+  // every scan runs at once: no output may overlap any input or another output
+  for (size_t i = 0; i < live.size(); i++)
+    for (size_t k = 0; k < live.size(); k++) {
+      MZK_REQUIRE(!spans_overlap(live[i].d_out, live[i].n * 32, live[k].d_a, live[k].n * 32),
+                  "mzk_fr_prefix_product_batch: an output overlaps an input");
+      MZK_REQUIRE(i == k || !spans_overlap(live[i].d_out, live[i].n * 32, live[k].d_out, live[k].n * 32),
+                  "mzk_fr_prefix_product_batch: two outputs overlap");
+    }
+  if (live.empty()) return MZK_OK;
+  MZK_TRY(vec_ready(c));
+  std::lock_guard<std::mutex> g(c->mu);
+  size_t all_tiles = 0;
+  for (auto& j : live) all_tiles += (j.n + PROD_TILE - 1) / PROD_TILE;
+  MZK_TRY(c->scratch[SCR_MSM_D].reserve(all_tiles * 32 + 64));
+  uint32_t* totals = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
+  for (size_t i0 = 0; i0 < live.size(); i0 += PROD_MAXJ) {
+    const size_t nj = std::min<size_t>(PROD_MAXJ, live.size() - i0);
+    ProdBatch B;
+    memset(&B, 0, sizeof B);
+    B.njobs = (uint32_t)nj;
+    uint32_t t = 0;
+    for (size_t i = 0; i < nj; i++) {
+      const mzk_prefix_job& j = live[i0 + i];
+      B.j[i].a = (const uint32_t*)j.d_a;
+      B.j[i].out = (uint32_t*)j.d_out;
+      B.j[i].n = j.n;
+      B.j[i].totals = totals;
+      B.first_tile[i] = t;
+      const uint32_t tiles = (uint32_t)((j.n + PROD_TILE - 1) / PROD_TILE);
+      t += tiles;
+      totals += 8 * (size_t)tiles;
+    }
+    B.first_tile[nj] = t;
+    MZK_LAUNCH(c, "k_prod_tiles", k_prod_tiles, dim3(t), dim3(SCAN_T), 0, B);
+    MZK_LAUNCH(c, "k_prod_totals", k_prod_totals, dim3((unsigned)nj), dim3(SCAN_T), 0, B);
+    MZK_LAUNCH(c, "k_prod_apply", k_prod_apply, dim3(t * (PROD_TILE / 256)), dim3(256), 0, B);
+  }
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
+}
+
 int mzk_fr_prefix_product(mzk_ctx* c, const void* a, void* out, size_t n) {
   MZK_REQUIRE(c != nullptr, "null context");
   if (n == 0) return MZK_OK;
   MZK_REQUIRE(a && out, "mzk_fr_prefix_product: null data");
-  const size_t tile = (size_t)SCAN_T * SCAN_E;
-  const size_t tiles = (n + tile - 1) / tile;
-  MZK_REQUIRE(tiles <= tile, "mzk_fr_prefix_product: n above %zu", tile * tile);
-  MZK_TRY(vec_ready(c));
-  std::lock_guard<std::mutex> g(c->mu);
-  MZK_TRY(c->scratch[SCR_MSM_D].reserve(tiles * 32 + 64));
-  uint32_t* totals = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
-  MZK_LAUNCH(c, "k_prod_tiles", k_prod_tiles, dim3((unsigned)tiles), dim3(SCAN_T), 0, (const uint32_t*)a,
-             (uint32_t*)out, n, totals);
-  MZK_LAUNCH(c, "k_prod_totals", k_prod_totals, dim3(1), dim3(SCAN_T), 0, totals, tiles);
-  MZK_LAUNCH(c, "k_prod_apply", k_prod_apply, dim3(grid_for(n, 256)), dim3(256), 0, (uint32_t*)out, n, totals);
-  MZK_HIP(hipGetLastError());
-  return MZK_OK;
+  mzk_prefix_job j;
+  j.d_a = a;
+  j.d_out = out;
+  j.n = n;
+  return mzk_fr_prefix_product_batch(c, &j, 1);
 }
 
 int mzk_fr_eval(mzk_ctx* c, const void* a, size_t n, const uint64_t x[4], uint64_t out[4]) {
@@ -763,10 +850,6 @@ static int kate_launch(mzk_ctx* c, const mzk_kate_job* jobs, size_t nj) {
   return MZK_OK;
 }
 
-static bool spans_overlap(const void* p, size_t pb, const void* q, size_t qb) {
-  const uintptr_t a = (uintptr_t)p, b = (uintptr_t)q;
-  return a < b + qb && b < a + pb;
-}
 
 int mzk_fr_kate_div_batch(mzk_ctx* c, const mzk_kate_job* jobs, size_t njobs) {
   MZK_REQUIRE(c != nullptr && (jobs != nullptr || njobs == 0), "mzk_fr_kate_div_batch: null argument");

@@ -408,12 +408,19 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
     PV_TRY(mzk_plonk_perm_products_dev(c, key->omega_powers, n, sets.data(), J));
     PV_TRY(mzk_fr_batch_invert(c, den_all, J * n));
     PV_TRY(mzk_fr_vec_op(c, 2, num_all, den_all, num_all, J * n));
+    // z_0 = 1, z_i = z_(i-1) mv_(i-1): every set's running products in one batch
+    std::vector<mzk_prefix_job> pj(J);
+    for (size_t js = 0; js < J; js++) {
+      pj[js].d_a = num_all + 8 * js * n;
+      pj[js].d_out = B[js / NSETS].z[js % NSETS] + 8;
+      pj[js].n = n - 1;
+    }
+    PV_TRY(mzk_fr_prefix_product_batch(c, pj.data(), J));
     for (size_t p = 0; p < np; p++)
       for (int s = 0; s < NSETS; s++) {
         const size_t js = p * NSETS + s;
         uint32_t* z = B[p].z[s];
-        // z_0 = 1, z_i = z_(i-1) mv_(i-1); blinding rows from the draws
-        PV_TRY(mzk_fr_prefix_product(c, num_all + 8 * js * n, z + 8, n - 1));
+        // blinding rows from the draws
         const uint32_t* blind = B[p].draws + 8 * (size_t)(NADV * (BF + 1) + s * BF);
         std::lock_guard<std::mutex> g(c->mu);
         ElemArg one_w;

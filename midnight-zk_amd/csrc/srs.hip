@@ -418,6 +418,8 @@ int mzk_msm_srs_dev(mzk_ctx* c, const mzk_srs* s, size_t offset, const void* d_c
                    out);
   }
   MsmPlan p = msm_plan(n, 1, c->num_cus, 0);
+  if (s->shifted)   // window 0 of the shifted tables is the bases themselves, as records
+    return msm_run(c, p, (const uint32_t*)d_coeffs, n, (const uint32_t*)s->shifted, 0, (uint32_t)offset, true, out);
   return msm_run(c, p, (const uint32_t*)d_coeffs, n, (const uint32_t*)s->bases + 24 * offset, 0, 0, false, out);
 }
 
@@ -448,6 +450,8 @@ int mzk_msm_srs_batch_dev(mzk_ctx* c, const mzk_srs* s, const mzk_msm_job* jobs,
   }
   MsmPlan p = msm_plan(total / njobs + 1, 1, c->num_cus, 0);
   MsmPlan pt = msm_plan(total, 1, c->num_cus, p.c);   // K from the batch's total digits
+  if (s->shifted)   // window 0 of the shifted tables: the bases as records
+    return msm_run_jobs(c, pt, v, (const uint32_t*)s->shifted, 0, outs, true);
   return msm_run_jobs(c, pt, v, (const uint32_t*)s->bases, 0, outs);
 }
 

@@ -82,6 +82,11 @@ class PermSet(ctypes.Structure):
                 ("den", ctypes.c_void_p)]
 
 
+class PrefixJob(ctypes.Structure):
+    """mzk_prefix_job: d_out = inclusive prefix products of d_a (n elements)."""
+    _fields_ = [("d_a", ctypes.c_void_p), ("d_out", ctypes.c_void_p), ("n", ctypes.c_size_t)]
+
+
 class KateJob(ctypes.Structure):
     """mzk_kate_job: d_q = quotient of d_a (n coefficients) by (X - b)."""
     _fields_ = [("d_a", ctypes.c_void_p), ("n", ctypes.c_size_t), ("b", ctypes.c_uint64 * 4),
@@ -172,6 +177,7 @@ SIGNATURES = {
     "mzk_fr_lincomb_dev": ([_P, _P, _P, _P, _S, _P, _S], _I),
     "mzk_fr_kate_div": ([_P, _P, _S, _P, _P], _I),
     "mzk_fr_kate_div_batch": ([_P, _P, _S], _I),
+    "mzk_fr_prefix_product_batch": ([_P, _P, _S], _I),
     "mzk_fr_powers": ([_P, _P, _S, _P], _I),
 }
 

@@ -75,6 +75,26 @@ def test_prefix_product(be, n):
     assert np.array_equal(got, C.fr_prefix_product(a))
 
 
+def test_prefix_product_batch_equals_per_call(be):
+    L = mzk.load_library()
+    sizes = [1, 2, 4095, 4096, 4097, 0, 3 * 4096 + 5, 300, (1 << 17) + 3]
+    ins = [to_dev(C.fr_from_seed(60 + i, n, n)) if n else None for i, n in enumerate(sizes)]
+    outs = [be._empty(n) for n in sizes]
+    jobs = (mzk.PrefixJob * len(sizes))()
+    for j, (a, o, n) in enumerate(zip(ins, outs, sizes)):
+        jobs[j].d_a = a.data_ptr() if n else None
+        jobs[j].d_out = o.data_ptr() if n else None
+        jobs[j].n = n
+    print(f"\n[prefix batch] {len(sizes)} jobs", flush=True)
+    assert L.mzk_fr_prefix_product_batch(be.ctx.h, ctypes.addressof(jobs), len(sizes)) == 0
+    for a, o, n in zip(ins, outs, sizes):
+        if n:
+            assert np.array_equal(host(be, o), C.fr_prefix_product(host(be, a)))
+    # an output inside another job's input is refused before anything runs
+    jobs[1].d_out = ins[2].data_ptr()
+    assert L.mzk_fr_prefix_product_batch(be.ctx.h, ctypes.addressof(jobs), len(sizes)) == 1
+
+
 def test_prefix_product_integers(be):
     vals = [2, 3, 5, 7, 11, 0, 13]
     got = C.fr_rows_to_ints(host(be, be.prefix_product(to_dev(C.ints_to_fr_rows(vals)))))
