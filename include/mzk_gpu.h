@@ -320,6 +320,16 @@ int mzk_fr_from_seed_dev(mzk_ctx *ctx, uint64_t seed, uint64_t stream, uint64_t 
  * "<name> <total ms> <launches>\n", then clears the totals.  Enabling or disabling
  * also clears them.  Timing adds event overhead: leave it off for timed runs. */
 int mzk_prof_enable(mzk_ctx *ctx, int on);
+/* ------------------------------------------------------------------ launch graphs
+ * No reference counterpart (the reference's CPU path has no launches).  The fixed
+ * device launch sequences of a call -- an MSM's sort, accumulation and reduction
+ * kernels, a batch of NTTs' two passes -- run as HIP graphs, one per distinct shape and
+ * buffer set, captured on first use and replayed afterwards (a prover's rounds repeat
+ * the same shapes on the same buffers proof after proof).  On by default; off replays
+ * nothing and frees the graphs.  Results are identical either way; the kernel profiler
+ * bypasses graphs while it is on.  out = {graphs captured, replays, graphs held}. */
+int mzk_ctx_set_graphs(mzk_ctx *ctx, int on);
+int mzk_ctx_graph_stats(mzk_ctx *ctx, uint64_t out[3]);
 int mzk_prof_read(mzk_ctx *ctx, char *buf, size_t len);
 /* Plan of the last MSM run on the context: out = {window bits c, windows W, bucket
  * sets, references per accumulation thread K, non-zero window digits M, n}. */

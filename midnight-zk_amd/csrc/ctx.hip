@@ -73,6 +73,53 @@ int KernelProfiler::collect() {
   return MZK_OK;
 }
 
+// ---------------------------------------------------------------- launch graphs
+int run_graph(mzk_ctx* c, const std::string& key, const std::function<int()>& launch) {
+  GraphCache& G = c->graphs;
+  if (!G.on || c->prof.on) return launch();
+  auto it = G.map.find(key);
+  if (it == G.map.end()) {
+    if (G.map.size() >= GraphCache::MAX_GRAPHS) {
+      // drop the least recently used graph; wait first, it may still be running
+      auto old = G.map.begin();
+      for (auto e = G.map.begin(); e != G.map.end(); ++e)
+        if (e->second.last_use < old->second.last_use) old = e;
+      MZK_HIP(hipStreamSynchronize(c->stream));
+      (void)hipGraphExecDestroy(old->second.exec);
+      G.map.erase(old);
+    }
+    MZK_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = launch();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc != MZK_OK || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      if (rc != MZK_OK) return rc;
+      set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+      return MZK_ERR_HIP;
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      set_error("hipGraphInstantiate: %s", hipGetErrorString(ei));
+      return MZK_ERR_HIP;
+    }
+    it = G.map.emplace(key, GraphCache::Entry{ex, 0}).first;
+    G.captures++;
+  } else {
+    G.replays++;
+  }
+  it->second.last_use = ++G.tick;
+  MZK_HIP(hipGraphLaunch(it->second.exec, c->stream));
+  return MZK_OK;
+}
+
+void free_graphs(mzk_ctx* c) {
+  for (auto& kv : c->graphs.map) (void)hipGraphExecDestroy(kv.second.exec);
+  c->graphs.map.clear();
+}
+
 // ---------------------------------------------------------------- seeded stream
 __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
   x += 0x9E3779B97F4A7C15ull;
@@ -154,6 +201,7 @@ void mzk_ctx_destroy(mzk_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   free_ntt_plans(c);
+  free_graphs(c);
   for (auto& s : c->scratch) s.release();
   if (c->host_staging) (void)hipHostFree(c->host_staging);
   (void)hipStreamDestroy(c->stream);
@@ -176,6 +224,27 @@ int mzk_prof_enable(mzk_ctx* c, int on) {
   MZK_TRY(c->prof.collect());
   c->prof.totals.clear();
   c->prof.on = on != 0;
+  return MZK_OK;
+}
+
+int mzk_ctx_set_graphs(mzk_ctx* c, int on) {
+  MZK_REQUIRE(c != nullptr, "null context");
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  c->graphs.on = on != 0;
+  if (!c->graphs.on) {
+    MZK_HIP(hipStreamSynchronize(c->stream));
+    free_graphs(c);
+  }
+  return MZK_OK;
+}
+
+int mzk_ctx_graph_stats(mzk_ctx* c, uint64_t out[3]) {
+  MZK_REQUIRE(c != nullptr && out != nullptr, "mzk_ctx_graph_stats: null argument");
+  std::lock_guard<std::mutex> g(c->mu);
+  out[0] = c->graphs.captures;
+  out[1] = c->graphs.replays;
+  out[2] = c->graphs.map.size();
   return MZK_OK;
 }
 

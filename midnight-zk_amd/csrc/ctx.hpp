@@ -1,5 +1,6 @@
 // The context object behind the opaque mzk_ctx handle.
 #pragma once
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -56,11 +57,42 @@ struct KernelProfiler {
 // Wall-clock milliseconds since an arbitrary origin (host-phase timing).
 double host_ms();
 
+// Executable graphs of launch sequences, keyed by everything the sequence's launches
+// depend on (kernel, grid, block, LDS bytes and every argument, serialised): replaying
+// a graph whose key matches is the same work as launching the sequence again.
+struct GraphCache {
+  static constexpr size_t MAX_GRAPHS = 48;
+  struct Entry {
+    hipGraphExec_t exec;
+    uint64_t last_use;
+  };
+  std::map<std::string, Entry> map;
+  uint64_t tick = 0;
+  uint64_t captures = 0, replays = 0;
+  bool on = true;
+};
+
+// Byte string of plain values and pointers, for graph keys.
+struct GraphKey {
+  std::string s;
+  template <class T>
+  GraphKey& add(const T& v) {
+    s.append(reinterpret_cast<const char*>(&v), sizeof v);
+    return *this;
+  }
+};
+
 }  // namespace mzk
 struct mzk_ctx;
 namespace mzk {
 // Frees the context's cached NTT plans (ntt.hip).
 void free_ntt_plans(mzk_ctx* c);
+// Runs `launch` (kernel launches on the context stream and nothing else: no waits, no
+// copies, no allocations) through the context's graph cache: a cached graph under `key`
+// is replayed, otherwise the launches are captured into one, instantiated and launched.
+// With graphs off or the kernel profiler on, `launch` simply runs.  Caller holds c->mu.
+int run_graph(mzk_ctx* c, const std::string& key, const std::function<int()>& launch);
+void free_graphs(mzk_ctx* c);
 
 }  // namespace mzk
 
@@ -84,4 +116,5 @@ struct mzk_ctx {
   mzk::KernelProfiler prof;
   uint64_t last_msm[6] = {0, 0, 0, 0, 0, 0};   // c, W, sets, K, M, n of the last MSM
   std::map<int, mzk::NttPlan*> ntt_plans;        // (log_n, inverse) -> resident twiddle tables
+  mzk::GraphCache graphs;
 };

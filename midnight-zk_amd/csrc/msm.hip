@@ -747,6 +747,7 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
     canon_off += jobs[j].n;
   }
   SortArgs sa;
+  memset(&sa, 0, sizeof sa);   // padding too: the struct is part of the graph key
   sa.G = (uint32_t)td.size();
   sa.c = p.c;
   sa.W = p.W;
@@ -818,27 +819,47 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   uint32_t* chan_parts = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
   uint32_t* chan = chan_parts + nchan * parts * MZK_X28_WORDS;
 
-  MZK_LAUNCH(c, "k_sort_hist", k_sort_hist, dim3(sa.G), dim3(SORT_BLOCK), sa.D1 * 4, sa, canon, hist);
-  MZK_LAUNCH(c, "k_scan_tiles", k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, hist, hoff, HN, tsums);
-  MZK_LAUNCH(c, "k_scan_sums", k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, tsums, (int)tiles, hoff, HN);
-  MZK_LAUNCH(c, "k_scan_add", k_scan_add, dim3(grid_for(HN, 256)), dim3(256), 0, hoff, HN, tsums);
-  MZK_LAUNCH(c, "k_sort_scatter", k_sort_scatter, dim3(sa.G), dim3(SORT_BLOCK),
-             (3 * sa.D1 + 3 * SCAT_SLOTS + SORT_BLOCK) * 4, canon, sa, hoff, tmp_key, tmp_ref);
-  MZK_LAUNCH(c, "k_sort_bins", k_sort_bins, dim3(sa.D1), dim3(SORT_BLOCK), (1u << sa.l2) * 4, hoff, sa, NB, tmp_key,
-             tmp_ref, refs, off);
-  if (records)
-    MZK_LAUNCH(c, "k_msm_accum", k_msm_accum<true>, dim3(grid_for(nchunks, 64)), dim3(64), 0, refs, off, NB, K,
-               d_bases, partials);
-  else
-    MZK_LAUNCH(c, "k_msm_accum", k_msm_accum<false>, dim3(grid_for(nchunks, 64)), dim3(64), 0, refs, off, NB, K,
-               d_bases, partials);
-  if (small_sets) {
-    // per-set weighted sums in LDS, then the window combination on the host
-    uint32_t* tot = chan;
-    const uint32_t spb = 256 / p.B;
-    MZK_LAUNCH(c, "k_set_reduce", k_set_reduce, dim3((unsigned)((sets + spb - 1) / spb)), dim3(256),
-               256 * MZK_X28_WORDS * 4, off, NB, K, p.B, (uint32_t)sets, partials, tot);
+  // the device sequence (sort, accumulation, reduction) as one cached graph: every value
+  // its launches take is in the key; the descriptors above are copied before it runs
+  GraphKey gk;
+  gk.add('M').add(sa).add(canon).add(hist).add(hoff).add(HN).add(tsums).add(tiles).add(tmp_key).add(tmp_ref);
+  gk.add(refs).add(off).add(NB).add(K).add(nchunks).add(records).add(d_bases).add(partials).add(small_sets);
+  gk.add(p.B).add(sets).add(chan).add(segA).add(segR).add(L).add(nseg).add(T).add(nch).add(parts).add(chan_parts);
+  gk.add(nchan);
+  MZK_TRY(run_graph(c, gk.s, [&]() -> int {
+    MZK_LAUNCH(c, "k_sort_hist", k_sort_hist, dim3(sa.G), dim3(SORT_BLOCK), sa.D1 * 4, sa, canon, hist);
+    MZK_LAUNCH(c, "k_scan_tiles", k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, hist, hoff, HN, tsums);
+    MZK_LAUNCH(c, "k_scan_sums", k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, tsums, (int)tiles, hoff, HN);
+    MZK_LAUNCH(c, "k_scan_add", k_scan_add, dim3(grid_for(HN, 256)), dim3(256), 0, hoff, HN, tsums);
+    MZK_LAUNCH(c, "k_sort_scatter", k_sort_scatter, dim3(sa.G), dim3(SORT_BLOCK),
+               (3 * sa.D1 + 3 * SCAT_SLOTS + SORT_BLOCK) * 4, canon, sa, hoff, tmp_key, tmp_ref);
+    MZK_LAUNCH(c, "k_sort_bins", k_sort_bins, dim3(sa.D1), dim3(SORT_BLOCK), (1u << sa.l2) * 4, hoff, sa, NB,
+               tmp_key, tmp_ref, refs, off);
+    if (records)
+      MZK_LAUNCH(c, "k_msm_accum", k_msm_accum<true>, dim3(grid_for(nchunks, 64)), dim3(64), 0, refs, off, NB, K,
+                 d_bases, partials);
+    else
+      MZK_LAUNCH(c, "k_msm_accum", k_msm_accum<false>, dim3(grid_for(nchunks, 64)), dim3(64), 0, refs, off, NB, K,
+                 d_bases, partials);
+    if (small_sets) {
+      // per-set weighted sums in LDS (one per set, into chan)
+      const uint32_t spb = 256 / p.B;
+      MZK_LAUNCH(c, "k_set_reduce", k_set_reduce, dim3((unsigned)((sets + spb - 1) / spb)), dim3(256),
+                 256 * MZK_X28_WORDS * 4, off, NB, K, p.B, (uint32_t)sets, partials, chan);
+    } else {
+      MZK_LAUNCH(c, "k_bucket_seg", k_bucket_seg, dim3(grid_for(nseg, 64)), dim3(64), 0, off, NB, K, L, partials,
+                 segA, segR);
+      MZK_LAUNCH(c, "k_chan_sum", k_chan_sum, dim3((unsigned)(nchan * parts)), dim3(CHAN_BLOCK),
+                 CHAN_BLOCK * MZK_X28_WORDS * 4, segA, segR, T, nch, parts, chan_parts);
+      MZK_LAUNCH(c, "k_seg_sum", k_seg_sum, dim3((unsigned)nchan), dim3(64), 64 * MZK_X28_WORDS * 4, chan_parts,
+                 parts, chan);
+    }
     MZK_HIP(hipGetLastError());
+    return MZK_OK;
+  }));
+  if (small_sets) {
+    // the window combination on the host
+    uint32_t* tot = chan;
     std::vector<uint32_t> h(sets * 48);
     uint32_t M = 0;
     double h0 = host_ms();
@@ -860,13 +881,6 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
     c->prof.host("host_horner", host_ms() - h1);
     return MZK_OK;
   }
-  MZK_LAUNCH(c, "k_bucket_seg", k_bucket_seg, dim3(grid_for(nseg, 64)), dim3(64), 0, off, NB, K, L, partials, segA,
-             segR);
-  MZK_LAUNCH(c, "k_chan_sum", k_chan_sum, dim3((unsigned)(nchan * parts)), dim3(CHAN_BLOCK), CHAN_BLOCK * MZK_X28_WORDS * 4, segA,
-             segR, T, nch, parts, chan_parts);
-  MZK_LAUNCH(c, "k_seg_sum", k_seg_sum, dim3((unsigned)nchan), dim3(64), 64 * MZK_X28_WORDS * 4, chan_parts, parts, chan);
-  MZK_HIP(hipGetLastError());
-
   std::vector<uint32_t> h(nchan * 48);
   uint32_t M = 0;
   double h0 = host_ms();

@@ -504,14 +504,16 @@ int ntt_run_batch(mzk_ctx* c, const mzk_ntt_job* jobs, size_t njobs) {
   MZK_HIP(hipMemcpyAsync(d, h.data(), o, hipMemcpyHostToDevice, c->stream));
   const size_t lds = (size_t)NTT_LDS_ELEMS * ELEM29 * 4;
   MZK_TRY(ntt_lds_attr());
-  if (!t1.empty())
-    MZK_LAUNCH(c, "k_ntt_pass_batch1", k_ntt_pass_batch, dim3(f1.back()), dim3(256), lds, d_t1, d_f1,
-               (uint32_t)t1.size());
-  if (!t2.empty())
-    MZK_LAUNCH(c, "k_ntt_pass_batch2", k_ntt_pass_batch, dim3(f2.back()), dim3(256), lds, d_t2, d_f2,
-               (uint32_t)t2.size());
-  MZK_HIP(hipGetLastError());
-  return MZK_OK;
+  // the two launches as one cached graph (the pass tables are copied before it runs)
+  const uint32_t n1 = (uint32_t)t1.size(), n2 = (uint32_t)t2.size(), g1 = f1.back(), g2 = f2.back();
+  GraphKey gk;
+  gk.add('N').add(d_t1).add(d_f1).add(n1).add(g1).add(d_t2).add(d_f2).add(n2).add(g2).add(lds);
+  return run_graph(c, gk.s, [&]() -> int {
+    if (n1) MZK_LAUNCH(c, "k_ntt_pass_batch1", k_ntt_pass_batch, dim3(g1), dim3(256), lds, d_t1, d_f1, n1);
+    if (n2) MZK_LAUNCH(c, "k_ntt_pass_batch2", k_ntt_pass_batch, dim3(g2), dim3(256), lds, d_t2, d_f2, n2);
+    MZK_HIP(hipGetLastError());
+    return MZK_OK;
+  });
 }
 
 }  // namespace mzk

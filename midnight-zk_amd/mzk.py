@@ -153,6 +153,8 @@ SIGNATURES = {
     "mzk_diag_g1_op": ([_P, _I, _P, _P, _P, _S], _I),
     "mzk_diag_bench": ([_P, _I, _I, _I, _I, _D, _D], _I),
     "mzk_prof_enable": ([_P, _I], _I),
+    "mzk_ctx_set_graphs": ([_P, _I], _I),
+    "mzk_ctx_graph_stats": ([_P, _P], _I),
     "mzk_prof_read": ([_P, ctypes.c_char_p, _S], _I),
     "mzk_msm_last_plan": ([_P, _P], _I),
     "mzk_srs_from_scalars_dev": ([_P, _P, _S, ctypes.POINTER(_P)], _I),
@@ -257,6 +259,15 @@ class Context:
 
     def synchronize(self):
         _check(load_library().mzk_ctx_synchronize(self.h))
+
+    def set_graphs(self, on: bool = True):
+        """Replay cached HIP graphs of the MSM / NTT launch sequences (default on)."""
+        _check(load_library().mzk_ctx_set_graphs(self.h, 1 if on else 0))
+
+    def graph_stats(self) -> dict:
+        out = np.zeros(3, dtype=np.uint64)
+        _check(load_library().mzk_ctx_graph_stats(self.h, out.ctypes.data))
+        return {"captured": int(out[0]), "replayed": int(out[1]), "held": int(out[2])}
 
     def profile(self, on: bool = True):
         """Per-kernel HIP-event timing on this context's stream (off by default)."""
