@@ -362,39 +362,52 @@ __global__ void k_scan_add(uint32_t* __restrict__ out, size_t n, const uint32_t*
 // The point of reference r: blst affine (24 words, converted here) for caller bases, or
 // a 128-byte record of a resident SRS's shifted tables (already in the radix-2^28 form,
 // identity flagged): one cache line per gathered point instead of 1.5 on average.
+// The loads are issued one reference ahead (raw words held in registers) so that each
+// gather's latency overlaps the previous mixed addition.
 template <bool REC>
-__device__ __forceinline__ bool load_point(const uint32_t* __restrict__ bases, uint32_t r, f28& x, f28& y) {
+struct RawPoint {
+  static constexpr int W = REC ? 29 : 24;
+  uint32_t w[W];
+};
+
+template <bool REC>
+__device__ __forceinline__ void issue_point(const uint32_t* __restrict__ bases, uint32_t r, RawPoint<REC>& o) {
   const size_t idx = (size_t)(r & ~NEG_FLAG);
   if (REC) {
     const uint4* q = reinterpret_cast<const uint4*>(bases + MZK_SRS_REC_WORDS * idx);
-    uint32_t w[MZK_SRS_REC_WORDS - 3];
 #pragma unroll
     for (int i = 0; i < 7; i++) {
       const uint4 a = q[i];
-      w[4 * i] = a.x; w[4 * i + 1] = a.y; w[4 * i + 2] = a.z; w[4 * i + 3] = a.w;
+      o.w[4 * i] = a.x; o.w[4 * i + 1] = a.y; o.w[4 * i + 2] = a.z; o.w[4 * i + 3] = a.w;
     }
-    w[28] = q[7].x;
+    o.w[28] = bases[MZK_SRS_REC_WORDS * idx + 28];
+  } else {
+    const uint4* q = reinterpret_cast<const uint4*>(bases + 24 * idx);
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      const uint4 a = q[i];
+      o.w[4 * i] = a.x; o.w[4 * i + 1] = a.y; o.w[4 * i + 2] = a.z; o.w[4 * i + 3] = a.w;
+    }
+  }
+}
+
+// raw words -> (x, y) in the radix-2^28 form; false for the identity (adds nothing)
+template <bool REC>
+__device__ __forceinline__ bool decode_point(const RawPoint<REC>& o, f28& x, f28& y) {
+  if (REC) {
 #pragma unroll
     for (int i = 0; i < 14; i++) {
-      x.l[i] = w[i];
-      y.l[i] = w[14 + i];
+      x.l[i] = o.w[i];
+      y.l[i] = o.w[14 + i];
     }
-    return w[28] == 0;
-  }
-  const uint4* q = reinterpret_cast<const uint4*>(bases + 24 * idx);
-  uint32_t xs[12], ys[12];
-#pragma unroll
-  for (int i = 0; i < 3; i++) {
-    const uint4 a = q[i], b = q[3 + i];
-    xs[4 * i] = a.x; xs[4 * i + 1] = a.y; xs[4 * i + 2] = a.z; xs[4 * i + 3] = a.w;
-    ys[4 * i] = b.x; ys[4 * i + 1] = b.y; ys[4 * i + 2] = b.z; ys[4 * i + 3] = b.w;
+    return o.w[28] == 0;
   }
   uint32_t any = 0;
 #pragma unroll
-  for (int i = 0; i < 12; i++) any |= xs[i] | ys[i];
-  x = f28_from_std(xs);
-  y = f28_from_std(ys);
-  return any != 0;   // the affine identity adds nothing
+  for (int i = 0; i < 24; i++) any |= o.w[i];
+  x = f28_from_std(o.w);
+  y = f28_from_std(o.w + 12);
+  return any != 0;
 }
 
 template <bool REC>
@@ -417,7 +430,17 @@ __global__ void __launch_bounds__(64) k_msm_accum(const uint32_t* __restrict__ r
   uint32_t cb = lo;
   uint32_t next = off[cb + 1];
   g1x28 acc = g1x28_inf();
+  uint32_t r_next = refs[e0];
+  RawPoint<REC> raw;
+  issue_point<REC>(bases, r_next, raw);
   for (uint32_t e = e0; e < e1; e++) {
+    const uint32_t r = r_next;
+    f28 x, y;
+    const bool live = decode_point<REC>(raw, x, y);
+    if (e + 1 < e1) {   // the next gather, in flight during this addition
+      r_next = refs[e + 1];
+      issue_point<REC>(bases, r_next, raw);
+    }
     if (e >= next) {
       store_x28_raw(partials + MZK_X28_WORDS * (size_t)(k + cb), acc);
       acc = g1x28_inf();
@@ -426,10 +449,7 @@ __global__ void __launch_bounds__(64) k_msm_accum(const uint32_t* __restrict__ r
         next = off[cb + 1];
       } while (next <= e);
     }
-    const uint32_t r = refs[e];
-    f28 x, y;
-    if (!load_point<REC>(bases, r, x, y)) continue;
-    acc = g1x28_madd(acc, x, y, (r & NEG_FLAG) != 0);
+    if (live) acc = g1x28_madd(acc, x, y, (r & NEG_FLAG) != 0);
   }
   store_x28_raw(partials + MZK_X28_WORDS * (size_t)(k + cb), acc);
 }
