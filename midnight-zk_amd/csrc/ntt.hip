@@ -49,6 +49,7 @@ struct PassArgs {
   int do_twist;
   int in_coset;              // input index i times [1, zeta, zeta^2][i mod 3]
   int out_mode;              // 0 none, 1 times scale, 2 times scale*[1, zeta^2, zeta][k mod 3]
+  int pairs;                 // stages two at a time in registers (large transforms)
   FrWords zeta, zeta2;
   FrWords post[3];           // scale * pattern, indexed by output index mod 3
 };
@@ -64,15 +65,24 @@ __device__ __forceinline__ uint32_t bitrev(uint32_t j, uint32_t bits) {
   return bits == 0 ? 0u : (__brev(j) >> (32 - bits));
 }
 
+// 512 threads per workgroup: two 72 KiB workgroups fit a CU's LDS, so 16 waves per CU
+// (4 per SIMD) instead of 8 with 256 threads; 2^22 forward 0.78 -> 0.74 ms
+static constexpr uint32_t NTT_THREADS = 512;
+// stages two per LDS round trip once a launch holds at least 2^21 points (measured: 2^22
+// 0.75 -> 0.71 ms, seven coset 2^20 outputs 1.21 -> 1.18 ms; a lone 2^18 0.055 -> 0.069,
+// 2^20 unchanged), for transforms of at least 2^16 points
+static constexpr uint32_t NTT_PAIRS_MIN_LAUNCH_LOG = 21;
+static constexpr uint32_t NTT_PAIRS_MIN_LOG = 16;
+
 __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block, uint32_t* lds);
 
-__global__ void __launch_bounds__(256) k_ntt_pass(PassArgs a) {
+__global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass(PassArgs a) {
   extern __shared__ uint32_t lds[];   // C * L elements of ELEM29 words
   ntt_pass_body(a, blockIdx.x, lds);
 }
 
 // Many transforms in one launch: workgroups [first[i], first[i+1]) run pass table[i].
-__global__ void __launch_bounds__(256) k_ntt_pass_batch(const PassArgs* __restrict__ table,
+__global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass_batch(const PassArgs* __restrict__ table,
                                                         const uint32_t* __restrict__ first, uint32_t njobs) {
   extern __shared__ uint32_t lds[];
   uint32_t lo = 0, hi = njobs;   // first[lo] <= blockIdx.x < first[hi]
@@ -150,22 +160,79 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block,
   }
   __syncthreads();
 
-  // log_l radix-2 DIT stages
-  const uint32_t halfL = L >> 1;
-  for (uint32_t s = 0; s < a.log_l; s++) {
-    const uint32_t half = 1u << s;
+  // log_l radix-2 DIT stages.  For large launches they are taken two at a time in
+  // registers (four elements per thread per LDS round trip; an odd first stage, twiddle 1
+  // throughout, runs alone); on smaller launches the halved thread parallelism of the
+  // longer per-thread chains costs more than the round trips save, so they keep one stage
+  // per round trip.  Either way each butterfly is the radix-2 one -- t = y w (or
+  // reduce_small(y) where w = 1), x + t, x + 2r - t -- so the values are those of
+  // stage-by-stage order.
+  uint32_t s = 0;
+  if (!a.pairs) {
+    const uint32_t halfL = L >> 1;
+    for (; s < a.log_l; s++) {
+      const uint32_t half = 1u << s;
+      for (uint32_t u = threadIdx.x; u < a.C * halfL; u += nt) {
+        const uint32_t c = u / halfL, v = u % halfL;
+        const uint32_t pos = v & (half - 1);
+        const uint32_t i = ((v >> s) << (s + 1)) + pos;
+        uint32_t* pa = lds + 9 * (c * L + i);
+        uint32_t* pb = pa + 9 * half;
+        const fr29 x = load29(pa);
+        const fr29 y = load29(pb);
+        const fr29 t = pos != 0 ? fr29_mul(y, load_rec29(a.tw + TW29 * (pos << (a.log_l - 1 - s))))
+                                : fr29_reduce_small(y);
+        store29(pa, fr29_add(x, t));
+        store29(pb, fr29_sub_kr<2>(x, t));
+      }
+      __syncthreads();
+    }
+  } else if (a.log_l & 1) {
+    const uint32_t halfL = L >> 1;
     for (uint32_t u = threadIdx.x; u < a.C * halfL; u += nt) {
       const uint32_t c = u / halfL, v = u % halfL;
-      const uint32_t pos = v & (half - 1);
-      const uint32_t i = ((v >> s) << (s + 1)) + pos;
-      uint32_t* pa = lds + 9 * (c * L + i);
-      uint32_t* pb = pa + 9 * half;
+      uint32_t* pa = lds + 9 * (c * L + 2 * v);
+      uint32_t* pb = pa + 9;
       const fr29 x = load29(pa);
-      const fr29 y = load29(pb);
-      const fr29 t = pos != 0 ? fr29_mul(y, load_rec29(a.tw + TW29 * (pos << (a.log_l - 1 - s))))
-                              : fr29_reduce_small(y);
+      const fr29 t = fr29_reduce_small(load29(pb));
       store29(pa, fr29_add(x, t));
       store29(pb, fr29_sub_kr<2>(x, t));
+    }
+    __syncthreads();
+    s = 1;
+  }
+  const uint32_t quarterL = L >> 2;
+  for (; a.pairs && s + 1 < a.log_l; s += 2) {
+    const uint32_t half = 1u << s;
+    for (uint32_t u = threadIdx.x; u < a.C * quarterL; u += nt) {
+      const uint32_t c = u / quarterL, v = u % quarterL;
+      const uint32_t pos = v & (half - 1);
+      const uint32_t i = ((v >> s) << (s + 2)) + pos;
+      uint32_t* p0 = lds + 9 * (c * L + i);
+      uint32_t* p1 = p0 + 9 * half;
+      uint32_t* p2 = p1 + 9 * half;
+      uint32_t* p3 = p2 + 9 * half;
+      const fr29 x0 = load29(p0), x1 = load29(p1), x2 = load29(p2), x3 = load29(p3);
+      // stage s: (x0, x1) and (x2, x3), both with w = omega_(2 half)^pos
+      fr29 t1, t3;
+      if (pos != 0) {
+        const fr29 w = load_rec29(a.tw + TW29 * (pos << (a.log_l - 1 - s)));
+        t1 = fr29_mul(x1, w);
+        t3 = fr29_mul(x3, w);
+      } else {
+        t1 = fr29_reduce_small(x1);
+        t3 = fr29_reduce_small(x3);
+      }
+      const fr29 y0 = fr29_add(x0, t1), y1 = fr29_sub_kr<2>(x0, t1);
+      const fr29 y2 = fr29_add(x2, t3), y3 = fr29_sub_kr<2>(x2, t3);
+      // stage s + 1: (y0, y2) with omega_(4 half)^pos, (y1, y3) with omega_(4 half)^(pos + half)
+      const fr29 t2 = pos != 0 ? fr29_mul(y2, load_rec29(a.tw + TW29 * (pos << (a.log_l - 2 - s))))
+                               : fr29_reduce_small(y2);
+      const fr29 t4 = fr29_mul(y3, load_rec29(a.tw + TW29 * ((pos + half) << (a.log_l - 2 - s))));
+      store29(p0, fr29_add(y0, t2));
+      store29(p2, fr29_sub_kr<2>(y0, t2));
+      store29(p1, fr29_add(y1, t4));
+      store29(p3, fr29_sub_kr<2>(y1, t4));
     }
     __syncthreads();
   }
@@ -338,7 +405,7 @@ static uint32_t choose_c(uint32_t L, uint64_t nbatch, uint32_t groups) {
 
 static int launch_pass(mzk_ctx* c, const char* name, PassArgs a, uint64_t nbatch) {
   const uint32_t L = 1u << a.log_l;
-  const uint32_t threads = std::min<uint32_t>(256, std::max<uint32_t>(64, a.C * L / 2));
+  const uint32_t threads = std::min<uint32_t>(NTT_THREADS, std::max<uint32_t>(64, a.C * L / 2));
   const size_t lds = (size_t)a.C * L * ELEM29 * 4;
   MZK_TRY(ntt_lds_attr());
   MZK_LAUNCH(c, name, k_ntt_pass, dim3((unsigned)(nbatch / a.C)), dim3(threads), lds, a);
@@ -349,7 +416,8 @@ static int launch_pass(mzk_ctx* c, const char* name, PassArgs a, uint64_t nbatch
 // The passes of one transform: 1 (N <= 2^11, straight from src to dst) or 2 (through
 // tmp, which must hold N elements).  Returns the pass count; groups[i] = workgroups.
 static int build_passes(const NttPlan& plan, const uint32_t* src, uint32_t* dst, uint32_t* tmp, uint64_t n_real,
-                        int in_coset, int out_mode, const host::Fr& scale, PassArgs pa[2], uint64_t groups[2]) {
+                        int in_coset, int out_mode, const host::Fr& scale, uint64_t launch_elems, PassArgs pa[2],
+                        uint64_t groups[2]) {
   const uint32_t log_n = plan.log_n;
   const uint64_t N = 1ull << log_n;
   host::Fr z = host::fr_const(host::kFrZeta);
@@ -363,6 +431,7 @@ static int build_passes(const NttPlan& plan, const uint32_t* src, uint32_t* dst,
   a.post[2] = words(host::fr_mul(scale, z));
   a.n_real = n_real;
   a.in_coset = in_coset;
+  a.pairs = log_n >= NTT_PAIRS_MIN_LOG && launch_elems >= (1ull << NTT_PAIRS_MIN_LAUNCH_LOG) ? 1 : 0;
   if (log_n <= (uint32_t)NTT_MAX_LOG_L) {
     a.src = src;
     a.dst = dst;
@@ -426,7 +495,7 @@ int ntt_run(mzk_ctx* c, const NttPlan& plan, const uint32_t* src, uint32_t* dst,
   }
   PassArgs pa[2];
   uint64_t groups[2];
-  const int np = build_passes(plan, src, dst, tmp, n_real, in_coset, out_mode, scale, pa, groups);
+  const int np = build_passes(plan, src, dst, tmp, n_real, in_coset, out_mode, scale, N, pa, groups);
   static const char* names[3] = {"k_ntt_pass_single", "k_ntt_pass1", "k_ntt_pass2"};
   for (int i = 0; i < np; i++) MZK_TRY(launch_pass(c, names[np == 1 ? 0 : 1 + i], pa[i], groups[i] * pa[i].C));
   return MZK_OK;
@@ -457,9 +526,11 @@ void free_ntt_plans(mzk_ctx* c) {
 // Runs many transforms with two launches in all: every job's first pass, then
 // every two-pass job's second pass (each workgroup finds its job in a table).
 int ntt_run_batch(mzk_ctx* c, const mzk_ntt_job* jobs, size_t njobs) {
-  size_t tmp_elems = 0;
-  for (size_t j = 0; j < njobs; j++)
+  size_t tmp_elems = 0, all_elems = 0;
+  for (size_t j = 0; j < njobs; j++) {
     if (jobs[j].log_n > (uint32_t)NTT_MAX_LOG_L) tmp_elems += 1ull << jobs[j].log_n;
+    all_elems += 1ull << jobs[j].log_n;
+  }
   MZK_TRY(c->scratch[SCR_NTT].reserve(tmp_elems * 32 + 64));
   uint32_t* tmp = (uint32_t*)c->scratch[SCR_NTT].ptr;
   std::vector<PassArgs> t1, t2;
@@ -476,7 +547,7 @@ int ntt_run_batch(mzk_ctx* c, const mzk_ntt_job* jobs, size_t njobs) {
     uint64_t groups[2];
     const int np = build_passes(*plan, (const uint32_t*)jb.d_in, (uint32_t*)jb.d_out, tmp,
                                 inverse ? N : jb.n_real, inverse ? 0 : coset,
-                                inverse ? (coset ? 2 : 1) : 0, scale, pa, groups);
+                                inverse ? (coset ? 2 : 1) : 0, scale, all_elems, pa, groups);
     if (np == 2) tmp += 8 * N;
     t1.push_back(pa[0]);
     f1.push_back(f1.back() + (uint32_t)groups[0]);
@@ -509,8 +580,8 @@ int ntt_run_batch(mzk_ctx* c, const mzk_ntt_job* jobs, size_t njobs) {
   GraphKey gk;
   gk.add('N').add(d_t1).add(d_f1).add(n1).add(g1).add(d_t2).add(d_f2).add(n2).add(g2).add(lds);
   return run_graph(c, gk.s, [&]() -> int {
-    if (n1) MZK_LAUNCH(c, "k_ntt_pass_batch1", k_ntt_pass_batch, dim3(g1), dim3(256), lds, d_t1, d_f1, n1);
-    if (n2) MZK_LAUNCH(c, "k_ntt_pass_batch2", k_ntt_pass_batch, dim3(g2), dim3(256), lds, d_t2, d_f2, n2);
+    if (n1) MZK_LAUNCH(c, "k_ntt_pass_batch1", k_ntt_pass_batch, dim3(g1), dim3(NTT_THREADS), lds, d_t1, d_f1, n1);
+    if (n2) MZK_LAUNCH(c, "k_ntt_pass_batch2", k_ntt_pass_batch, dim3(g2), dim3(NTT_THREADS), lds, d_t2, d_f2, n2);
     MZK_HIP(hipGetLastError());
     return MZK_OK;
   });

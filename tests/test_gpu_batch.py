@@ -102,6 +102,25 @@ def test_ntt_batch_mixed_sizes_and_flags(ctx):
         assert np.array_equal(from_dev(dout), e)
 
 
+def test_ntt_batch_large_launch(ctx):
+    # a launch of >= 2^21 points runs its stages two per LDS round trip: even and odd
+    # sub-DFT sizes (2^20 = 2^10 x 2^10, 2^21 = 2^11 x 2^10), forward, coset, inverse
+    specs = [(20, 0, 1 << 20), (20, mzk.NTT_COSET, 1 << 18), (21, mzk.NTT_INVERSE, 1 << 21)]
+    outs, jobs, expect = [], [], []
+    for i, (log_n, flags, n_real) in enumerate(specs):
+        vals = C.fr_from_seed(300 + i, log_n, n_real)
+        din = to_dev(vals)
+        dout = empty_dev(1 << log_n, 4)
+        outs.append((din, dout))
+        jobs.append(mzk.NttJob.make(din, dout, log_n, flags, n_real))
+        expect.append(_oracle_transform(vals, log_n, flags, n_real))
+    print("\n[ntt batch 2^20, 2^20 coset, 2^21 inverse]", flush=True)
+    ctx.ntt_batch(jobs)
+    ctx.synchronize()
+    for (_, o), e in zip(outs, expect):
+        assert np.array_equal(from_dev(o), e)
+
+
 def test_ntt_in_place_two_pass(ctx):
     vals = C.fr_from_seed(9, 9, 1 << 15)
     d = to_dev(vals)
