@@ -155,7 +155,8 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_hist(SortArgs a, uint32_t* 
 //    order, then writes each bin's entries of the round as one contiguous run (one
 //    scattered 4-byte store per digit would make this kernel bound by partial-line
 //    HBM writes).
-static constexpr int SCAT_WG = 8;
+static constexpr int SCAT_WG = 16;   // 16 windows per round: 2^20 scatter 0.196 -> 0.187 ms against 8
+static constexpr uint32_t SCAT_MAX_D1 = 8192;   // LDS: (3 D1 + 3 SCAT_SLOTS + SORT_BLOCK) words <= 148.5 KiB
 static constexpr int SCAT_SLOTS = SORT_BLOCK * SCAT_WG;
 
 __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __restrict__ canon, SortArgs a,
@@ -636,6 +637,21 @@ MsmPlan msm_plan(size_t n, int per_window, int num_cus, int c_override) {
   return p;
 }
 
+// Raises k_sort_scatter's dynamic-LDS limit to what SCAT_MAX_D1 bins need.
+static int scatter_lds_attr() {
+  static std::once_flag once;
+  static int rc = MZK_OK;
+  std::call_once(once, [] {
+    const int bytes = (int)((3 * SCAT_MAX_D1 + 3 * SCAT_SLOTS + SORT_BLOCK) * 4);
+    if (hipFuncSetAttribute((const void*)k_sort_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
+        hipSuccess) {
+      set_error("msm: cannot raise the scatter's LDS limit to %d bytes", bytes);
+      rc = MZK_ERR_HIP;
+    }
+  });
+  return rc;
+}
+
 // Host workers for the per-job tails: up to 15 threads started once and kept for the
 // process (a batch's tails took ~0.1 ms more when each call started its own threads).
 // One batch runs at a time; a caller that finds the pool busy runs its tails itself.
@@ -839,6 +855,12 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   uint32_t* chan_parts = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
   uint32_t* chan = chan_parts + nchan * parts * MZK_X28_WORDS;
 
+  // the scatter stages (3 D1 + 3 SCAT_SLOTS + SORT_BLOCK) words in LDS: above the 64 KiB a
+  // kernel gets without asking once D1 passes ~1300 bins
+  MZK_REQUIRE(sa.D1 <= SCAT_MAX_D1, "msm: %u top-level sort bins exceed %u (too many bucket sets)", sa.D1,
+              SCAT_MAX_D1);
+  const size_t scat_lds = (3 * (size_t)sa.D1 + 3 * SCAT_SLOTS + SORT_BLOCK) * 4;
+  MZK_TRY(scatter_lds_attr());
   // the device sequence (sort, accumulation, reduction) as one cached graph: every value
   // its launches take is in the key; the descriptors above are copied before it runs
   GraphKey gk;
@@ -851,8 +873,8 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
     MZK_LAUNCH(c, "k_scan_tiles", k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, hist, hoff, HN, tsums);
     MZK_LAUNCH(c, "k_scan_sums", k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, tsums, (int)tiles, hoff, HN);
     MZK_LAUNCH(c, "k_scan_add", k_scan_add, dim3(grid_for(HN, 256)), dim3(256), 0, hoff, HN, tsums);
-    MZK_LAUNCH(c, "k_sort_scatter", k_sort_scatter, dim3(sa.G), dim3(SORT_BLOCK),
-               (3 * sa.D1 + 3 * SCAT_SLOTS + SORT_BLOCK) * 4, canon, sa, hoff, tmp_key, tmp_ref);
+    MZK_LAUNCH(c, "k_sort_scatter", k_sort_scatter, dim3(sa.G), dim3(SORT_BLOCK), scat_lds, canon, sa, hoff, tmp_key,
+               tmp_ref);
     MZK_LAUNCH(c, "k_sort_bins", k_sort_bins, dim3(sa.D1), dim3(SORT_BLOCK), (1u << sa.l2) * 4, hoff, sa, NB,
                tmp_key, tmp_ref, refs, off);
     if (records)
