@@ -53,13 +53,20 @@ static constexpr int CHAN_BLOCK = 256;
 // Signed window digit w of a canonical scalar (8 x u32 LE); `carry` threads the
 // recoding from window to window: digit = bits + carry, digits above 2^(c-1)
 // become digit - 2^c with a carry into the next window.
+// The limb pair is picked by selects, not by indexing: an indexed read of a register
+// array puts the scalar in scratch memory.
 __device__ __forceinline__ int window_digit(const uint32_t s[8], int w, int c, uint32_t& carry) {
   const int bit = w * c;
   uint32_t bits = 0;
   if (bit < 256) {
     const int limb = bit >> 5, sh = bit & 31;
-    uint64_t two = (uint64_t)s[limb];
-    if (limb + 1 < 8) two |= (uint64_t)s[limb + 1] << 32;
+    uint32_t lo = 0, hi = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      lo = limb == k ? s[k] : lo;
+      hi = limb + 1 == k ? s[k] : hi;
+    }
+    const uint64_t two = ((uint64_t)hi << 32) | lo;
     bits = (uint32_t)(two >> sh) & ((1u << c) - 1u);
   }
   uint32_t d = bits + carry;
