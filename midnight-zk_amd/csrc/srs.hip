@@ -10,6 +10,7 @@
 #include "ctx.hpp"
 #include "fp28.hpp"
 #include "g1.hpp"
+#include "g1_28.hpp"
 #include "host_ec.hpp"
 #include "msm.hpp"
 #include "srs.hpp"
@@ -51,37 +52,29 @@ __global__ void __launch_bounds__(64) k_fixed_base(const uint32_t* __restrict__ 
   store_xyzz(out + 48 * i, acc);
 }
 
-// a^(p-2) on the device (one dependent chain; used once per 64 points).
+// a^(p-2) on the device (one dependent chain per chunk of points): square and multiply
+// from the top bit of the constant exponent.  The exponent bits are uniform across the
+// wave (scalar branches) and no table of powers is kept, so nothing spills to scratch (a
+// 16-entry table of powers did, 784 bytes per lane).
 __device__ fp_t fp_inv_dev(const fp_t& a) {
-  // exponent p - 2, scanned from the top bit, 4-bit fixed window
   constexpr uint32_t E[12] = {0xffffaaa9u, 0xb9feffffu, 0xb153ffffu, 0x1eabfffeu, 0xf6b0f624u, 0x6730d2a0u,
                               0xf38512bfu, 0x64774b84u, 0x434bacd7u, 0x4b1ba7b6u, 0x397fe69au, 0x1a0111eau};
-  fp_t tab[16];
-  tab[0] = fp_t::one();
-  tab[1] = a;
-  for (int i = 2; i < 16; i++) tab[i] = mul(tab[i - 1], a);
-  fp_t r = fp_t::one();
-  for (int nib = 95; nib >= 0; nib--) {
-    r = sqr(sqr(sqr(sqr(r))));
-    const uint32_t d = (E[nib >> 3] >> (4 * (nib & 7))) & 15u;
-    fp_t m = tab[0];
-    for (int t = 1; t < 16; t++)
-      if ((uint32_t)t == d) m = tab[t];
-    if (d) r = mul(r, m);
+  fp_t r = a;   // bit 380, the top bit of p - 2
+  for (int i = 379; i >= 0; i--) {
+    r = sqr(r);
+    if ((E[i >> 5] >> (i & 31)) & 1u) r = mul(r, a);
   }
   return r;
 }
 
-static constexpr int NORM_CHUNK = 64;
-
-// XYZZ -> affine for n points, NORM_CHUNK per thread: with u = ZZ * ZZZ,
-// 1/ZZ = ZZZ / u and 1/ZZZ = ZZ / u; one inversion per chunk.
-__global__ void __launch_bounds__(64) k_batch_normalize(const uint32_t* __restrict__ in, size_t n,
+// XYZZ -> affine for n points, `chunk` per thread: with u = ZZ * ZZZ,
+// 1/ZZ = ZZZ / u and 1/ZZZ = ZZ / u; one inversion (~570 dependent products) per chunk.
+__global__ void __launch_bounds__(64) k_batch_normalize(const uint32_t* __restrict__ in, size_t n, uint32_t chunk,
                                                         uint32_t* __restrict__ out, uint32_t* __restrict__ prefix) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t i0 = t * NORM_CHUNK;
+  size_t i0 = t * chunk;
   if (i0 >= n) return;
-  size_t i1 = i0 + NORM_CHUNK < n ? i0 + NORM_CHUNK : n;
+  size_t i1 = i0 + chunk < n ? i0 + chunk : n;
   fp_t acc = fp_t::one();
   for (size_t i = i0; i < i1; i++) {
     store<FpP>(prefix + 12 * i, acc);
@@ -151,25 +144,81 @@ static std::vector<uint64_t> comb_table() {
   return out;
 }
 
-int batch_normalize(mzk_ctx* c, const uint32_t* d_xyzz, size_t n, uint32_t* d_affine) {
-  MZK_TRY(c->scratch[SCR_MSM_D].reserve(n * 48 + 64));
-  uint32_t* prefix = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
-  size_t threads = (n + NORM_CHUNK - 1) / NORM_CHUNK;
-  MZK_LAUNCH(c, "k_batch_normalize", k_batch_normalize, dim3(grid_for(threads, 64)), dim3(64), 0, d_xyzz, n, d_affine,
-                     prefix);
+int batch_normalize(mzk_ctx* c, const uint32_t* d_xyzz, size_t n, uint32_t* d_affine, uint32_t* prefix) {
+  if (!prefix) {
+    MZK_TRY(c->scratch[SCR_MSM_D].reserve(n * 48 + 64));
+    prefix = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
+  }
+  // chunks of 16 while that still leaves fewer than ~2^18 threads (a 2^20 normalisation on
+  // chunks of 64 kept three quarters of the SIMDs idle), 64 for the largest batches (the
+  // shifted windows of an SRS: one inversion per 64 points)
+  const uint32_t chunk = n >= ((size_t)1 << 22) ? 64 : 16;
+  size_t threads = (n + chunk - 1) / chunk;
+  MZK_LAUNCH(c, "k_batch_normalize", k_batch_normalize, dim3(grid_for(threads, 64)), dim3(64), 0, d_xyzz, n, chunk,
+             d_affine, prefix);
   MZK_HIP(hipGetLastError());
   return MZK_OK;
 }
 
-// out[i] = 2^c * in[i] (affine in, XYZZ out): one doubling from affine, c - 1 more.
+// out[i] = 2^c * in[i] (affine in, blst-layout XYZZ out): one doubling from affine, c - 1
+// more, in the radix-2^28 form of the accumulation (fp28.hpp: 73 against 54 G products/s
+// for the 32-bit form).
 __global__ void __launch_bounds__(64) k_shift_window(const uint32_t* __restrict__ in, size_t n, int c,
                                                      uint32_t* __restrict__ out) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  g1_aff_t a = load_affine(in + 24 * i);
-  g1_xyzz_t p = a.is_inf() ? g1_xyzz_t::inf() : g1_dbl_affine(a);
-  for (int k = 1; k < c; k++) p = g1_dbl(p);
-  store_xyzz(out + 48 * i, p);
+  const uint4* q = reinterpret_cast<const uint4*>(in + 24 * i);
+  uint32_t w[24];
+  uint32_t any = 0;
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const uint4 v = q[k];
+    w[4 * k] = v.x; w[4 * k + 1] = v.y; w[4 * k + 2] = v.z; w[4 * k + 3] = v.w;
+    any |= v.x | v.y | v.z | v.w;
+  }
+  g1x28 p = g1x28_inf();
+  if (any != 0) {   // the affine identity doubles to the identity
+    const g1x28 a = g1x28_from_affine(f28_from_std(w), f28_from_std(w + 12), false);
+    p = g1x28_dbl_affine(a.X, a.Y);
+    for (int k = 1; k < c; k++) p = g1x28_dbl(p);
+  }
+  uint32_t o[48];
+  g1x28_to_std(p, o);
+  uint4* qo = reinterpret_cast<uint4*>(out + 48 * i);
+#pragma unroll
+  for (int k = 0; k < 12; k++) qo[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+}
+
+// out[i] = 2^c * in[i], blst-layout XYZZ in and out (the next window from the previous
+// one, with no normalisation in between); coordinates enter the radix-2^28 form below 2p.
+__global__ void __launch_bounds__(64) k_shift_window_xyzz(const uint32_t* __restrict__ in, size_t n, int c,
+                                                          uint32_t* __restrict__ out) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint4* q = reinterpret_cast<const uint4*>(in + 48 * i);
+  uint32_t w[48];
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    const uint4 v = q[k];
+    w[4 * k] = v.x; w[4 * k + 1] = v.y; w[4 * k + 2] = v.z; w[4 * k + 3] = v.w;
+  }
+  uint32_t zz_any = 0;
+#pragma unroll
+  for (int k = 24; k < 36; k++) zz_any |= w[k];
+  g1x28 p = g1x28_inf();
+  if (zz_any != 0) {
+    const f28 one = f28_one();
+    p.X = f28_mul(f28_from_std(w), one);
+    p.Y = f28_mul(f28_from_std(w + 12), one);
+    p.ZZ = f28_mul(f28_from_std(w + 24), one);
+    p.ZZZ = f28_mul(f28_from_std(w + 36), one);
+    for (int k = 0; k < c; k++) p = g1x28_dbl(p);
+  }
+  uint32_t o[48];
+  g1x28_to_std(p, o);
+  uint4* qo = reinterpret_cast<uint4*>(out + 48 * i);
+#pragma unroll
+  for (int k = 0; k < 12; k++) qo[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
 }
 
 // Affine points (blst layout) -> 128-byte records for the bucket accumulation: x and y
@@ -216,24 +265,49 @@ int srs_build_shifted(mzk_ctx* c, mzk_srs* s, int cbits) {
     set_error("srs: hipMalloc for shifted bases failed: %s", hipGetErrorString(e));
     return MZK_ERR_NOMEM;
   }
+  // windows 1..W-1 as XYZZ, each doubled c times from the one before with no normalisation
+  // in between, then normalised a group of windows per call (one inversion per 64 points
+  // once a group holds 2^22 points); temporaries of at most ~4 GiB (XYZZ + prefixes)
+  const size_t group = std::max<size_t>(1, std::min<size_t>(W - 1, ((size_t)4 << 30) / (n * 240)));
+  void* xyz_mem = nullptr;
+  if (W > 1) {
+    e = hipMalloc(&xyz_mem, group * n * 240);
+    if (e != hipSuccess) {
+      (void)hipFree(std_tab);
+      (void)hipFree(s->shifted);
+      s->shifted = nullptr;
+      set_error("srs: hipMalloc for shifted-window temporaries failed: %s", hipGetErrorString(e));
+      return MZK_ERR_NOMEM;
+    }
+  }
   int rc = MZK_OK;
   do {
-    if ((rc = c->scratch[SCR_MSM_C].reserve(n * 192)) != MZK_OK) break;
-    uint32_t* tmp = (uint32_t*)c->scratch[SCR_MSM_C].ptr;
+    uint32_t* xyz = (uint32_t*)xyz_mem;
+    uint32_t* prefix = xyz + 48 * group * n;
     uint32_t* tab = (uint32_t*)std_tab;
     if ((e = hipMemcpyAsync(tab, s->bases, n * 96, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) break;
-    for (int w = 1; w < W && rc == MZK_OK; w++) {
-      MZK_LAUNCH(c, "k_shift_window", k_shift_window, dim3(grid_for(n, 64)), dim3(64), 0,
-                 tab + 24 * (size_t)(w - 1) * n, n, cbits, tmp);
-      rc = batch_normalize(c, tmp, n, tab + 24 * (size_t)w * n);
+    for (int w0 = 1; w0 < W && rc == MZK_OK; w0 += (int)group) {
+      const int g = std::min<int>((int)group, W - w0);
+      for (int k = 0; k < g; k++) {
+        uint32_t* dst = xyz + 48 * (size_t)k * n;
+        if (w0 == 1 && k == 0)
+          MZK_LAUNCH(c, "k_shift_window", k_shift_window, dim3(grid_for(n, 64)), dim3(64), 0, tab, n, cbits, dst);
+        else   // from the previous window: slot k - 1, or the last slot of the previous group
+          MZK_LAUNCH(c, "k_shift_window", k_shift_window_xyzz, dim3(grid_for(n, 64)), dim3(64), 0,
+                     xyz + 48 * (size_t)(k > 0 ? k - 1 : group - 1) * n, n, cbits, dst);
+      }
+      if ((e = hipGetLastError()) != hipSuccess) break;
+      rc = batch_normalize(c, xyz, (size_t)g * n, tab + 24 * (size_t)w0 * n, prefix);
     }
-    if (rc != MZK_OK) break;
+    if (rc != MZK_OK || e != hipSuccess) break;
     MZK_LAUNCH(c, "k_to_records", k_to_records, dim3(grid_for((size_t)W * n, 64)), dim3(64), 0, tab, (size_t)W * n,
                (uint32_t*)s->shifted);
     if ((e = hipGetLastError()) != hipSuccess) break;
     e = hipStreamSynchronize(c->stream);
   } while (0);
+  (void)hipStreamSynchronize(c->stream);
   (void)hipFree(std_tab);
+  if (xyz_mem) (void)hipFree(xyz_mem);
   if (rc == MZK_OK && e != hipSuccess) {
     set_error("srs: shifted bases: %s", hipGetErrorString(e));
     rc = MZK_ERR_HIP;

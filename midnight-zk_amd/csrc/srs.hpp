@@ -15,7 +15,8 @@ struct mzk_srs {
 
 namespace mzk {
 // XYZZ (n x 48 u32) -> affine (n x 24 u32) on the context stream.
-int batch_normalize(mzk_ctx* c, const uint32_t* d_xyzz, size_t n, uint32_t* d_affine);
+// XYZZ (blst layout) -> affine; prefix: n * 48 bytes of scratch, or null for the context's
+int batch_normalize(mzk_ctx* c, const uint32_t* d_xyzz, size_t n, uint32_t* d_affine, uint32_t* prefix = nullptr);
 // For the SRS builders outside srs.hip (called with the context lock held and the
 // device set): a new SRS of n bases, and its shifted tables once the bases are written.
 int srs_alloc_bases(mzk_ctx* c, size_t n, mzk_srs** out);
