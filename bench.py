@@ -300,7 +300,7 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-prove-k", type=int, default=16)
     ap.add_argument("--serve-requests", type=int, default=120)
-    ap.add_argument("--serve-rates", type=float, nargs="+", default=[20.0, 40.0])
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[20.0, 40.0, 80.0])
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run a 256 MiB elementwise copy and 3 MSM commits (for rocprofv3 --pmc runs)")
     ap.add_argument("--prove-only", action="store_true",
