@@ -102,6 +102,9 @@ struct SortArgs {
   uint32_t l2;           // low key bits sorted in k_sort_bins
   uint32_t D1;           // bins of the top key bits
   uint32_t ref_stride;   // table stride between windows (shared-set layout)
+  // > 0: key low bits and reference share one word between scatter and bins, as
+  // (low key << pack) | reference index | sign << (pack - 1); 0: two arrays
+  uint32_t pack;
 };
 
 // Bucket key of digit d of window w of a job.
@@ -225,7 +228,7 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __r
           if (!a.per_window) ref += (uint32_t)w * a.ref_stride;
           const uint32_t p = offs[key >> a.l2] + rank[tid * SCAT_WG + wl];
           skey[p] = key;
-          sref[p] = ref | (d < 0 ? NEG_FLAG : 0u);
+          sref[p] = a.pack ? (ref | (d < 0 ? 1u << (a.pack - 1) : 0u)) : (ref | (d < 0 ? NEG_FLAG : 0u));
         }
       }
       carry0 = carry;
@@ -236,8 +239,12 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __r
         const uint32_t key = skey[e];
         const uint32_t bin = key >> a.l2;
         const uint32_t pos = cur[bin] + (e - offs[bin]);
-        tmp_key[pos] = key & mask2;
-        tmp_ref[pos] = sref[e];
+        if (a.pack) {
+          tmp_key[pos] = ((key & mask2) << a.pack) | sref[e];
+        } else {
+          tmp_key[pos] = key & mask2;
+          tmp_ref[pos] = sref[e];
+        }
       }
       __syncthreads();
       for (uint32_t b = tid; b < a.D1; b += SORT_BLOCK) cur[b] += cnt[b];
@@ -259,7 +266,8 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_bins(const uint32_t* __rest
   const uint32_t hi = hoff[(size_t)(bin + 1) * a.G];   // hoff has D1 * G + 1 entries
   for (uint32_t d = threadIdx.x; d < R; d += SORT_BLOCK) c2[d] = 0;
   __syncthreads();
-  for (uint32_t e = lo + threadIdx.x; e < hi; e += SORT_BLOCK) atomicAdd(&c2[tmp_key[e]], 1u);
+  const uint32_t sh = a.pack;   // 0: keys alone in tmp_key (a shift by 0 keeps them)
+  for (uint32_t e = lo + threadIdx.x; e < hi; e += SORT_BLOCK) atomicAdd(&c2[tmp_key[e] >> sh], 1u);
   __syncthreads();
   {   // exclusive scan of the R counters from lo: per-thread ranges, then a block scan
     __shared__ uint32_t wtot[SORT_BLOCK / 64];
@@ -282,9 +290,18 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_bins(const uint32_t* __rest
     if (b < NB) off[b] = c2[d];
   }
   __syncthreads();
-  for (uint32_t e = lo + threadIdx.x; e < hi; e += SORT_BLOCK) {
-    uint32_t pos = atomicAdd(&c2[tmp_key[e]], 1u);
-    refs[pos] = tmp_ref[e];
+  if (sh) {
+    const uint32_t idx_mask = (1u << (sh - 1)) - 1u;
+    for (uint32_t e = lo + threadIdx.x; e < hi; e += SORT_BLOCK) {
+      const uint32_t v = tmp_key[e];
+      const uint32_t pos = atomicAdd(&c2[v >> sh], 1u);
+      refs[pos] = (v & idx_mask) | (((v >> (sh - 1)) & 1u) ? NEG_FLAG : 0u);
+    }
+  } else {
+    for (uint32_t e = lo + threadIdx.x; e < hi; e += SORT_BLOCK) {
+      const uint32_t pos = atomicAdd(&c2[tmp_key[e]], 1u);
+      refs[pos] = tmp_ref[e];
+    }
   }
 }
 
@@ -801,6 +818,13 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   sa.l2 = (uint32_t)std::min(11, std::max(0, nb_bits - 9));
   sa.D1 = (uint32_t)((NB + (1u << sa.l2) - 1) >> sa.l2);
   sa.ref_stride = ref_stride;
+  {   // one word per sorted digit when the low key bits, the reference and its sign fit
+    uint64_t ref_end = 0;   // references are below this
+    for (auto& j : jobs) ref_end = std::max<uint64_t>(ref_end, (uint64_t)j.ref_offset + j.n);
+    if (!p.per_window) ref_end += (uint64_t)(p.W - 1) * ref_stride;
+    const uint32_t ref_bits = (uint32_t)ceil_log2(std::max<uint64_t>(ref_end, 2));
+    sa.pack = sa.l2 + ref_bits + 1 <= 32 ? ref_bits + 1 : 0;
+  }
   const size_t HN = (size_t)sa.D1 * sa.G;   // histogram entries
   const size_t tiles = (HN + SCAN_TILE - 1) / SCAN_TILE;
   MZK_REQUIRE(tiles <= (size_t)SCAN_TILE, "msm: sort histogram too large (%zu)", HN);

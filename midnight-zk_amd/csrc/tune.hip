@@ -133,13 +133,11 @@ int mzk_tune_msm(mzk_ctx* c, const mzk_srs* s, uint32_t log_lo, uint32_t log_hi,
         t.tpc_shared = (uint16_t)tpc;
         cand.push_back(t);
       }
-    double best = 1e300;
-    MsmTune pick;
     std::vector<uint64_t> ref;
-    for (const MsmTune& t : cand) {
+    for (const MsmTune& t : cand) {   // warm-up, and the result check
       ForceMsm f(t);
       uint64_t out[18];
-      if ((rc = mzk_msm_srs_dev(c, s, 0, d, n, out)) != MZK_OK) break;   // warm-up, and the result check
+      if ((rc = mzk_msm_srs_dev(c, s, 0, d, n, out)) != MZK_OK) break;
       const std::vector<uint64_t> a = affine_of(out);
       if (ref.empty()) {
         ref = a;
@@ -148,19 +146,21 @@ int mzk_tune_msm(mzk_ctx* c, const mzk_srs* s, uint32_t log_lo, uint32_t log_hi,
         rc = MZK_ERR_INVALID;
         break;
       }
-      double tmin = 1e300;
-      for (uint32_t r = 0; r < reps && rc == MZK_OK; r++) {
-        const double t0 = now_ms();
-        rc = mzk_msm_srs_dev(c, s, 0, d, n, out);
-        tmin = std::min(tmin, now_ms() - t0);
-      }
-      if (rc != MZK_OK) break;
-      if (tmin < best) {
-        best = tmin;
-        pick = t;
-      }
     }
     if (rc != MZK_OK) break;
+    // rounds over all candidates, best of each candidate's rounds: clock changes during
+    // the tuning then weigh on every candidate alike instead of on the last ones timed
+    std::vector<double> tmin(cand.size(), 1e300);
+    for (uint32_t r = 0; r < reps && rc == MZK_OK; r++)
+      for (size_t i = 0; i < cand.size() && rc == MZK_OK; i++) {
+        ForceMsm f(cand[i]);
+        uint64_t out[18];
+        const double t0 = now_ms();
+        rc = mzk_msm_srs_dev(c, s, 0, d, n, out);
+        tmin[i] = std::min(tmin[i], now_ms() - t0);
+      }
+    if (rc != MZK_OK) break;
+    const MsmTune pick = cand[std::min_element(tmin.begin(), tmin.end()) - tmin.begin()];
     TuneTable& tt = table();
     std::lock_guard<std::mutex> g(tt.mu);
     MsmTune& e = tt.msm[L];
@@ -219,10 +219,8 @@ int mzk_tune_ntt(mzk_ctx* c, uint32_t log_lo, uint32_t log_hi, uint32_t reps) {
     job.n_real = n;
     job.log_n = L;
     job.flags = 0;
-    double best = 1e300;
-    NttTune pick;
     ref.clear();
-    for (const NttTune& t : cand) {
+    for (const NttTune& t : cand) {   // warm-up, and the result check
       ForceNtt f(t);
       if ((rc = mzk_ntt_dev(c, &job)) != MZK_OK) break;
       got.resize(n * 32);
@@ -242,20 +240,19 @@ int mzk_tune_ntt(mzk_ctx* c, uint32_t log_lo, uint32_t log_hi, uint32_t reps) {
         rc = MZK_ERR_INVALID;
         break;
       }
-      double tmin = 1e300;
-      for (uint32_t r = 0; r < reps && rc == MZK_OK; r++) {
+    }
+    if (rc != MZK_OK) break;
+    std::vector<double> tmin(cand.size(), 1e300);   // interleaved rounds, as for the MSM
+    for (uint32_t r = 0; r < reps && rc == MZK_OK; r++)
+      for (size_t i = 0; i < cand.size() && rc == MZK_OK; i++) {
+        ForceNtt f(cand[i]);
         const double t0 = now_ms();
         rc = mzk_ntt_dev(c, &job);
         if (rc == MZK_OK) rc = mzk_ctx_synchronize(c);
-        tmin = std::min(tmin, now_ms() - t0);
+        tmin[i] = std::min(tmin[i], now_ms() - t0);
       }
-      if (rc != MZK_OK) break;
-      if (tmin < best) {
-        best = tmin;
-        pick = t;
-      }
-    }
     if (rc != MZK_OK) break;
+    const NttTune pick = cand[std::min_element(tmin.begin(), tmin.end()) - tmin.begin()];
     TuneTable& tt = table();
     std::lock_guard<std::mutex> g(tt.mu);
     tt.ntt[L] = pick;

@@ -19,8 +19,10 @@ library or a GPU is missing, calls fail loudly.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
+import sys
 
 import numpy as np
 
@@ -231,6 +233,48 @@ def _t_ptr(t) -> int:
     return t.data_ptr()
 
 
+@contextlib.contextmanager
+def _order(ctx):
+    """Orders work on the context's stream with the caller's current torch stream: the
+    context stream first waits for the torch stream (the tensors' producers), and the
+    torch stream afterwards waits for the context stream (their consumers), so a call
+    orders like a torch op on the current stream.  Nothing is added when the current
+    torch stream is the context's own, or when torch has not touched the GPU."""
+    torch = sys.modules.get("torch")
+    cur = None
+    if torch is not None and torch.cuda.is_initialized():
+        c = torch.cuda.current_stream()
+        if c.cuda_stream != ctx.stream:
+            cur = c
+            ev = torch.cuda.Event()
+            ev.record(c)
+            ctx.torch_stream().wait_event(ev)
+    yield
+    if cur is not None:
+        ev = torch.cuda.Event()
+        ev.record(ctx.torch_stream())
+        cur.wait_event(ev)
+
+
+def _ordered(ctx_of):
+    """Decorator: the call runs inside _order(ctx_of(self))."""
+    def deco(fn):
+        def run(self, *a, **kw):
+            with _order(ctx_of(self)):
+                return fn(self, *a, **kw)
+        run.__name__, run.__doc__ = fn.__name__, fn.__doc__
+        return run
+    return deco
+
+
+def _self_ctx(obj):
+    return obj
+
+
+def _owner_ctx(obj):
+    return obj.ctx
+
+
 class Context:
     """One device context (stream + scratch).  Mirrors no reference type: the
     reference runs on the rayon pool; this is the device it runs on."""
@@ -256,6 +300,13 @@ class Context:
     @property
     def stream(self) -> int:
         return load_library().mzk_ctx_stream(self.h) or 0
+
+    def torch_stream(self):
+        """The context's stream as a torch stream (for ordering torch work with it)."""
+        if getattr(self, "_tstream", None) is None:
+            import torch
+            self._tstream = torch.cuda.ExternalStream(self.stream, device=self.device)
+        return self._tstream
 
     def synchronize(self):
         _check(load_library().mzk_ctx_synchronize(self.h))
@@ -309,6 +360,7 @@ class Context:
         _check(load_library().mzk_msm_specific(self.h, _np_ptr(coeffs), _np_ptr(bases), coeffs.shape[0], _np_ptr(out)))
         return out
 
+    @_ordered(_self_ctx)
     def msm_dev(self, d_coeffs, d_bases, n: int) -> np.ndarray:
         out = np.zeros(18, dtype=np.uint64)
         _check(load_library().mzk_msm_dev(self.h, _t_ptr(d_coeffs), _t_ptr(d_bases), n, _np_ptr(out)))
@@ -323,9 +375,11 @@ class Context:
         _check(load_library().mzk_best_fft(self.h, _np_ptr(a), _np_ptr(om), log_n))
         return a
 
+    @_ordered(_self_ctx)
     def ntt(self, job: "NttJob"):
         _check(load_library().mzk_ntt_dev(self.h, ctypes.byref(job)))
 
+    @_ordered(_self_ctx)
     def ntt_batch(self, jobs):
         jobs = list(jobs)
         arr = (NttJob * max(1, len(jobs)))(*jobs)
@@ -341,6 +395,7 @@ class Context:
         return ResidentSrs.upload(self, bases)
 
     # ------------------------------------------------------------------ inputs
+    @_ordered(_self_ctx)
     def fr_from_seed_dev(self, seed: int, stream: int, n: int, out, offset: int = 0):
         _check(load_library().mzk_fr_from_seed_dev(self.h, seed, stream, offset, n, _t_ptr(out)))
         return out
@@ -371,22 +426,27 @@ class EvaluationDomain:
         except Exception:
             pass
 
+    @_ordered(_owner_ctx)
     def lagrange_to_coeff(self, d_a):
         _check(load_library().mzk_lagrange_to_coeff_dev(self.h, _t_ptr(d_a)))
         return d_a
 
+    @_ordered(_owner_ctx)
     def coeff_to_lagrange(self, d_a):
         _check(load_library().mzk_coeff_to_lagrange_dev(self.h, _t_ptr(d_a)))
         return d_a
 
+    @_ordered(_owner_ctx)
     def coeff_to_extended(self, d_in, d_out):
         _check(load_library().mzk_coeff_to_extended_dev(self.h, _t_ptr(d_in), _t_ptr(d_out)))
         return d_out
 
+    @_ordered(_owner_ctx)
     def extended_to_coeff(self, d_a):
         _check(load_library().mzk_extended_to_coeff_dev(self.h, _t_ptr(d_a)))
         return d_a
 
+    @_ordered(_owner_ctx)
     def divide_by_vanishing_poly(self, d_a):
         _check(load_library().mzk_divide_by_vanishing_poly_dev(self.h, _t_ptr(d_a)))
         return d_a
@@ -486,10 +546,12 @@ class ResidentSrs:
             arr[i].n = n
             arr[i].offset = off
         out = np.zeros((max(1, len(jobs)), 18), dtype=np.uint64)
-        _check(load_library().mzk_msm_srs_batch_dev(ctx.h, self.h, ctypes.cast(arr, ctypes.c_void_p),
-                                                     len(jobs), _np_ptr(out)))
+        with _order(ctx):
+            _check(load_library().mzk_msm_srs_batch_dev(ctx.h, self.h, ctypes.cast(arr, ctypes.c_void_p),
+                                                         len(jobs), _np_ptr(out)))
         return out[:len(jobs)]
 
+    @_ordered(_owner_ctx)
     def commit(self, d_coeffs, n: int | None = None, offset: int = 0) -> np.ndarray:
         """KZG commit of device coefficients against bases[offset:offset+n]."""
         n = d_coeffs.shape[0] if n is None else n
