@@ -2,6 +2,7 @@
 #include <stdarg.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 
 #include "ctx.hpp"
@@ -38,6 +39,30 @@ int Scratch::reserve(size_t bytes) {
 
 void Scratch::release() {
   if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+}
+
+int Pinned::reserve(size_t bytes) {
+  if (bytes <= cap) return MZK_OK;
+  if (ptr) {
+    MZK_HIP(hipHostFree(ptr));
+    ptr = nullptr;
+    cap = 0;
+  }
+  const size_t want = std::max<size_t>(bytes + bytes / 4, 1 << 16);
+  const hipError_t e = hipHostMalloc(&ptr, want, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    set_error("hipHostMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    return MZK_ERR_NOMEM;
+  }
+  cap = want;
+  return MZK_OK;
+}
+
+void Pinned::release() {
+  if (ptr) (void)hipHostFree(ptr);
   ptr = nullptr;
   cap = 0;
 }
@@ -203,7 +228,7 @@ void mzk_ctx_destroy(mzk_ctx* c) {
   free_ntt_plans(c);
   free_graphs(c);
   for (auto& s : c->scratch) s.release();
-  if (c->host_staging) (void)hipHostFree(c->host_staging);
+  c->pinned.release();
   (void)hipStreamDestroy(c->stream);
   delete c;
 }

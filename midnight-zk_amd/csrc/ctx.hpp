@@ -18,6 +18,16 @@ struct Scratch {
   void release();
 };
 
+// Pinned host memory reused across calls (hipHostMalloc): copies from and to it run
+// asynchronously on the stream instead of through the runtime's pageable staging.  A user
+// must wait for its copies before the next call may reuse the buffer.
+struct Pinned {
+  void* ptr = nullptr;
+  size_t cap = 0;
+  int reserve(size_t bytes);
+  void release();
+};
+
 enum ScratchSlot {
   SCR_NTT = 0,      // NTT ping-pong buffer
   SCR_MSM_A,        // MSM digits / sort keys
@@ -112,7 +122,7 @@ struct mzk_ctx {
   int num_cus = 0;
   size_t lds_per_block = 0;
   mzk::Scratch scratch[mzk::SCR_COUNT];
-  void* host_staging = nullptr;
+  mzk::Pinned pinned;   // MSM descriptors in, channel sums out (the MSM waits before returning)
   mzk::KernelProfiler prof;
   uint64_t last_msm[6] = {0, 0, 0, 0, 0, 0};   // c, W, sets, K, M, n of the last MSM
   std::map<int, mzk::NttPlan*> ntt_plans;        // (log_n, inverse) -> resident twiddle tables

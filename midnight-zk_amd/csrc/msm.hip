@@ -862,8 +862,18 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   uint32_t* hoff = hist + HN;
   uint32_t* tsums = hoff + HN + 1;
   uint32_t* off = tsums + tiles;
-  MZK_HIP(hipMemcpyAsync(d_jobs, jd.data(), nj * sizeof(JobDesc), hipMemcpyHostToDevice, c->stream));
-  MZK_HIP(hipMemcpyAsync(d_tiles, td.data(), td.size() * sizeof(TileDesc), hipMemcpyHostToDevice, c->stream));
+  // descriptors through pinned memory: [jobs | pad | tiles] as laid out in scratch A, one
+  // asynchronous copy; the channel sums come back through the same buffer after it
+  const size_t tiles_at = (size_t)((char*)d_tiles - (char*)d_jobs);
+  const size_t desc_bytes = tiles_at + td.size() * sizeof(TileDesc);
+  const size_t res_at = (desc_bytes + 255) & ~(size_t)255;
+  const size_t res_words = std::max(nchan, sets) * 48 + 4;
+  MZK_TRY(c->pinned.reserve(res_at + res_words * 4));
+  char* pin = (char*)c->pinned.ptr;
+  memcpy(pin, jd.data(), nj * sizeof(JobDesc));
+  memcpy(pin + tiles_at, td.data(), td.size() * sizeof(TileDesc));
+  MZK_HIP(hipMemcpyAsync(d_jobs, pin, desc_bytes, hipMemcpyHostToDevice, c->stream));
+  uint32_t* res = (uint32_t*)(pin + res_at);
   sa.jobs = d_jobs;
   sa.tiles = d_tiles;
   // scratch B: tmp keys | tmp refs | sorted refs
@@ -933,12 +943,12 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   if (small_sets) {
     // the window combination on the host
     uint32_t* tot = chan;
-    std::vector<uint32_t> h(sets * 48);
-    uint32_t M = 0;
+    const uint32_t* h = res;
     double h0 = host_ms();
-    MZK_HIP(hipMemcpyAsync(h.data(), tot, sets * 48 * 4, hipMemcpyDeviceToHost, c->stream));
-    MZK_HIP(hipMemcpyAsync(&M, off + NB, 4, hipMemcpyDeviceToHost, c->stream));
+    MZK_HIP(hipMemcpyAsync(res, tot, sets * 48 * 4, hipMemcpyDeviceToHost, c->stream));
+    MZK_HIP(hipMemcpyAsync(res + sets * 48, off + NB, 4, hipMemcpyDeviceToHost, c->stream));
     MZK_HIP(hipStreamSynchronize(c->stream));
+    const uint32_t M = res[sets * 48];
     double h1 = host_ms();
     c->prof.host("host_wait", h1 - h0);
     c->last_msm[0] = (uint64_t)p.c;
@@ -949,17 +959,17 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
     c->last_msm[5] = ntot;
     parallel_jobs(nj, [&](size_t j) {
       if (jobs[j].n == 0) memset(outs + 18 * j, 0, 18 * 8);
-      else job_tail_sets(h.data() + j * (size_t)spj * 48, spj, p.c, outs + 18 * j);
+      else job_tail_sets(h + j * (size_t)spj * 48, spj, p.c, outs + 18 * j);
     });
     c->prof.host("host_horner", host_ms() - h1);
     return MZK_OK;
   }
-  std::vector<uint32_t> h(nchan * 48);
-  uint32_t M = 0;
+  const uint32_t* h = res;
   double h0 = host_ms();
-  MZK_HIP(hipMemcpyAsync(h.data(), chan, nchan * 48 * 4, hipMemcpyDeviceToHost, c->stream));
-  MZK_HIP(hipMemcpyAsync(&M, off + NB, 4, hipMemcpyDeviceToHost, c->stream));
+  MZK_HIP(hipMemcpyAsync(res, chan, nchan * 48 * 4, hipMemcpyDeviceToHost, c->stream));
+  MZK_HIP(hipMemcpyAsync(res + nchan * 48, off + NB, 4, hipMemcpyDeviceToHost, c->stream));
   MZK_HIP(hipStreamSynchronize(c->stream));
+  const uint32_t M = res[nchan * 48];
   double h1 = host_ms();
   c->prof.host("host_wait", h1 - h0);
   c->last_msm[0] = (uint64_t)p.c;
@@ -971,7 +981,7 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   const size_t per_job = (size_t)spj * nch * 48;
   parallel_jobs(nj, [&](size_t j) {
     if (jobs[j].n == 0) memset(outs + 18 * j, 0, 18 * 8);
-    else job_tail(h.data() + j * per_job, spj, p.c, nch, lbits, tbits, outs + 18 * j);
+    else job_tail(h + j * per_job, spj, p.c, nch, lbits, tbits, outs + 18 * j);
   });
   c->prof.host("host_horner", host_ms() - h1);
   return MZK_OK;
