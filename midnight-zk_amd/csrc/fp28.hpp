@@ -272,12 +272,17 @@ MZK_HD bool f28_is_zero_mod_p(const f28& a) {
   const uint32_t kk = (lo * f28c::P0INV) & f28c::M28;
   if (kk - 1u >= (uint32_t)(KMAX - 1)) return false;
   const f28 n = f28_norm(a);
+  // unrolled: the digits of k p are then constants (a runtime k or i would index the
+  // constant tables in scratch memory)
+  bool hit = false;
+#pragma unroll
   for (int k = 1; k < KMAX; k++) {
     uint32_t diff = 0;
+#pragma unroll
     for (int i = 0; i < f28c::N; i++) diff |= n.l[i] ^ f28c::KP_DIGIT(k, i);
-    if (diff == 0) return true;
+    hit = hit || diff == 0;
   }
-  return false;
+  return hit;
 }
 
 // blst Montgomery (12 x 32-bit limbs, canonical x 2^384) -> this form (x 2^392): the

@@ -181,6 +181,103 @@ __device__ __forceinline__ g1x28 load_x28(const uint32_t* __restrict__ p) {
   return r;
 }
 
+// ---- quad-cooperative addition: the four lanes of an aligned quad hold the same P and Q
+// and compute P + Q together, each lane one of the formula's independent products per
+// step (lane q's operands chosen by selects, so the quad runs one instruction stream), the
+// products shared by DPP quad broadcasts.  The dependent chain is 4 product steps instead
+// of the 13 reductions of g1x28_add: for reductions whose waves are too few to fill the
+// SIMDs, where each addition's latency is the cost.  Same formulas and bounds as
+// g1x28_add, so the result is the same point (the same representative where the
+// products' operands coincide: Montgomery outputs depend only on the product's value).
+template <int K>   // lane K of the quad, to every lane of the quad (DPP quad_perm K,K,K,K)
+__device__ __forceinline__ uint32_t quad_bcast(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, K * 0x55, 0xF, 0xF, false);
+}
+
+template <int K>
+__device__ __forceinline__ f28 f28_quad_bcast(const f28& a) {
+  f28 r;
+#pragma unroll
+  for (int i = 0; i < f28c::N; i++) r.l[i] = quad_bcast<K>(a.l[i]);
+  return r;
+}
+
+// Lane q's operand by masks, not by ?: (the compiler turns a select of two loaded values
+// into a load from a selected address, which keeps the operands in scratch memory)
+__device__ __forceinline__ f28 f28_sel4(int q, const f28& a0, const f28& a1, const f28& a2, const f28& a3) {
+  const uint32_t m0 = 0u - (uint32_t)(q == 0), m1 = 0u - (uint32_t)(q == 1);
+  const uint32_t m2 = 0u - (uint32_t)(q == 2), m3 = 0u - (uint32_t)(q == 3);
+  f28 r;
+#pragma unroll
+  for (int i = 0; i < f28c::N; i++) r.l[i] = (a0.l[i] & m0) | (a1.l[i] & m1) | (a2.l[i] & m2) | (a3.l[i] & m3);
+  return r;
+}
+
+__device__ __forceinline__ f28 f28_sel2(bool first, const f28& a, const f28& b) {
+  const uint32_t m = 0u - (uint32_t)first;
+  f28 r;
+#pragma unroll
+  for (int i = 0; i < f28c::N; i++) r.l[i] = (a.l[i] & m) | (b.l[i] & ~m);
+  return r;
+}
+
+__device__ __forceinline__ f28 load_f28(const uint32_t* __restrict__ p) {
+  f28 r;
+#pragma unroll
+  for (int i = 0; i < f28c::N; i++) r.l[i] = p[i];
+  return r;
+}
+
+// P + Q on a quad (every lane of the quad active, holding the same P; Q read from q, a
+// point in the layout of store_x28_raw -- global memory or LDS -- one coordinate per lane
+// and step as the step needs it, so Q never occupies registers whole).  The four product
+// steps and their broadcasts run unconditionally (straight-line code around the DPP
+// moves; branching around them made the compiler pass the result through scratch
+// memory); the identity and equal-x cases then choose the result.
+__device__ __forceinline__ g1x28 g1x28_add_quad(const g1x28 p, const uint32_t* __restrict__ q) {
+  uint32_t q_any = 0;
+#pragma unroll
+  for (int i = 0; i < f28c::N; i++) q_any |= q[28 + i];
+  const bool q_inf = q_any == 0;                  // Q is the identity (ZZ literally zero)
+  const bool p_inf = g1x28_is_inf(p);
+  const int ql = (int)(threadIdx.x & 3u);
+  // step 1: U1 = X1 ZZ2, U2 = X2 ZZ1, S1 = Y1 ZZZ2, S2 = Y2 ZZZ1
+  const f28 qa = load_f28(q + (ql == 1 ? 0 : 14));      // X2 (lane 1) or Y2 (lane 3)
+  const f28 qb = load_f28(q + (ql == 0 ? 28 : 42));     // ZZ2 (lane 0) or ZZZ2 (lane 2)
+  f28 m = f28_mul(f28_sel4(ql, p.X, qa, p.Y, qa), f28_sel4(ql, qb, p.ZZ, qb, p.ZZZ));
+  const f28 U1 = f28_quad_bcast<0>(m), U2 = f28_quad_bcast<1>(m);
+  const f28 S1 = f28_quad_bcast<2>(m), S2 = f28_quad_bcast<3>(m);
+  const f28 P = f28_sub<4, 1>(U2, U1);            // < 6p
+  const f28 R = f28_sub<4, 1>(S2, S1);            // < 6p
+  // step 2: PP = P^2, R^2, ZZ1 ZZ2, ZZZ1 ZZZ2
+  const f28 qc = load_f28(q + (ql == 2 ? 28 : 42));     // ZZ2 (lane 2) or ZZZ2 (lane 3)
+  m = f28_mul(f28_sel4(ql, P, R, p.ZZ, p.ZZZ), f28_sel4(ql, P, R, qc, qc));
+  const f28 PP = f28_quad_bcast<0>(m), RR = f28_quad_bcast<1>(m);
+  const f28 ZZ12 = f28_quad_bcast<2>(m), ZZZ12 = f28_quad_bcast<3>(m);
+  // step 3: PPP = P PP, Q = U1 PP, ZZ3 = ZZ1 ZZ2 PP (lane 3 repeats lane 2)
+  m = f28_mul(f28_sel4(ql, P, U1, ZZ12, ZZ12), PP);
+  const f28 PPP = f28_quad_bcast<0>(m), Qv = f28_quad_bcast<1>(m);
+  g1x28 r;
+  r.ZZ = f28_quad_bcast<2>(m);
+  r.X = f28_norm(f28_sub<8, 3>(RR, f28_add(PPP, f28_add(Qv, Qv))));   // < 10p
+  const f28 T = f28_sub<16, 1>(Qv, r.X);                               // < 18p
+  // step 4: ZZZ3 = ZZZ1 ZZZ2 PPP (lane 0, as ZZZ12 PPP + 0 0) and
+  // Y3 = R T + S1 (4p - PPP) (lanes 1..3), one two-product reduction each
+  const f28 z = f28_zero();
+  const f28 nPPP = f28_sub<4, 1>(z, PPP);
+  const bool l0 = ql == 0;
+  m = f28_mul_sum(f28_sel2(l0, ZZZ12, R), f28_sel2(l0, PPP, T), f28_sel2(l0, z, S1), f28_sel2(l0, z, nPPP));
+  r.ZZZ = f28_quad_bcast<0>(m);
+  r.Y = f28_quad_bcast<1>(m);
+  if (f28_is_zero_mod_p<6>(P)) {   // no struct-valued ?: here: it would go through memory
+    if (f28_is_zero_mod_p<6>(R)) r = g1x28_dbl(p);
+    else r = g1x28_inf();
+  }
+  if (q_inf) r = p;
+  if (p_inf) r = load_x28(q);
+  return r;
+}
+
 __device__ __forceinline__ void store_x28_raw(uint32_t* __restrict__ p, const g1x28& a) {
   uint32_t w[MZK_X28_WORDS];
 #pragma unroll

@@ -398,7 +398,7 @@ struct RawPoint {
 template <bool REC>
 __device__ __forceinline__ void issue_point(const uint32_t* __restrict__ bases, uint32_t r, RawPoint<REC>& o) {
   const size_t idx = (size_t)(r & ~NEG_FLAG);
-  if (REC) {
+  if constexpr (REC) {
     const uint4* q = reinterpret_cast<const uint4*>(bases + MZK_SRS_REC_WORDS * idx);
 #pragma unroll
     for (int i = 0; i < 7; i++) {
@@ -419,7 +419,7 @@ __device__ __forceinline__ void issue_point(const uint32_t* __restrict__ bases, 
 // raw words -> (x, y) in the radix-2^28 form; false for the identity (adds nothing)
 template <bool REC>
 __device__ __forceinline__ bool decode_point(const RawPoint<REC>& o, f28& x, f28& y) {
-  if (REC) {
+  if constexpr (REC) {
 #pragma unroll
     for (int i = 0; i < 14; i++) {
       x.l[i] = o.w[i];
@@ -546,6 +546,58 @@ __global__ void __launch_bounds__(256) k_set_reduce(const uint32_t* __restrict__
     __syncthreads();
   }
   if (live && i == 0) {
+    uint32_t o[48];
+    g1x28_to_std(x, o);
+    uint4* qv = reinterpret_cast<uint4*>(out + 48 * (size_t)set);
+#pragma unroll
+    for (int w = 0; w < 12; w++) qv[w] = make_uint4(o[4 * w], o[4 * w + 1], o[4 * w + 2], o[4 * w + 3]);
+  }
+}
+
+// k_set_reduce with a quad of lanes per bucket (g1x28_add_quad): for batches whose
+// sets x buckets are too few to put a wave on every SIMD, where each level's addition
+// latency is the kernel's time.  128 quads per workgroup, 128 / B whole sets (B <= 128).
+static constexpr uint32_t QUAD_SETS_MAX_BUCKETS = 16384;   // sets x B at most: 1 wave per SIMD as quads
+
+__global__ void __launch_bounds__(512) k_set_reduce_quad(const uint32_t* __restrict__ off, uint32_t NB, uint32_t K,
+                                                         uint32_t B, uint32_t nsets,
+                                                         const uint32_t* __restrict__ partials,
+                                                         uint32_t* __restrict__ out) {
+  extern __shared__ uint32_t lds[];   // two buffers of 128 points: the scan reads one, writes the other
+  const uint32_t t = threadIdx.x, qd = t >> 2, ql = t & 3u;
+  const uint32_t i = qd % B;
+  const uint32_t set = blockIdx.x * (128u / B) + qd / B;
+  const bool live = set < nsets;
+  g1x28 x = g1x28_inf();
+  if (live) {
+    const uint32_t b = set * B + i;
+    const uint32_t lo = off[b], hi = off[b + 1];
+    if (hi > lo) {   // the first partial loaded, the rest added (an addition to the identity costs a full one here)
+      const uint32_t k0 = lo / K, k1 = (hi - 1) / K;
+      x = load_x28(partials + MZK_X28_WORDS * (size_t)(k0 + b));
+      for (uint32_t k = k0 + 1; k <= k1; k++) x = g1x28_add_quad(x, partials + MZK_X28_WORDS * (size_t)(k + b));
+    }
+  }
+  uint32_t* cur = lds;
+  uint32_t* nxt = lds + 128 * MZK_X28_WORDS;
+  if (ql == 0) store_x28_raw(cur + MZK_X28_WORDS * qd, x);
+  __syncthreads();
+  for (uint32_t d = 1; d < B; d <<= 1) {   // suffix sums within the set's B quads
+    if (i + d < B) x = g1x28_add_quad(x, cur + MZK_X28_WORDS * (qd + d));
+    if (ql == 0) store_x28_raw(nxt + MZK_X28_WORDS * qd, x);
+    __syncthreads();
+    uint32_t* tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+  }
+  for (uint32_t d = B / 2; d >= 1; d >>= 1) {   // tree sum of the suffix sums: reads slots >= d, writes < d
+    if (i < d) {
+      x = g1x28_add_quad(x, cur + MZK_X28_WORDS * (qd + d));
+      if (ql == 0) store_x28_raw(cur + MZK_X28_WORDS * qd, x);
+    }
+    __syncthreads();
+  }
+  if (live && i == 0 && ql == 0) {
     uint32_t o[48];
     g1x28_to_std(x, o);
     uint4* qv = reinterpret_cast<uint4*>(out + 48 * (size_t)set);
@@ -925,10 +977,17 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
       MZK_LAUNCH(c, "k_msm_accum", k_msm_accum<false>, dim3(grid_for(nchunks, 64)), dim3(64), 0, refs, off, NB, K,
                  d_bases, partials);
     if (small_sets) {
-      // per-set weighted sums in LDS (one per set, into chan)
-      const uint32_t spb = 256 / p.B;
-      MZK_LAUNCH(c, "k_set_reduce", k_set_reduce, dim3((unsigned)((sets + spb - 1) / spb)), dim3(256),
-                 256 * MZK_X28_WORDS * 4, off, NB, K, p.B, (uint32_t)sets, partials, chan);
+      // per-set weighted sums in LDS (one per set, into chan); a quad of lanes per bucket
+      // when the sets are too few to fill the SIMDs one lane per bucket
+      if (p.B <= 128 && sets * p.B <= QUAD_SETS_MAX_BUCKETS) {
+        const uint32_t spb = 128 / p.B;
+        MZK_LAUNCH(c, "k_set_reduce", k_set_reduce_quad, dim3((unsigned)((sets + spb - 1) / spb)), dim3(512),
+                   2 * 128 * MZK_X28_WORDS * 4, off, NB, K, p.B, (uint32_t)sets, partials, chan);
+      } else {
+        const uint32_t spb = 256 / p.B;
+        MZK_LAUNCH(c, "k_set_reduce", k_set_reduce, dim3((unsigned)((sets + spb - 1) / spb)), dim3(256),
+                   256 * MZK_X28_WORDS * 4, off, NB, K, p.B, (uint32_t)sets, partials, chan);
+      }
     } else {
       MZK_LAUNCH(c, "k_bucket_seg", k_bucket_seg, dim3(grid_for(nseg, 64)), dim3(64), 0, off, NB, K, L, partials,
                  segA, segR);
