@@ -606,6 +606,27 @@ __global__ void __launch_bounds__(512) k_set_reduce_quad(const uint32_t* __restr
   }
 }
 
+// k_bucket_seg for segments of one bucket (a single set): a quad per bucket merges its
+// chunk partials with quad-lane additions; A_t = R_t = S_b.  2^15 quads put two waves on
+// each SIMD where one lane per bucket left half of them idle.
+__global__ void __launch_bounds__(256) k_bucket_merge_quad(const uint32_t* __restrict__ off, uint32_t NB, uint32_t K,
+                                                           const uint32_t* __restrict__ partials,
+                                                           uint32_t* __restrict__ A, uint32_t* __restrict__ R) {
+  const uint32_t b = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
+  if (b >= NB) return;   // whole quads leave together (NB quads, 4 lanes each)
+  const uint32_t lo = off[b], hi = off[b + 1];
+  g1x28 x = g1x28_inf();
+  if (hi > lo) {
+    const uint32_t k0 = lo / K, k1 = (hi - 1) / K;
+    x = load_x28(partials + MZK_X28_WORDS * (size_t)(k0 + b));
+    for (uint32_t k = k0 + 1; k <= k1; k++) x = g1x28_add_quad(x, partials + MZK_X28_WORDS * (size_t)(k + b));
+  }
+  if ((threadIdx.x & 3u) == 0) {
+    store_x28_raw(A + MZK_X28_WORDS * (size_t)b, x);
+    store_x28_raw(R + MZK_X28_WORDS * (size_t)b, x);
+  }
+}
+
 // Block-wide sum of one point per thread (LDS tree); thread 0 returns it.
 __device__ __forceinline__ g1x28 block_sum(g1x28 x, uint32_t* lds) {
   const uint32_t t = threadIdx.x;
@@ -989,8 +1010,12 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
                    256 * MZK_X28_WORDS * 4, off, NB, K, p.B, (uint32_t)sets, partials, chan);
       }
     } else {
-      MZK_LAUNCH(c, "k_bucket_seg", k_bucket_seg, dim3(grid_for(nseg, 64)), dim3(64), 0, off, NB, K, L, partials,
-                 segA, segR);
+      if (L == 1)   // a single set: a quad per bucket
+        MZK_LAUNCH(c, "k_bucket_seg", k_bucket_merge_quad, dim3(grid_for((size_t)NB * 4, 256)), dim3(256), 0, off, NB,
+                   K, partials, segA, segR);
+      else
+        MZK_LAUNCH(c, "k_bucket_seg", k_bucket_seg, dim3(grid_for(nseg, 64)), dim3(64), 0, off, NB, K, L, partials,
+                   segA, segR);
       MZK_LAUNCH(c, "k_chan_sum", k_chan_sum, dim3((unsigned)(nchan * parts)), dim3(CHAN_BLOCK),
                  CHAN_BLOCK * MZK_X28_WORDS * 4, segA, segR, T, nch, parts, chan_parts);
       MZK_LAUNCH(c, "k_seg_sum", k_seg_sum, dim3((unsigned)nchan), dim3(64), 64 * MZK_X28_WORDS * 4, chan_parts,
