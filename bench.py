@@ -300,7 +300,10 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-prove-k", type=int, default=16)
     ap.add_argument("--serve-requests", type=int, default=120)
-    ap.add_argument("--serve-rates", type=float, nargs="+", default=[20.0, 40.0, 80.0])
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[20.0, 40.0, 80.0, 120.0])
+    ap.add_argument("--serve-workers", type=int, default=4, help="server contexts (streams)")
+    ap.add_argument("--serve-slots", type=int, default=2, help="proofs in flight per server context")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving runs (for sweeps of its settings)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run a 256 MiB elementwise copy and 3 MSM commits (for rocprofv3 --pmc runs)")
     ap.add_argument("--prove-only", action="store_true",
@@ -331,6 +334,11 @@ def main() -> int:
     if args.prove_only:
         r = bench_prove(pkg, ctx, 18, reps=3)
         print(json.dumps({k: v for k, v in r.items() if not k.startswith("_")}), flush=True)
+        return 0
+    if args.serve_only:
+        r = bench_serve(pkg, ctx, args.serve_requests, args.serve_rates, workers=args.serve_workers,
+                        slots=args.serve_slots)
+        print(json.dumps({"serve_only": True, **r}), flush=True)
         return 0
 
     # ---- resident inputs: tau, SRS bases on the device, scalars on the device
@@ -455,7 +463,8 @@ def main() -> int:
 
     # ---- serving: Poisson arrivals of mixed-size proofs, continuous batching vs one at a time
     try:
-        served = bench_serve(pkg, ctx, args.serve_requests, args.serve_rates)
+        served = bench_serve(pkg, ctx, args.serve_requests, args.serve_rates, workers=args.serve_workers,
+                             slots=args.serve_slots)
     except Exception as e:
         served = {"error": repr(e)}
 
