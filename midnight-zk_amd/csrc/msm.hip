@@ -28,6 +28,8 @@
 //  * per-window sets (any bases): key = w * B + |d| - 1, set weight 2^(c w);
 //  * one shared set (resident SRS with window-shifted copies 2^(cw) g_i built once):
 //    key = |d| - 1, reference = w * N + offset + i, a single set of weight 1.
+#include <pthread.h>
+
 #include <algorithm>
 #include <atomic>
 #include <condition_variable>
@@ -761,7 +763,7 @@ class TailPool {
   // fn(j) for j < n on the workers and the calling thread
   void run(size_t n, const std::function<void(size_t)>& fn) {
     std::unique_lock<std::mutex> busy(run_mu, std::try_to_lock);
-    if (!busy.owns_lock() || workers_.empty() || n < 2) {
+    if (!busy.owns_lock() || workers_.empty() || n < 2 || forked_child()) {
       for (size_t j = 0; j < n; j++) fn(j);
       return;
     }
@@ -781,7 +783,14 @@ class TailPool {
   }
 
  private:
+  // a child of fork() has none of the parent's worker threads: it runs its tails itself
+  static std::atomic<bool>& forked_flag() {
+    static std::atomic<bool> f{false};
+    return f;
+  }
+  static bool forked_child() { return forked_flag().load(std::memory_order_relaxed); }
   TailPool() {
+    pthread_atfork(nullptr, nullptr, [] { forked_flag().store(true); });
     const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
     const size_t nt = std::min<size_t>(15, hw > 1 ? hw - 1 : 0);
     for (size_t t = 0; t < nt; t++) workers_.emplace_back([this] { loop(); });
