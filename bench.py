@@ -16,7 +16,7 @@ byte.  Before anything is timed the autotuner picks the MSM and NTT plans on thi
 device (--no-tune keeps the models; --tune-file imports a saved table).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--log-n 20] [--no-cpu-baseline]
-                  [--cpu-prove-k 16] [--no-tune | --tune-file F] [--msm-only | --prove-only]
+                  [--cpu-prove-k 16] [--no-tune | --tune-file F] [--msm-only | --prove-only | --ntt-only | --serve-only]
 """
 from __future__ import annotations
 
@@ -60,6 +60,37 @@ def measure_mad_peak(mzk, ctx) -> float:
         ms, ops = mzk.diag_bench(ctx, 4, blocks, 256, 4096)
         best = max(best, ops / (ms * 1e-3))
     return best
+
+
+def bench_ntt(ctx, torch, peak: float, reps: int = 10) -> dict:
+    """2^22 forward NTT on a resident input (the domain's coeff_to_lagrange): wall time per
+    transform, the passes' HIP-event time, and their HBM and mad fractions."""
+    dom = ctx.domain(1, 22)
+    a = torch.zeros((1 << 22, 4), dtype=torch.int64, device="cuda")
+    ctx.fr_from_seed_dev(SEED, 2, 1 << 22, a)
+    for _ in range(2):
+        dom.coeff_to_lagrange(a)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        dom.coeff_to_lagrange(a)
+    ctx.synchronize()
+    ntt_ms = (time.perf_counter() - t1) / reps * 1e3
+    ctx.profile(True)
+    dom.coeff_to_lagrange(a)
+    p2 = ctx.profile_read()
+    ctx.profile(False)
+    kern_ms = sum(v[0] for k, v in p2.items() if k.startswith("k_ntt_pass"))
+    N = 1 << 22
+    frmul = (N // 2) * 22
+    dom.close()
+    return {
+        "log_n": 22, "ms": round(ntt_ms, 4), "kernel_ms": round(kern_ms, 4),
+        "hbm_GBps_algorithmic": round(64 * N / (kern_ms * 1e-3) / 1e9, 1),
+        "hbm_frac_of_8TBps": round(64 * N / (kern_ms * 1e-3) / 8e12, 4),
+        "fr_mul_per_s": round(frmul / (kern_ms * 1e-3), 1),
+        "mad_frac": round(frmul * MAD_PER_FR_MUL / (kern_ms * 1e-3) / peak, 4),
+    }
 
 
 def bench_batched(mzk, ctx, srs, n_srs: int, njobs: int = 96, seed: int = 96) -> dict:
@@ -304,6 +335,8 @@ def main() -> int:
     ap.add_argument("--serve-workers", type=int, default=4, help="server contexts (streams)")
     ap.add_argument("--serve-slots", type=int, default=2, help="proofs in flight per server context")
     ap.add_argument("--serve-only", action="store_true", help="only the serving runs (for sweeps of its settings)")
+    ap.add_argument("--ntt-only", action="store_true",
+                    help="only the NTT figures: 2^22 forward transforms (for rocprofv3 traces of the NTT passes)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run a 256 MiB elementwise copy and 3 MSM commits (for rocprofv3 --pmc runs)")
     ap.add_argument("--prove-only", action="store_true",
@@ -370,6 +403,11 @@ def main() -> int:
         with open(args.tune_out, "w") as f:
             f.write(mzk.tune_export())
 
+    if args.ntt_only:
+        r = bench_ntt(ctx, torch, measure_mad_peak(mzk, ctx), reps=20)
+        print(json.dumps({"ntt_only": True, "ntt": r, "tune": tune}), flush=True)
+        return 0
+
     if args.traffic_probe:
         # calibration: one elementwise pass that reads and writes exactly COPY_BYTES each
         a = torch.zeros(COPY_BYTES // 8, dtype=torch.int64, device="cuda")
@@ -415,36 +453,8 @@ def main() -> int:
         return 0
 
     # ---- NTT at 2^22 beside it (forward, resident input)
-    ntt = {}
     try:
-        dom = ctx.domain(1, 22)
-        a = torch.zeros((1 << 22, 4), dtype=torch.int64, device="cuda")
-        ctx.fr_from_seed_dev(SEED, 2, 1 << 22, a)
-        for _ in range(2):
-            dom.coeff_to_lagrange(a)
-        ctx.synchronize()
-        reps = 10
-        t1 = time.perf_counter()
-        for _ in range(reps):
-            dom.coeff_to_lagrange(a)
-        ctx.synchronize()
-        ntt_ms = (time.perf_counter() - t1) / reps * 1e3
-        ctx.profile(True)
-        dom.coeff_to_lagrange(a)
-        p2 = ctx.profile_read()
-        ctx.profile(False)
-        kern_ms = sum(v[0] for k, v in p2.items() if k.startswith("k_ntt_pass"))
-        N = 1 << 22
-        frmul = (N // 2) * 22
-        ntt = {
-            "log_n": 22, "ms": round(ntt_ms, 4), "kernel_ms": round(kern_ms, 4),
-            "hbm_GBps_algorithmic": round(64 * N / (kern_ms * 1e-3) / 1e9, 1),
-            "hbm_frac_of_8TBps": round(64 * N / (kern_ms * 1e-3) / 8e12, 4),
-            "fr_mul_per_s": round(frmul / (kern_ms * 1e-3), 1),
-            "mad_frac": round(frmul * MAD_PER_FR_MUL / (kern_ms * 1e-3) / peak, 4),
-        }
-        dom.close()
-        del a
+        ntt = bench_ntt(ctx, torch, peak)
     except Exception as e:  # NTT figures are secondary; the MSM line still prints
         ntt = {"error": str(e)}
 

@@ -76,22 +76,32 @@ static constexpr uint32_t NTT_PAIRS_MIN_LOG = 16;
 
 __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block, uint32_t* lds);
 
+// Workgroups go to the 8 XCDs round robin (workgroup i to XCD i mod 8), each XCD with its own
+// L2.  Neighbouring sub-DFTs of the column pass read neighbouring 32-byte pieces of the same
+// 128-byte lines, so the logical index is remapped to put consecutive sub-DFTs on one XCD
+// (a permutation of [0, G) when 8 divides G): the other order fetched every line from HBM
+// once per XCD touching it.
+__device__ __forceinline__ uint32_t xcd_local_block(uint32_t b, uint32_t G) {
+  return (G & 7u) ? b : (b & 7u) * (G >> 3) + (b >> 3);
+}
+
 __global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass(PassArgs a) {
   extern __shared__ uint32_t lds[];   // C * L elements of ELEM29 words
-  ntt_pass_body(a, blockIdx.x, lds);
+  ntt_pass_body(a, xcd_local_block(blockIdx.x, gridDim.x), lds);
 }
 
 // Many transforms in one launch: workgroups [first[i], first[i+1]) run pass table[i].
 __global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass_batch(const PassArgs* __restrict__ table,
                                                         const uint32_t* __restrict__ first, uint32_t njobs) {
   extern __shared__ uint32_t lds[];
-  uint32_t lo = 0, hi = njobs;   // first[lo] <= blockIdx.x < first[hi]
+  const uint32_t blk = xcd_local_block(blockIdx.x, gridDim.x);
+  uint32_t lo = 0, hi = njobs;   // first[lo] <= blk < first[hi]
   while (hi - lo > 1) {
     uint32_t mid = (lo + hi) >> 1;
-    if (first[mid] <= blockIdx.x) lo = mid; else hi = mid;
+    if (first[mid] <= blk) lo = mid; else hi = mid;
   }
   const PassArgs a = table[lo];
-  ntt_pass_body(a, blockIdx.x - first[lo], lds);
+  ntt_pass_body(a, blk - first[lo], lds);
 }
 
 // Element arithmetic of the passes is the radix-2^29 form (fr29.hpp): each butterfly's
