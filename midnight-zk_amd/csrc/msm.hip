@@ -21,6 +21,9 @@
 //  7. k_chan_sum      channels: sum_t A_t, and U_j = sum of the R_t whose index t has
 //                     bit j set, so sum_b (b+1) S_b = sum_t A_t + L sum_j 2^j U_j --
 //                     plain sums in any order, short dependency chains
+//     (one set of 2^t buckets instead: k_chan_tree merges the partials of 128
+//     buckets per block and reduces them as a binary tree whose nodes carry their
+//     per-bit sums, k_chan_top sums the block records into the same channels)
 //  8. host tail       Horner over the channel bit positions (one host core runs a
 //                     dependent chain of point operations ~20x faster than one wave).
 //
@@ -644,6 +647,112 @@ __device__ __forceinline__ g1x28 block_sum(g1x28 x, uint32_t* lds) {
   return x;
 }
 
+// 6'+7'. A single set of T = 2^t buckets (segments of one bucket, A_t = R_t = S_t) by
+// blocks of CT_LEAVES = 2^CT_LEVELS buckets.  The channels ask for sum_t S_t and, for
+// each bit j, U_j = sum of the S_t whose index t has bit j set.  Inside a block, a binary
+// tree over its buckets gives every node its sum P and, for each bit i below the node's
+// level, A_i = the sum of its buckets with bit i set: a parent's A_i is the sum of its
+// children's A_i, and its A_j (j its children's level) is its right child's P.  The root
+// carries the block's P and A_0..A_6 with (CT_LEAVES - 1) + (CT_LEAVES - 1 - CT_LEVELS)
+// additions, where the channel loop spends CT_LEAVES (1 + CT_LEVELS / 2); every level is
+// one addition step, the (CT_LEAVES / 2 >> j) (j + 1) additions of level j on separate
+// quads.  k_chan_top then sums U_j = sum over blocks of A_j for j < 7, and for the
+// higher bits the block sums P_k of the blocks k with bit j - 7 set.
+static constexpr uint32_t CT_LEVELS = 7;
+static constexpr uint32_t CT_LEAVES = 1u << CT_LEVELS;   // buckets per block, a quad each
+static constexpr uint32_t CT_REC = CT_LEVELS + 1;         // points per block record: P, A_0..A_6
+static constexpr uint32_t CT_TOP_QUADS = 128;
+
+// Slot of A_i of node m at the current level: A_i exists for 64 >> i nodes when created.
+__device__ __forceinline__ uint32_t ct_acc_slot(uint32_t i, uint32_t m) {
+  return CT_LEAVES + (CT_LEAVES - (CT_LEAVES >> i)) + m;
+}
+
+// Block k: buckets [k CT_LEAVES, (k + 1) CT_LEAVES), each the merge of its chunk partials
+// (as k_bucket_merge_quad), then the tree; writes out[CT_REC k + 0] = P, out[CT_REC k + 1 + i]
+// = A_i (radix-2^28 XYZZ, MZK_X28_WORDS words each).  LDS: P of the current level (in place,
+// CT_LEAVES slots) and the A_i (CT_LEAVES - 1 slots), (2 CT_LEAVES - 1) points.
+__global__ void __launch_bounds__(4 * CT_LEAVES) k_chan_tree(const uint32_t* __restrict__ off, uint32_t K,
+                                                            const uint32_t* __restrict__ partials,
+                                                            uint32_t* __restrict__ out) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t t = threadIdx.x, qd = t >> 2, ql = t & 3u;
+  {
+    const uint32_t b = blockIdx.x * CT_LEAVES + qd;
+    const uint32_t lo = off[b], hi = off[b + 1];
+    g1x28 x = g1x28_inf();
+    if (hi > lo) {
+      const uint32_t k0 = lo / K, k1 = (hi - 1) / K;
+      x = load_x28(partials + MZK_X28_WORDS * (size_t)(k0 + b));
+      for (uint32_t k = k0 + 1; k <= k1; k++) x = g1x28_add_quad(x, partials + MZK_X28_WORDS * (size_t)(k + b));
+    }
+    if (ql == 0) store_x28_raw(lds + MZK_X28_WORDS * qd, x);
+  }
+  __syncthreads();
+  for (uint32_t j = 0; j < CT_LEVELS; j++) {
+    const uint32_t parents = (CT_LEAVES / 2) >> j, per = j + 1;
+    // A_j of parent m = P of its right child 2m + 1 (the A_j slots are not read this level)
+    for (uint32_t w = t; w < parents * MZK_X28_WORDS; w += blockDim.x) {
+      const uint32_t m = w / MZK_X28_WORDS, e = w % MZK_X28_WORDS;
+      lds[MZK_X28_WORDS * ct_acc_slot(j, m) + e] = lds[MZK_X28_WORDS * (2 * m + 1) + e];
+    }
+    // task (m, u): u = 0 adds the children's P, u = i + 1 their A_i (i < j)
+    const bool act = qd < parents * per;
+    const uint32_t m = qd / per, u = qd % per;
+    const uint32_t s0 = u == 0 ? 2 * m : ct_acc_slot(u - 1, 2 * m);
+    g1x28 x = g1x28_inf();
+    if (act) x = g1x28_add_quad(load_x28(lds + MZK_X28_WORDS * s0), lds + MZK_X28_WORDS * (s0 + 1));
+    __syncthreads();   // every read of this level before any write
+    if (act && ql == 0) store_x28_raw(lds + MZK_X28_WORDS * (u == 0 ? m : ct_acc_slot(u - 1, m)), x);
+    __syncthreads();
+  }
+  // the root: P in slot 0, A_i in ct_acc_slot(i, 0)
+  for (uint32_t w = t; w < CT_REC * MZK_X28_WORDS; w += blockDim.x) {
+    const uint32_t r = w / MZK_X28_WORDS, e = w % MZK_X28_WORDS;
+    const uint32_t s = r == 0 ? 0 : ct_acc_slot(r - 1, 0);
+    out[MZK_X28_WORDS * ((size_t)blockIdx.x * CT_REC + r) + e] = lds[MZK_X28_WORDS * s + e];
+  }
+}
+
+// Channel h of nblk block records (nblk = 2^kbits): h = 0 sums every P_k; h = 1 + i
+// (i < CT_LEVELS) sums every A_i; h = 1 + CT_LEVELS + j sums the P_k whose k has bit j
+// set.  A quad per CT_TOP_QUADS-th record (the first of its terms loaded, not added to the
+// identity: each addition step is a serial step of the kernel), then a tree over the quads
+// in LDS; writes the blst-layout XYZZ (canonical 12 x 32-bit limbs) of channel h to out[48 h].
+__global__ void __launch_bounds__(4 * CT_TOP_QUADS) k_chan_top(const uint32_t* __restrict__ in, uint32_t nblk,
+                                                              uint32_t* __restrict__ out) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t qd = threadIdx.x >> 2, ql = threadIdx.x & 3u;
+  const uint32_t h = blockIdx.x;
+  const uint32_t slot = (h >= 1 && h <= CT_LEVELS) ? h : 0;
+  const uint32_t bit = h > CT_LEVELS ? h - 1 - CT_LEVELS : 0;
+  const bool all = h <= CT_LEVELS;
+  uint32_t k = qd;
+  while (k < nblk && !all && !((k >> bit) & 1u)) k += CT_TOP_QUADS;
+  g1x28 x = g1x28_inf();
+  if (k < nblk) {
+    x = load_x28(in + MZK_X28_WORDS * ((size_t)k * CT_REC + slot));
+    for (k += CT_TOP_QUADS; k < nblk; k += CT_TOP_QUADS)
+      if (all || ((k >> bit) & 1u)) x = g1x28_add_quad(x, in + MZK_X28_WORDS * ((size_t)k * CT_REC + slot));
+  }
+  if (ql == 0) store_x28_raw(lds + MZK_X28_WORDS * qd, x);
+  __syncthreads();
+  for (uint32_t d = CT_TOP_QUADS / 2; d >= 1; d >>= 1) {   // reads slots >= d, writes < d
+    if (qd < d) {
+      x = g1x28_add_quad(x, lds + MZK_X28_WORDS * (qd + d));
+      if (ql == 0) store_x28_raw(lds + MZK_X28_WORDS * qd, x);
+    }
+    __syncthreads();
+  }
+  if (qd == 0 && ql == 0) {
+    uint32_t o[48];
+    g1x28_to_std(x, o);
+    uint4* qv = reinterpret_cast<uint4*>(out + 48 * (size_t)h);
+#pragma unroll
+    for (int w = 0; w < 12; w++) qv[w] = make_uint4(o[4 * w], o[4 * w + 1], o[4 * w + 2], o[4 * w + 3]);
+  }
+}
+
 // 7. channels over the T segments of each set.  Channel 0 sums all A_t; channel
 // 1 + j sums the R_t whose index t has bit j set, so sum_t t R_t = sum_j 2^j U_j.
 // Block (set s, channel h, part q) writes out[(s * nch + h) * parts + q].
@@ -931,6 +1040,9 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   const uint32_t parts = std::max<uint32_t>(1, T / 2048);
   const size_t nchan = sets * nch;
   const uint32_t lbits = (uint32_t)ceil_log2(L);
+  // one set of 2^t buckets, t >= 8: channels by the block tree (k_chan_tree, k_chan_top)
+  const bool tree = sets == 1 && L == 1 && !(p.per_window && p.B <= 256) && T == NB && T >= 2 * CT_LEAVES &&
+                    (T & (T - 1)) == 0;
 
   // scratch A: descriptors | canonical scalars | hist | hoff (HN+1) | tile sums | off (NB+1)
   const size_t desc_words = (nj * sizeof(JobDesc) + td.size() * sizeof(TileDesc) + 15) / 4 + 8;
@@ -990,7 +1102,7 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   gk.add('M').add(sa).add(canon).add(hist).add(hoff).add(HN).add(tsums).add(tiles).add(tmp_key).add(tmp_ref);
   gk.add(refs).add(off).add(NB).add(K).add(nchunks).add(records).add(d_bases).add(partials).add(small_sets);
   gk.add(p.B).add(sets).add(chan).add(segA).add(segR).add(L).add(nseg).add(T).add(nch).add(parts).add(chan_parts);
-  gk.add(nchan);
+  gk.add(nchan).add(tree);
   MZK_TRY(run_graph(c, gk.s, [&]() -> int {
     MZK_LAUNCH(c, "k_sort_hist", k_sort_hist, dim3(sa.G), dim3(SORT_BLOCK), sa.D1 * 4, sa, canon, hist);
     MZK_LAUNCH(c, "k_scan_tiles", k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, hist, hoff, HN, tsums);
@@ -1018,6 +1130,12 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
         MZK_LAUNCH(c, "k_set_reduce", k_set_reduce, dim3((unsigned)((sets + spb - 1) / spb)), dim3(256),
                    256 * MZK_X28_WORDS * 4, off, NB, K, p.B, (uint32_t)sets, partials, chan);
       }
+    } else if (tree) {
+      // a single set of 2^t buckets: the merge and the tree by blocks, then the channels
+      MZK_LAUNCH(c, "k_chan_tree", k_chan_tree, dim3(NB / CT_LEAVES), dim3(4 * CT_LEAVES),
+                 (2 * CT_LEAVES - 1) * MZK_X28_WORDS * 4, off, K, partials, segA);
+      MZK_LAUNCH(c, "k_chan_top", k_chan_top, dim3((unsigned)nch), dim3(4 * CT_TOP_QUADS),
+                 CT_TOP_QUADS * MZK_X28_WORDS * 4, segA, NB / CT_LEAVES, chan);
     } else {
       if (L == 1)   // a single set: a quad per bucket
         MZK_LAUNCH(c, "k_bucket_seg", k_bucket_merge_quad, dim3(grid_for((size_t)NB * 4, 256)), dim3(256), 0, off, NB,

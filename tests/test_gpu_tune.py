@@ -118,3 +118,23 @@ def test_imported_split_inverse_and_coset(ctx, log_n, l1, groups):
     ctx.ntt(mzk.NttJob.make(d_small, d_out, log_n, mzk.NTT_COSET, n_real=n // 4))
     ctx.synchronize()
     assert np.array_equal(d_out.cpu().numpy().view(np.uint64), C.coeff_to_extended(small, log_n - 2, log_n))
+
+
+@pytest.mark.parametrize("n", [1, 3, 100, 4097])
+def test_shared_set_channel_tree_sparse(ctx, n):
+    """The shared bucket set (2^15 buckets) forced at tiny n: nearly every bucket is empty,
+    so the block tree of the channel sums runs mostly over identity leaves and whole blocks
+    of identities.  The commitment equals the oracle's msm_best and the tree kernels ran."""
+    tau = plonk.mont_limbs(0x7EE)
+    srs = ctx.srs_generate(tau, 1 << 14)   # the window-shifted tables exist from 2^14 bases on
+    bases = srs.download()
+    mzk.tune_import("".join(f"msm {L} 1 0 0 256\n" for L in range(15)))
+    s = C.fr_from_seed(12, 1, n)
+    ctx.profile(True)
+    got = srs.commit(dev(s), n)
+    names = set(ctx.profile_read())
+    ctx.profile(False)
+    assert ctx.msm_last_plan()["bucket_sets"] == 1
+    assert {"k_chan_tree", "k_chan_top"} <= names
+    assert np.array_equal(affine(got), affine(C.msm_best(s, bases[:n])))
+    srs.close()
