@@ -753,6 +753,78 @@ __global__ void __launch_bounds__(4 * CT_TOP_QUADS) k_chan_top(const uint32_t* _
   }
 }
 
+// The same tree for sets of segments (L > 1, T = 2^t segments per set, t >= 7): leaves are
+// the segments' (R_t, A_t), a lane each, ST_LEAVES per block; a node also carries Asum, the
+// sum of its A_t (channel 0).  Level j has (ST_LEAVES / 2 >> j)(j + 2) one-lane additions.
+// The records (P, Asum, A_0..A_5) of a set's T / ST_LEAVES blocks go to k_seg_top.
+static constexpr uint32_t ST_LEVELS = 6;
+static constexpr uint32_t ST_LEAVES = 1u << ST_LEVELS;
+static constexpr uint32_t ST_REC = ST_LEVELS + 2;
+
+__device__ __forceinline__ uint32_t st_acc_slot(uint32_t i, uint32_t m) {
+  return 2 * ST_LEAVES + (ST_LEAVES - (ST_LEAVES >> i)) + m;
+}
+
+// LDS: P (ST_LEAVES slots, in place), Asum (ST_LEAVES), the A_i (ST_LEAVES - 1).
+__global__ void __launch_bounds__(ST_LEAVES) k_seg_tree(const uint32_t* __restrict__ A,
+                                                       const uint32_t* __restrict__ R,
+                                                       uint32_t* __restrict__ out) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t t = threadIdx.x;
+  const size_t leaf = (size_t)blockIdx.x * ST_LEAVES + t;
+  for (uint32_t e = 0; e < MZK_X28_WORDS; e += 4) {
+    *reinterpret_cast<uint4*>(lds + MZK_X28_WORDS * t + e) =
+        *reinterpret_cast<const uint4*>(R + MZK_X28_WORDS * leaf + e);
+    *reinterpret_cast<uint4*>(lds + MZK_X28_WORDS * (ST_LEAVES + t) + e) =
+        *reinterpret_cast<const uint4*>(A + MZK_X28_WORDS * leaf + e);
+  }
+  __syncthreads();
+  for (uint32_t j = 0; j < ST_LEVELS; j++) {
+    const uint32_t parents = (ST_LEAVES / 2) >> j, per = j + 2;
+    for (uint32_t w = t; w < parents * MZK_X28_WORDS; w += blockDim.x) {   // A_j of parent m = P of child 2m + 1
+      const uint32_t m = w / MZK_X28_WORDS, e = w % MZK_X28_WORDS;
+      lds[MZK_X28_WORDS * st_acc_slot(j, m) + e] = lds[MZK_X28_WORDS * (2 * m + 1) + e];
+    }
+    // task (m, u): u = 0 adds the children's P, u = 1 their Asum, u = i + 2 their A_i (i < j)
+    const bool act = t < parents * per;
+    const uint32_t m = t / per, u = t % per;
+    const uint32_t s0 = u == 0 ? 2 * m : u == 1 ? ST_LEAVES + 2 * m : st_acc_slot(u - 2, 2 * m);
+    g1x28 x = g1x28_inf();
+    if (act) x = g1x28_add(load_x28(lds + MZK_X28_WORDS * s0), load_x28(lds + MZK_X28_WORDS * (s0 + 1)));
+    __syncthreads();
+    if (act) store_x28_raw(lds + MZK_X28_WORDS * (u == 0 ? m : u == 1 ? ST_LEAVES + m : st_acc_slot(u - 2, m)), x);
+    __syncthreads();
+  }
+  for (uint32_t w = t; w < ST_REC * MZK_X28_WORDS; w += blockDim.x) {
+    const uint32_t r = w / MZK_X28_WORDS, e = w % MZK_X28_WORDS;
+    const uint32_t s = r == 0 ? 0 : r == 1 ? ST_LEAVES : st_acc_slot(r - 2, 0);
+    out[MZK_X28_WORDS * ((size_t)blockIdx.x * ST_REC + r) + e] = lds[MZK_X28_WORDS * s + e];
+  }
+}
+
+// Channel h of set s from its nrec block records (block (s nch + h)): h = 0 sums the Asum,
+// h = 1 + i (i < ST_LEVELS) the A_i, h = 1 + ST_LEVELS + j the P of the records whose
+// index has bit j set.  A lane per record (blockDim.x >= nrec), a tree sum, then the
+// blst-layout XYZZ of the channel at out[48 (s nch + h)].
+__global__ void __launch_bounds__(256) k_seg_top(const uint32_t* __restrict__ in, uint32_t nrec, uint32_t nch,
+                                                 uint32_t* __restrict__ out) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t k = threadIdx.x;
+  const uint32_t s = blockIdx.x / nch, h = blockIdx.x % nch;
+  const uint32_t slot = h == 0 ? 1 : h <= ST_LEVELS ? h + 1 : 0;
+  g1x28 x = g1x28_inf();
+  if (k < nrec && (h <= ST_LEVELS || ((k >> (h - 1 - ST_LEVELS)) & 1u)))
+    x = load_x28(in + MZK_X28_WORDS * (((size_t)s * nrec + k) * ST_REC + slot));
+  x = block_sum(x, lds);
+  if (k == 0) {
+    uint32_t o[48];
+    g1x28_to_std(x, o);
+    uint4* qv = reinterpret_cast<uint4*>(out + 48 * (size_t)blockIdx.x);
+#pragma unroll
+    for (int w = 0; w < 12; w++) qv[w] = make_uint4(o[4 * w], o[4 * w + 1], o[4 * w + 2], o[4 * w + 3]);
+  }
+}
+
 // 7. channels over the T segments of each set.  Channel 0 sums all A_t; channel
 // 1 + j sums the R_t whose index t has bit j set, so sum_t t R_t = sum_j 2^j U_j.
 // Block (set s, channel h, part q) writes out[(s * nch + h) * parts + q].
@@ -1043,6 +1115,10 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   // one set of 2^t buckets, t >= 8: channels by the block tree (k_chan_tree, k_chan_top)
   const bool tree = sets == 1 && L == 1 && !(p.per_window && p.B <= 256) && T == NB && T >= 2 * CT_LEAVES &&
                     (T & (T - 1)) == 0;
+  // segments of L > 1 buckets, 2^7 <= T <= 2^14 per set: channels by the segment tree
+  // (k_seg_tree, k_seg_top), its records written over the chunk partials once merged
+  const bool segtree = L > 1 && !(p.per_window && p.B <= 256) && (T & (T - 1)) == 0 && T >= 2 * ST_LEAVES &&
+                       T <= 256 * ST_LEAVES;
 
   // scratch A: descriptors | canonical scalars | hist | hoff (HN+1) | tile sums | off (NB+1)
   const size_t desc_words = (nj * sizeof(JobDesc) + td.size() * sizeof(TileDesc) + 15) / 4 + 8;
@@ -1102,7 +1178,7 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   gk.add('M').add(sa).add(canon).add(hist).add(hoff).add(HN).add(tsums).add(tiles).add(tmp_key).add(tmp_ref);
   gk.add(refs).add(off).add(NB).add(K).add(nchunks).add(records).add(d_bases).add(partials).add(small_sets);
   gk.add(p.B).add(sets).add(chan).add(segA).add(segR).add(L).add(nseg).add(T).add(nch).add(parts).add(chan_parts);
-  gk.add(nchan).add(tree);
+  gk.add(nchan).add(tree).add(segtree);
   MZK_TRY(run_graph(c, gk.s, [&]() -> int {
     MZK_LAUNCH(c, "k_sort_hist", k_sort_hist, dim3(sa.G), dim3(SORT_BLOCK), sa.D1 * 4, sa, canon, hist);
     MZK_LAUNCH(c, "k_scan_tiles", k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, hist, hoff, HN, tsums);
@@ -1143,10 +1219,18 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
       else
         MZK_LAUNCH(c, "k_bucket_seg", k_bucket_seg, dim3(grid_for(nseg, 64)), dim3(64), 0, off, NB, K, L, partials,
                    segA, segR);
-      MZK_LAUNCH(c, "k_chan_sum", k_chan_sum, dim3((unsigned)(nchan * parts)), dim3(CHAN_BLOCK),
-                 CHAN_BLOCK * MZK_X28_WORDS * 4, segA, segR, T, nch, parts, chan_parts);
-      MZK_LAUNCH(c, "k_seg_sum", k_seg_sum, dim3((unsigned)nchan), dim3(64), 64 * MZK_X28_WORDS * 4, chan_parts,
-                 parts, chan);
+      if (segtree) {   // the segments' channels by the block tree: records into the chunk partials' space
+        const uint32_t nrec = T / ST_LEAVES, top_threads = std::max<uint32_t>(64, nrec);
+        MZK_LAUNCH(c, "k_seg_tree", k_seg_tree, dim3((unsigned)(nseg / ST_LEAVES)), dim3(ST_LEAVES),
+                   (3 * ST_LEAVES - 1) * MZK_X28_WORDS * 4, segA, segR, partials);
+        MZK_LAUNCH(c, "k_seg_top", k_seg_top, dim3((unsigned)nchan), dim3(top_threads),
+                   top_threads * MZK_X28_WORDS * 4, partials, nrec, nch, chan);
+      } else {
+        MZK_LAUNCH(c, "k_chan_sum", k_chan_sum, dim3((unsigned)(nchan * parts)), dim3(CHAN_BLOCK),
+                   CHAN_BLOCK * MZK_X28_WORDS * 4, segA, segR, T, nch, parts, chan_parts);
+        MZK_LAUNCH(c, "k_seg_sum", k_seg_sum, dim3((unsigned)nchan), dim3(64), 64 * MZK_X28_WORDS * 4, chan_parts,
+                   parts, chan);
+      }
     }
     MZK_HIP(hipGetLastError());
     return MZK_OK;
