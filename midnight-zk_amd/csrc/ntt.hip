@@ -111,6 +111,18 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass_batch(const PassArgs* 
 // a table entry below r, or reduce_small of the operand where the twiddle is 1), so they
 // stay below 25 r (< 2^261) through 11 stages; the store multiplies by the twist / scale
 // factors where there are any and leaves in canonical blst words.
+// LDS slot of element idx: ELEM29 words each, plus one word of padding per 2^NTT_LDS_SWIZZLE
+// elements.  With 9-word elements alone, the load phase's bit-reversed stores put the 32 lanes
+// of a half wave (indices 64 apart) in one bank, and the paired stages' stride-2 accesses
+// conflicted 4 ways; the padding spreads both over the banks.
+static constexpr uint32_t NTT_LDS_SWIZZLE = 5;
+__device__ __forceinline__ uint32_t* elem29(uint32_t* lds, uint32_t idx) {
+  return lds + ELEM29 * idx + (idx >> NTT_LDS_SWIZZLE);
+}
+__host__ __device__ constexpr size_t ntt_lds_bytes(size_t elems) {
+  return (elems * ELEM29 + (elems >> NTT_LDS_SWIZZLE)) * 4;
+}
+
 __device__ __forceinline__ fr29 load29(const uint32_t* __restrict__ p) {
   fr29 r;
 #pragma unroll
@@ -166,7 +178,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block,
 #pragma unroll
       for (int i = 0; i < 9; i++) x.l[i] = 0;
     }
-    store29(lds + 9 * (c * L + bitrev(j, a.log_l)), x);
+    store29(elem29(lds, c * L + bitrev(j, a.log_l)), x);
   }
   __syncthreads();
 
@@ -186,8 +198,8 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block,
         const uint32_t c = u / halfL, v = u % halfL;
         const uint32_t pos = v & (half - 1);
         const uint32_t i = ((v >> s) << (s + 1)) + pos;
-        uint32_t* pa = lds + 9 * (c * L + i);
-        uint32_t* pb = pa + 9 * half;
+        uint32_t* pa = elem29(lds, c * L + i);
+        uint32_t* pb = elem29(lds, c * L + i + half);
         const fr29 x = load29(pa);
         const fr29 y = load29(pb);
         const fr29 t = pos != 0 ? fr29_mul(y, load_rec29(a.tw + TW29 * (pos << (a.log_l - 1 - s))))
@@ -201,8 +213,8 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block,
     const uint32_t halfL = L >> 1;
     for (uint32_t u = threadIdx.x; u < a.C * halfL; u += nt) {
       const uint32_t c = u / halfL, v = u % halfL;
-      uint32_t* pa = lds + 9 * (c * L + 2 * v);
-      uint32_t* pb = pa + 9;
+      uint32_t* pa = elem29(lds, c * L + 2 * v);
+      uint32_t* pb = elem29(lds, c * L + 2 * v + 1);
       const fr29 x = load29(pa);
       const fr29 t = fr29_reduce_small(load29(pb));
       store29(pa, fr29_add(x, t));
@@ -218,10 +230,10 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block,
       const uint32_t c = u / quarterL, v = u % quarterL;
       const uint32_t pos = v & (half - 1);
       const uint32_t i = ((v >> s) << (s + 2)) + pos;
-      uint32_t* p0 = lds + 9 * (c * L + i);
-      uint32_t* p1 = p0 + 9 * half;
-      uint32_t* p2 = p1 + 9 * half;
-      uint32_t* p3 = p2 + 9 * half;
+      uint32_t* p0 = elem29(lds, c * L + i);
+      uint32_t* p1 = elem29(lds, c * L + i + half);
+      uint32_t* p2 = elem29(lds, c * L + i + 2 * half);
+      uint32_t* p3 = elem29(lds, c * L + i + 3 * half);
       const fr29 x0 = load29(p0), x1 = load29(p1), x2 = load29(p2), x3 = load29(p3);
       // stage s: (x0, x1) and (x2, x3), both with w = omega_(2 half)^pos
       fr29 t1, t3;
@@ -257,7 +269,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block,
   for (uint32_t e = threadIdx.x; e < CL; e += nt) {
     const uint32_t c = e % a.C, k = e / a.C;
     const uint64_t b = b0 + c;
-    fr29 x = load29(lds + 9 * (c * L + k));
+    fr29 x = load29(elem29(lds, c * L + k));
     if (a.do_twist) {
       const uint64_t ex = b * k;   // < N
       if (ex != 0) {
@@ -393,7 +405,7 @@ static int ntt_lds_attr() {
   static std::once_flag once;
   static int rc = MZK_OK;
   std::call_once(once, [] {
-    const int bytes = NTT_LDS_ELEMS * ELEM29 * 4;
+    const int bytes = (int)ntt_lds_bytes(NTT_LDS_ELEMS);
     if (hipFuncSetAttribute((const void*)k_ntt_pass, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
             hipSuccess ||
         hipFuncSetAttribute((const void*)k_ntt_pass_batch, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
@@ -416,7 +428,7 @@ static uint32_t choose_c(uint32_t L, uint64_t nbatch, uint32_t groups) {
 static int launch_pass(mzk_ctx* c, const char* name, PassArgs a, uint64_t nbatch) {
   const uint32_t L = 1u << a.log_l;
   const uint32_t threads = std::min<uint32_t>(NTT_THREADS, std::max<uint32_t>(64, a.C * L / 2));
-  const size_t lds = (size_t)a.C * L * ELEM29 * 4;
+  const size_t lds = ntt_lds_bytes((size_t)a.C * L);
   MZK_TRY(ntt_lds_attr());
   MZK_LAUNCH(c, name, k_ntt_pass, dim3((unsigned)(nbatch / a.C)), dim3(threads), lds, a);
   MZK_HIP(hipGetLastError());
@@ -583,7 +595,7 @@ int ntt_run_batch(mzk_ctx* c, const mzk_ntt_job* jobs, size_t njobs) {
   uint32_t* d_f1 = (uint32_t*)put(f1.data(), f1.size() * 4);
   uint32_t* d_f2 = (uint32_t*)put(f2.data(), f2.size() * 4);
   MZK_HIP(hipMemcpyAsync(d, h.data(), o, hipMemcpyHostToDevice, c->stream));
-  const size_t lds = (size_t)NTT_LDS_ELEMS * ELEM29 * 4;
+  const size_t lds = ntt_lds_bytes(NTT_LDS_ELEMS);
   MZK_TRY(ntt_lds_attr());
   // the two launches as one cached graph (the pass tables are copied before it runs)
   const uint32_t n1 = (uint32_t)t1.size(), n2 = (uint32_t)t2.size(), g1 = f1.back(), g2 = f2.back();
