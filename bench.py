@@ -115,9 +115,16 @@ def bench_batched(mzk, ctx, srs, n_srs: int, njobs: int = 96, seed: int = 96) ->
         o += n
         views.append(v)
         jobs.append((v, n, (i * 7919) % (n_srs - n + 1)))
-    # MSM: per call, then batched (one warm-up each)
-    per = [srs.commit(v, n, offset=off) for v, n, off in jobs[:4]]
-    srs.commit_batch(jobs[:4])
+    # MSM: per call, then batched.  Each path runs once untimed first: a new launch shape
+    # captures and instantiates its HIP graph on its first call (~16 ms for a 96-job batch),
+    # which repeated prover rounds of the same shape do not pay; that first call is
+    # reported beside the timed second one.
+    t0 = time.perf_counter()
+    per = [srs.commit(v, n, offset=off) for v, n, off in jobs]
+    t_per_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    srs.commit_batch(jobs)
+    t_bat_first = time.perf_counter() - t0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     per = [srs.commit(v, n, offset=off) for v, n, off in jobs]
@@ -129,16 +136,22 @@ def bench_batched(mzk, ctx, srs, n_srs: int, njobs: int = 96, seed: int = 96) ->
     # NTT: coset transforms of size next_pow2(n), n coefficients each
     outs = [torch.zeros((1 << max(1, (n - 1).bit_length()), 4), dtype=torch.int64, device="cuda") for n in sizes]
     nj = [mzk.NttJob.make(v, d, max(1, (n - 1).bit_length()), mzk.NTT_COSET, n) for v, d, n in zip(views, outs, sizes)]
-    for j in nj[:4]:
+    for j in nj:
         ctx.ntt(j)
-    ctx.ntt_batch(nj[:4])
     ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.ntt_batch(nj)
+    ctx.synchronize()
+    t_nbat_first = time.perf_counter() - t0
     t0 = time.perf_counter()
     for j in nj:
         ctx.ntt(j)
     ctx.synchronize()
     t_nper = time.perf_counter() - t0
     ref = [d.clone() for d in outs]
+    for d in outs:
+        d.zero_()
+    ctx.synchronize()
     t0 = time.perf_counter()
     ctx.ntt_batch(nj)
     ctx.synchronize()
@@ -153,6 +166,8 @@ def bench_batched(mzk, ctx, srs, n_srs: int, njobs: int = 96, seed: int = 96) ->
         "ntt_total_elements": ntt_elems, "ntt_per_call_ms": round(t_nper * 1e3, 3),
         "ntt_batched_ms": round(t_nbat * 1e3, 3), "ntt_speedup": round(t_nper / t_nbat, 3),
         "ntt_batched_equals_per_call": ntt_same,
+        "first_call_ms": {"msm_per_call": round(t_per_first * 1e3, 3), "msm_batched": round(t_bat_first * 1e3, 3),
+                          "ntt_batched": round(t_nbat_first * 1e3, 3)},
     }
 
 

@@ -324,8 +324,10 @@ int mzk_prof_enable(mzk_ctx *ctx, int on);
  * No reference counterpart (the reference's CPU path has no launches).  The fixed
  * device launch sequences of a call -- an MSM's sort, accumulation and reduction
  * kernels, a batch of NTTs' two passes -- run as HIP graphs, one per distinct shape and
- * buffer set, captured on first use and replayed afterwards (a prover's rounds repeat
- * the same shapes on the same buffers proof after proof).  On by default; off replays
+ * buffer set: a shape's first call runs eagerly, its second is captured, later ones
+ * replay (a prover's rounds repeat the same shapes on the same buffers proof after
+ * proof; one-off shapes skip the capture cost and evict nothing; up to 48 held, least
+ * recently used dropped).  On by default; off replays
  * nothing and frees the graphs.  Results are identical either way; the kernel profiler
  * bypasses graphs while it is on.  out = {graphs captured, replays, graphs held}. */
 int mzk_ctx_set_graphs(mzk_ctx *ctx, int on);

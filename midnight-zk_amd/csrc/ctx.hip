@@ -104,6 +104,15 @@ int run_graph(mzk_ctx* c, const std::string& key, const std::function<int()>& la
   if (!G.on || c->prof.on) return launch();
   auto it = G.map.find(key);
   if (it == G.map.end()) {
+    // a shape's first call runs eagerly: capture and instantiation (~16 ms for a 96-job
+    // batch) pay only when the shape repeats, and one-off shapes do not evict cached ones
+    if (G.seen.insert(key).second) {
+      if (G.seen.size() > GraphCache::MAX_SEEN) {
+        G.seen.clear();
+        G.seen.insert(key);
+      }
+      return launch();
+    }
     if (G.map.size() >= GraphCache::MAX_GRAPHS) {
       // drop the least recently used graph; wait first, it may still be running
       auto old = G.map.begin();
@@ -143,6 +152,7 @@ int run_graph(mzk_ctx* c, const std::string& key, const std::function<int()>& la
 void free_graphs(mzk_ctx* c) {
   for (auto& kv : c->graphs.map) (void)hipGraphExecDestroy(kv.second.exec);
   c->graphs.map.clear();
+  c->graphs.seen.clear();
 }
 
 // ---------------------------------------------------------------- seeded stream

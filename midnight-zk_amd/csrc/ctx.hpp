@@ -4,6 +4,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "common.hpp"
@@ -72,6 +73,8 @@ double host_ms();
 // a graph whose key matches is the same work as launching the sequence again.
 struct GraphCache {
   static constexpr size_t MAX_GRAPHS = 48;
+  static constexpr size_t MAX_SEEN = 4096;   // keys run once so far (cleared when full)
+  std::unordered_set<std::string> seen;
   struct Entry {
     hipGraphExec_t exec;
     uint64_t last_use;
