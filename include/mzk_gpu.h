@@ -49,6 +49,9 @@ void mzk_ctx_destroy(mzk_ctx *ctx);
 /* The context's hipStream_t, for callers that record events or order work. */
 void *mzk_ctx_stream(mzk_ctx *ctx);
 int mzk_ctx_synchronize(mzk_ctx *ctx);
+/* Free and total device memory of the context's device (hipMemGetInfo), for callers that
+ * admit work by HBM budget.  No reference counterpart. */
+int mzk_ctx_mem_info(mzk_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
 
 /* ------------------------------------------------------------------ MSM
  * Replaces curves/src/msm.rs:460 `msm_best(coeffs: &[C::Scalar], bases: &[C]) -> C::Curve`
@@ -60,6 +63,18 @@ int mzk_msm_best(mzk_ctx *ctx, const uint64_t *coeffs, const uint64_t *bases, si
  * zero scalars contribute nothing; all-zero (or empty) input gives the identity. */
 int mzk_msm_specific(mzk_ctx *ctx, const uint64_t *coeffs, const uint64_t *bases, size_t n,
                      uint64_t out[18]);
+/* Replaces curves/src/msm.rs:430 `msm_parallel(coeffs, bases) -> C::Curve` (the reference's
+ * thread split of msm_serial chunks; the sum is the same unique point as msm_best). */
+int mzk_msm_parallel(mzk_ctx *ctx, const uint64_t *coeffs, const uint64_t *bases, size_t n,
+                     uint64_t out[18]);
+/* Replaces curves/src/msm.rs:338 `msm_serial(coeffs, bases, acc: &mut C::Curve)`, acc
+ * Jacobian in and out.  As the reference does, the incoming acc is doubled c times per
+ * window before the sum is added (msm.rs:364-367): acc <- [2^(c W)] acc + sum_i coeffs[i]
+ * bases[i], c = 1 (n < 4), 3 (n < 32) or ceil(ln n), W = 8 (highest non-zero byte of the
+ * canonical scalars) / c + 1; all-zero scalars leave acc unchanged.  From the identity
+ * (how msm_parallel calls it) acc becomes the MSM. */
+int mzk_msm_serial(mzk_ctx *ctx, const uint64_t *coeffs, const uint64_t *bases, size_t n,
+                   uint64_t acc[18]);
 /* Same result with device-resident scalars and bases. */
 int mzk_msm_dev(mzk_ctx *ctx, const void *d_coeffs, const void *d_bases, size_t n,
                 uint64_t out[18]);
@@ -132,13 +147,37 @@ int mzk_msm_srs_batch_dev(mzk_ctx *ctx, const mzk_srs *srs, const mzk_msm_job *j
                           uint64_t *outs);
 
 /* ------------------------------------------------------------------ NTT
+ * Transforms of every size the field allows, 2^1 .. 2^32 (S = 32, fq.rs:43-48): one pass up
+ * to 2^11, two up to 2^22, three above; device memory permitting.
+ *
  * Replaces curves/src/fft.rs:49 `best_fft(a: &mut [G], omega, log_n)` for G = Fq:
  * in-place A_j = sum_i a_i omega^(ij), natural order, host pointer, n = 2^log_n.
  * omega must be a primitive 2^log_n-th root of unity (not checked, as in the reference). */
 int mzk_best_fft(mzk_ctx *ctx, uint64_t *a, const uint64_t omega[4], uint32_t log_n);
+/* Replaces curves/src/fft.rs:31 `compute_twiddles(omega, log_n) -> Vec<F>`:
+ * out[i] = omega^i for i < 2^(log_n - 1) (host memory), 1 <= log_n <= 32. */
+int mzk_compute_twiddles(mzk_ctx *ctx, const uint64_t omega[4], uint32_t log_n, uint64_t *out);
+/* Replaces curves/src/fft.rs:56 `best_fft_with_twiddles(a, twiddles, log_n)` for G = Fq
+ * (as EvaluationDomain::fft / ifft call it, domain.rs:389-396): in place on 2^log_n host
+ * elements.  twiddles = compute_twiddles(omega, log_n) (2^(log_n - 1) elements); the
+ * transform is the DFT by omega = twiddles[1] (-1 for log_n = 1).  The table is checked
+ * where that is cheap (twiddles[0] = 1, omega^(n/2) = -1, the last entry = omega^(n/2 - 1)):
+ * a table that fails gives MZK_ERR_INVALID. */
+int mzk_best_fft_with_twiddles(mzk_ctx *ctx, uint64_t *a, const uint64_t *twiddles, uint32_t log_n);
+/* Replaces curves/src/fft.rs:110 `fft_coeff_to_extended(a, twiddles, log_n, n_real)`: the
+ * same DFT of a whose entries from n_real on are zero, as the reference's pruned DIF
+ * requires (fft.rs:126-143); only a[0, n_real) is read, all 2^log_n entries are written. */
+int mzk_fft_coeff_to_extended(mzk_ctx *ctx, uint64_t *a, const uint64_t *twiddles, uint32_t log_n,
+                              size_t n_real);
+/* best_fft on device memory with a caller's root: d_out = the DFT by omega of d_in's
+ * first n_real elements zero-padded to 2^log_n (n_real = 2^log_n for a plain best_fft;
+ * d_in == d_out allowed).  Twiddle tables are kept on the context per (size, root). */
+int mzk_best_fft_dev(mzk_ctx *ctx, const void *d_in, void *d_out, const uint64_t omega[4], uint32_t log_n,
+                     uint64_t n_real);
 
 /* Replaces proofs/src/poly/domain.rs:34-167 `EvaluationDomain::new(j, k)`:
- * k, extended_k = min{e >= k : 2^e >= 2^k (j-1)}, omega tables resident on device. */
+ * k, extended_k = min{e >= k : 2^e >= 2^k (j-1)} <= 32 (domain.rs:64-70), omega tables
+ * resident on device. */
 typedef struct mzk_domain mzk_domain;
 int mzk_domain_create(mzk_ctx *ctx, uint32_t j, uint32_t k, mzk_domain **out);
 void mzk_domain_destroy(mzk_domain *d);
@@ -154,7 +193,7 @@ int mzk_coeff_to_extended_dev(mzk_domain *d, const void *d_in, void *d_out);
 int mzk_extended_to_coeff_dev(mzk_domain *d, void *d_a);
 /* domain.rs:361-380 divide_by_vanishing_poly: in place on 2^extended_k elements. */
 int mzk_divide_by_vanishing_poly_dev(mzk_domain *d, void *d_a);
-/* Stand-alone transforms of size 2^log_n (log_n <= 22) on device memory, with the
+/* Stand-alone transforms of size 2^log_n (1 <= log_n <= 32) on device memory, with the
  * domain semantics of domain.rs:
  *   flags 0                          : DFT with omega_n of n_real coefficients zero-padded
  *                                      to 2^log_n (coeff_to_lagrange when n_real = 2^log_n);
@@ -172,9 +211,11 @@ typedef struct {
   uint32_t flags;
 } mzk_ntt_job;
 int mzk_ntt_dev(mzk_ctx *ctx, const mzk_ntt_job *job);
-/* Many transforms of mixed sizes in two launches (all first passes, then all second
- * passes): the batched form of the prover's per-column par_iter NTTs
- * (prover.rs:276-280, 652-659). */
+/* Many transforms of mixed sizes in up to three launches (all first passes, then all
+ * second passes, then the third passes of transforms above 2^22): the batched form of the
+ * prover's per-column par_iter NTTs (prover.rs:276-280, 652-659).  No job's output may
+ * overlap another job's input or output (a job's own d_in == d_out is allowed):
+ * MZK_ERR_INVALID names the first such pair. */
 int mzk_ntt_batch_dev(mzk_ctx *ctx, const mzk_ntt_job *jobs, size_t njobs);
 /* Host-pointer forms of the four transforms. */
 int mzk_lagrange_to_coeff(mzk_domain *d, uint64_t *a);
@@ -326,8 +367,10 @@ int mzk_prof_enable(mzk_ctx *ctx, int on);
  * kernels, a batch of NTTs' two passes -- run as HIP graphs, one per distinct shape and
  * buffer set: a shape's first call runs eagerly, its second is captured, later ones
  * replay (a prover's rounds repeat the same shapes on the same buffers proof after
- * proof; one-off shapes skip the capture cost and evict nothing; up to 48 held, least
- * recently used dropped).  On by default; off replays
+ * proof; one-off shapes skip the capture cost and evict nothing).  Up to 48 are held; a
+ * held graph is replaced only after 384 calls without use, so a working set of more
+ * shapes keeps 48 of them and runs the rest eagerly, with no recapture and no wait.
+ * On by default; off replays
  * nothing and frees the graphs.  Results are identical either way; the kernel profiler
  * bypasses graphs while it is on.  out = {graphs captured, replays, graphs held}. */
 int mzk_ctx_set_graphs(mzk_ctx *ctx, int on);

@@ -99,9 +99,26 @@ int KernelProfiler::collect() {
 }
 
 // ---------------------------------------------------------------- launch graphs
+// Destroys replaced graphs once nothing is queued on the stream (they may still be
+// running until then); with too many waiting, one wait frees them all.
+static int reap_retired(mzk_ctx* c, bool force) {
+  GraphCache& G = c->graphs;
+  if (G.retired.empty()) return MZK_OK;
+  if (force || G.retired.size() > GraphCache::MAX_RETIRED) {
+    MZK_HIP(hipStreamSynchronize(c->stream));
+  } else if (hipStreamQuery(c->stream) != hipSuccess) {
+    return MZK_OK;   // still busy (hipErrorNotReady): try again on a later call
+  }
+  for (hipGraphExec_t ex : G.retired) (void)hipGraphExecDestroy(ex);
+  G.retired.clear();
+  return MZK_OK;
+}
+
 int run_graph(mzk_ctx* c, const std::string& key, const std::function<int()>& launch) {
   GraphCache& G = c->graphs;
   if (!G.on || c->prof.on) return launch();
+  MZK_TRY(reap_retired(c, false));
+  const uint64_t now = ++G.tick;
   auto it = G.map.find(key);
   if (it == G.map.end()) {
     // a shape's first call runs eagerly: capture and instantiation (~16 ms for a 96-job
@@ -114,12 +131,12 @@ int run_graph(mzk_ctx* c, const std::string& key, const std::function<int()>& la
       return launch();
     }
     if (G.map.size() >= GraphCache::MAX_GRAPHS) {
-      // drop the least recently used graph; wait first, it may still be running
       auto old = G.map.begin();
       for (auto e = G.map.begin(); e != G.map.end(); ++e)
         if (e->second.last_use < old->second.last_use) old = e;
-      MZK_HIP(hipStreamSynchronize(c->stream));
-      (void)hipGraphExecDestroy(old->second.exec);
+      // every cached graph ran recently: the working set exceeds the cache, keep it
+      if (now - old->second.last_use < GraphCache::STALE_TICKS) return launch();
+      G.retired.push_back(old->second.exec);
       G.map.erase(old);
     }
     MZK_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -144,14 +161,17 @@ int run_graph(mzk_ctx* c, const std::string& key, const std::function<int()>& la
   } else {
     G.replays++;
   }
-  it->second.last_use = ++G.tick;
+  it->second.last_use = now;
   MZK_HIP(hipGraphLaunch(it->second.exec, c->stream));
   return MZK_OK;
 }
 
+// Caller has synchronised the stream.
 void free_graphs(mzk_ctx* c) {
   for (auto& kv : c->graphs.map) (void)hipGraphExecDestroy(kv.second.exec);
+  for (hipGraphExec_t ex : c->graphs.retired) (void)hipGraphExecDestroy(ex);
   c->graphs.map.clear();
+  c->graphs.retired.clear();
   c->graphs.seen.clear();
 }
 
@@ -249,6 +269,13 @@ int mzk_ctx_synchronize(mzk_ctx* c) {
   MZK_REQUIRE(c != nullptr, "null context");
   MZK_HIP(hipSetDevice(c->device));
   MZK_HIP(hipStreamSynchronize(c->stream));
+  return MZK_OK;
+}
+
+int mzk_ctx_mem_info(mzk_ctx* c, size_t* free_bytes, size_t* total_bytes) {
+  MZK_REQUIRE(c != nullptr && free_bytes != nullptr && total_bytes != nullptr, "mzk_ctx_mem_info: null argument");
+  MZK_HIP(hipSetDevice(c->device));
+  MZK_HIP(hipMemGetInfo(free_bytes, total_bytes));
   return MZK_OK;
 }
 

@@ -71,16 +71,26 @@ double host_ms();
 // Executable graphs of launch sequences, keyed by everything the sequence's launches
 // depend on (kernel, grid, block, LDS bytes and every argument, serialised): replaying
 // a graph whose key matches is the same work as launching the sequence again.
+// A shape is captured on its second call.  When the cache is full, the least recently
+// used graph is replaced only if it has not run during the last STALE_TICKS calls;
+// otherwise the new shape runs eagerly.  A working set that cycles through more shapes
+// than the cache holds (up to STALE_TICKS of them) then keeps MAX_GRAPHS of them cached
+// instead of recapturing on every call, and a workload that moves on to new shapes gets
+// graphs for them after STALE_TICKS calls.  Replaced graphs are destroyed once the
+// stream has gone idle, never with a wait.
 struct GraphCache {
   static constexpr size_t MAX_GRAPHS = 48;
   static constexpr size_t MAX_SEEN = 4096;   // keys run once so far (cleared when full)
+  static constexpr uint64_t STALE_TICKS = 8 * MAX_GRAPHS;
+  static constexpr size_t MAX_RETIRED = 64;  // above this, one wait frees them all
   std::unordered_set<std::string> seen;
   struct Entry {
     hipGraphExec_t exec;
     uint64_t last_use;
   };
   std::map<std::string, Entry> map;
-  uint64_t tick = 0;
+  std::vector<hipGraphExec_t> retired;
+  uint64_t tick = 0;                         // run_graph calls so far
   uint64_t captures = 0, replays = 0;
   bool on = true;
 };
@@ -128,6 +138,6 @@ struct mzk_ctx {
   mzk::Pinned pinned;   // MSM descriptors in, channel sums out (the MSM waits before returning)
   mzk::KernelProfiler prof;
   uint64_t last_msm[6] = {0, 0, 0, 0, 0, 0};   // c, W, sets, K, M, n of the last MSM
-  std::map<int, mzk::NttPlan*> ntt_plans;        // (log_n, inverse) -> resident twiddle tables
+  std::map<std::string, mzk::NttPlan*> ntt_plans;   // (log_n, split, root) -> resident twiddle tables
   mzk::GraphCache graphs;
 };

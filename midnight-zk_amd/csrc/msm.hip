@@ -35,6 +35,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <condition_variable>
 #include <functional>
 #include <mutex>
@@ -1338,6 +1339,62 @@ int mzk_msm_best(mzk_ctx* c, const uint64_t* coeffs, const uint64_t* bases, size
 // (kzg/msm.rs:176-181) is implicit in the bucket histogram.
 int mzk_msm_specific(mzk_ctx* c, const uint64_t* coeffs, const uint64_t* bases, size_t n, uint64_t out[18]) {
   return msm_host(c, coeffs, bases, n, out);
+}
+
+// msm_parallel (msm.rs:430-455): the reference splits the terms over threads; the sum
+// is the same unique point msm_best computes.
+int mzk_msm_parallel(mzk_ctx* c, const uint64_t* coeffs, const uint64_t* bases, size_t n, uint64_t out[18]) {
+  return msm_host(c, coeffs, bases, n, out);
+}
+
+// Jacobian (X, Y, Z) as XYZZ (X, Y, Z^2, Z^3); Z = 0 is the identity.
+static host::Xyzz jacobian_to_xyzz(const uint64_t jac[18]) {
+  host::Xyzz p;
+  memcpy(p.X.l, jac, 48);
+  memcpy(p.Y.l, jac + 6, 48);
+  host::Fp Z;
+  memcpy(Z.l, jac + 12, 48);
+  if (host::fp_is_zero(Z)) return host::Xyzz::inf();
+  p.ZZ = host::fp_sqr(Z);
+  p.ZZZ = host::fp_mul(p.ZZ, Z);
+  return p;
+}
+
+// The doublings msm_serial (msm.rs:338-423) applies to its incoming accumulator: c per
+// window (:364-367), W = 8 max_byte / c + 1 windows (:359), c by the rule of :340-346 and
+// max_byte the highest non-zero byte over the canonical scalars; 0 when every scalar is
+// zero (the reference then returns before touching acc, :360-362).
+static uint64_t serial_doublings(const uint64_t* coeffs, size_t n) {
+  const size_t c = n < 4 ? 1 : (n < 32 ? 3 : (size_t)std::ceil(std::log((double)(uint32_t)n)));
+  host::Fr raw_one = host::fr_zero();
+  raw_one.l[0] = 1;
+  int max_byte = 0;
+  for (size_t i = 0; i < n && max_byte < 32; i++) {
+    const host::Fr s = host::fr_mul(host::fr_const(coeffs + 4 * i), raw_one);   // canonical
+    for (int b = 31; b >= max_byte; b--)
+      if ((s.l[b / 8] >> (8 * (b % 8))) & 0xff) {
+        max_byte = b + 1;
+        break;
+      }
+  }
+  return max_byte == 0 ? 0 : (uint64_t)c * ((size_t)max_byte * 8 / c + 1);
+}
+
+// msm_serial (msm.rs:338-423), acc Jacobian in and out: acc <- [2^d] acc + sum_i coeffs[i]
+// bases[i] with d = serial_doublings (d = 0 and acc unchanged for all-zero scalars).
+// From the identity -- the only way the reference itself calls it (msm.rs:430-455) --
+// acc becomes the MSM.
+int mzk_msm_serial(mzk_ctx* c, const uint64_t* coeffs, const uint64_t* bases, size_t n, uint64_t acc[18]) {
+  MZK_REQUIRE(acc != nullptr, "mzk_msm_serial: null accumulator");
+  uint64_t m[18];
+  MZK_TRY(msm_host(c, coeffs, bases, n, m));
+  host::Xyzz a = jacobian_to_xyzz(acc);
+  if (!a.is_inf()) {
+    const uint64_t d = serial_doublings(coeffs, n);
+    for (uint64_t i = 0; i < d; i++) a = host::xyzz_dbl(a);
+  }
+  host::xyzz_to_jacobian(host::xyzz_add(a, jacobian_to_xyzz(m)), acc);
+  return MZK_OK;
 }
 
 int mzk_g1_jacobian_to_affine(const uint64_t jac[18], uint64_t aff[12]) {

@@ -1,6 +1,6 @@
 // Radix-2 NTTs over the BLS12-381 scalar field for gfx950, behind the
-// EvaluationDomain boundary (proofs/src/poly/domain.rs:54-414) and best_fft
-// (curves/src/fft.rs:49-101, 110-186).
+// EvaluationDomain boundary (proofs/src/poly/domain.rs:54-414) and best_fft /
+// best_fft_with_twiddles / fft_coeff_to_extended (curves/src/fft.rs:31-186).
 //
 // A transform of size N = N1 * N2 (N1 = 2^ceil(logN/2), N2 = 2^floor(logN/2),
 // both <= 2^11) runs as two passes of the Bailey four-step method:
@@ -8,13 +8,25 @@
 //           twist omega_N^(n1 k2), stored at tmp[n1 + N1 k2];
 //   pass 2: for each k2 < N2, a size-N1 DFT of the row tmp[k2 N1 + n1], stored at
 //           out[k2 + N2 k1] -- natural order, so no bit-reversal pass exists.
-// Each sub-DFT is staged through LDS (up to 2^11 elements = 64 KiB per workgroup):
+// Above 2^22, N = A B C (each <= 2^11, input index nA + A nB + A B nC) runs as three
+// passes, the column DFT of size M = B C itself a four-step:
+//   pass 1: size-C DFTs over nC for each (nA, nB), twist omega_M^(nB kC), into tmp
+//           at the same positions (nC -> kC);
+//   pass 2: size-B DFTs over nB for each (nA, kC), in place in tmp (nB -> kB), twist
+//           omega_N^(nA kM) with kM = kC + C kB;
+//   pass 3: size-A DFTs over nA (contiguous rows of tmp) for each kM, stored at
+//           out[kM + M kA].
+// Each sub-DFT is staged through LDS (up to 2^11 elements = 72 KiB per workgroup):
 // loaded in bit-reversed order, log2(L) Cooley-Tukey stages, stored naturally.
 // The zeta-coset distribution and zero padding of coeff_to_extended are fused into
-// the pass-1 load; the 1/N scaling and zeta^-(i mod 3) of the inverse transforms
-// into the pass-2 store.  Results are the unique DFT values, hence identical to the
-// reference's bit-reverse + DIT and pruned-DIF schedules.
+// the first pass's load; the 1/N scaling and zeta^-(i mod 3) of the inverse transforms
+// into the last pass's store.  Results are the unique DFT values, hence identical to
+// the reference's bit-reverse + DIT and pruned-DIF schedules.
+#include <stdint.h>
+
+#include <algorithm>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "ctx.hpp"
@@ -26,33 +38,10 @@
 
 namespace mzk {
 
-static constexpr int NTT_MAX_LOG_L = 11;
 static constexpr int NTT_LDS_ELEMS = 1 << NTT_MAX_LOG_L;
 static constexpr int TWIST_LO_BITS = 11;
-static constexpr int TW29 = 12;   // words per radix-2^29 table record (9 used; 16-byte aligned)
+static constexpr int TW29 = NTT_TW29;
 static constexpr int ELEM29 = 9;  // words per element staged in LDS
-
-struct FrWords {
-  uint32_t v[8];
-};
-
-struct PassArgs {
-  const uint32_t* src;
-  uint32_t* dst;
-  const uint32_t* tw;        // omega_L^i, i < L/2 (radix-2^29 records of TW29 words)
-  const uint32_t* twist_lo;  // omega_N^i, i < 2^TWIST_LO_BITS        (pass 1 only; records)
-  const uint32_t* twist_hi;  // omega_N^(i 2^TWIST_LO_BITS)            (pass 1 only; records)
-  uint64_t in_bstride, in_jstride, out_bstride, out_kstride;
-  uint64_t n_real;           // input indices >= n_real read as zero
-  uint32_t log_l;            // sub-DFT size L = 2^log_l
-  uint32_t C;                // sub-DFTs per workgroup
-  int do_twist;
-  int in_coset;              // input index i times [1, zeta, zeta^2][i mod 3]
-  int out_mode;              // 0 none, 1 times scale, 2 times scale*[1, zeta^2, zeta][k mod 3]
-  int pairs;                 // stages two at a time in registers (large transforms)
-  FrWords zeta, zeta2;
-  FrWords post[3];           // scale * pattern, indexed by output index mod 3
-};
 
 __device__ __forceinline__ fr_t to_fr(const FrWords& w) {
   fr_t r;
@@ -162,7 +151,8 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block,
   }
   for (uint32_t e = threadIdx.x; e < CL; e += nt) {
     const uint32_t c = e % a.C, j = e / a.C;
-    const uint64_t gi = (b0 + c) * a.in_bstride + (uint64_t)j * a.in_jstride;
+    const uint64_t b = b0 + c;
+    const uint64_t gi = (b & a.in_mask) * a.in_lo + (b >> a.in_shift) * a.in_hi + (uint64_t)j * a.in_j;
     fr29 x;
     if (gi < a.n_real) {
       const uint4* q = reinterpret_cast<const uint4*>(a.src + 8 * gi);
@@ -271,14 +261,14 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block,
     const uint64_t b = b0 + c;
     fr29 x = load29(elem29(lds, c * L + k));
     if (a.do_twist) {
-      const uint64_t ex = b * k;   // < N
+      const uint64_t ex = (b & a.tw_mask) * ((b >> a.tw_shift) + ((uint64_t)k << a.tw_kshift));   // < N
       if (ex != 0) {
         const fr29 w = fr29_mul(load_rec29(a.twist_lo + TW29 * (ex & ((1u << TWIST_LO_BITS) - 1))),
                                 load_rec29(a.twist_hi + TW29 * (ex >> TWIST_LO_BITS)));
         x = fr29_mul(x, w);
       }
     }
-    const uint64_t go = b * a.out_bstride + (uint64_t)k * a.out_kstride;
+    const uint64_t go = (b & a.out_mask) * a.out_lo + (b >> a.out_shift) * a.out_hi + (uint64_t)k * a.out_k;
     if (a.out_mode == 1) {
       x = fr29_mul(x, p0);
     } else if (a.out_mode == 2) {
@@ -293,30 +283,34 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block,
   }
 }
 
-// std blst values -> table records (TW29 words: the value below r in the radix-2^29
-// form, then padding)
-__global__ void k_to_rec29(const uint32_t* __restrict__ in, uint32_t count, uint32_t* __restrict__ out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// out[i] = rec29(g^i) for i < count, with g^i = prod_{j : bit j of i} pows[j]
+// (pows[j] = g^(2^j)): table records of TW29 words, the value below r in the radix-2^29
+// form, then zero padding.
+struct PowArgs {
+  FrWords pows[32];
+};
+__global__ void k_pow_rec29(uint32_t* __restrict__ out, uint64_t count, PowArgs p) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
-  const fr29 v = fr29_reduce_full(fr29_from_std(in + 8 * (size_t)i));
-  uint32_t* o = out + TW29 * (size_t)i;
+  fr_t r = fr_t::one();
+  for (int j = 0; j < 32; j++)
+    if ((i >> j) & 1) r = mul(r, to_fr(p.pows[j]));
+  const fr29 v = fr29_reduce_full(fr29_from_std(r.v));
+  uint32_t* o = out + TW29 * i;
 #pragma unroll
   for (int k = 0; k < 9; k++) o[k] = v.l[k];
 #pragma unroll
   for (int k = 9; k < TW29; k++) o[k] = 0;
 }
 
-// out[i] = prod_{j : bit j of i} pows[j]  (pows[j] = g^(2^j)), i.e. g^i.
-struct PowArgs {
-  FrWords pows[32];
-};
-__global__ void k_pow_table(uint32_t* __restrict__ out, uint32_t count, PowArgs p) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// out[i] = g^i in blst words (compute_twiddles)
+__global__ void k_pow_table(uint32_t* __restrict__ out, uint64_t count, PowArgs p) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   fr_t r = fr_t::one();
   for (int j = 0; j < 32; j++)
     if ((i >> j) & 1) r = mul(r, to_fr(p.pows[j]));
-  store<FrP>(out + 8 * (size_t)i, r);
+  store<FrP>(out + 8 * i, r);
 }
 
 // a[i] *= t[i & mask]
@@ -333,14 +327,18 @@ static FrWords words(const host::Fr& f) {
   return w;
 }
 
-static int pow_table(hipStream_t st, uint32_t* out, uint32_t count, const host::Fr& g) {
+static PowArgs pow_args(const host::Fr& g) {
   PowArgs p;
   host::Fr x = g;
   for (int j = 0; j < 32; j++) {
     p.pows[j] = words(x);
     x = host::fr_mul(x, x);
   }
-  hipLaunchKernelGGL(k_pow_table, dim3(grid_for(count, 256)), dim3(256), 0, st, out, count, p);
+  return p;
+}
+
+static int pow_records(hipStream_t st, uint32_t* out, uint64_t count, const host::Fr& g) {
+  hipLaunchKernelGGL(k_pow_rec29, dim3(grid_for(count, 256)), dim3(256), 0, st, out, count, pow_args(g));
   MZK_HIP(hipGetLastError());
   return MZK_OK;
 }
@@ -349,54 +347,80 @@ NttPlan::~NttPlan() {
   if (mem) (void)hipFree(mem);
 }
 
-int NttPlan::init(hipStream_t st, uint32_t log_n_, const host::Fr& omega) {
+void NttPlan::split(uint32_t log_n, uint32_t* np_out, uint32_t lp_out[3]) {
+  lp_out[0] = lp_out[1] = lp_out[2] = 0;
+  if (log_n <= (uint32_t)NTT_MAX_LOG_L) {
+    *np_out = 1;
+    lp_out[0] = log_n;
+  } else if (log_n <= 2u * NTT_MAX_LOG_L) {
+    // rows (pass 2) of 2^l1, columns (pass 1) of 2^(log_n - l1)
+    uint32_t l1 = (log_n + 1) / 2;
+    const NttTune tn = tune_ntt_get(log_n);
+    if (tn.l1 && tn.l1 <= (uint32_t)NTT_MAX_LOG_L && log_n > tn.l1 && log_n - tn.l1 <= (uint32_t)NTT_MAX_LOG_L)
+      l1 = tn.l1;
+    *np_out = 2;
+    lp_out[0] = log_n - l1;
+    lp_out[1] = l1;
+  } else {
+    // A (rows, pass 3) >= B (pass 2) >= C (columns, pass 1), as even as the sizes allow
+    const uint32_t la = (log_n + 2) / 3;
+    const uint32_t lb = (log_n - la + 1) / 2;
+    *np_out = 3;
+    lp_out[0] = log_n - la - lb;
+    lp_out[1] = lb;
+    lp_out[2] = la;
+  }
+}
+
+size_t NttPlan::records(uint32_t log_n, uint32_t np_, const uint32_t lp_[3]) {
+  size_t r = 0;
+  for (uint32_t p = 0; p < np_; p++) r += std::max<size_t>(((size_t)1 << lp_[p]) / 2, 1);
+  if (np_ >= 2) r += ((size_t)1 << std::min<uint32_t>(TWIST_LO_BITS, log_n)) + std::max<size_t>(1, ((size_t)1 << log_n) >> TWIST_LO_BITS);
+  return r;
+}
+
+int NttPlan::init_at(hipStream_t st, uint32_t log_n_, const host::Fr& omega, uint32_t* tables, size_t cap_records) {
   log_n = log_n_;
-  l1 = (log_n + 1) / 2;
-  const NttTune tn = tune_ntt_get(log_n);
-  if (tn.l1 && tn.l1 <= (uint32_t)NTT_MAX_LOG_L && log_n > tn.l1 && log_n - tn.l1 <= (uint32_t)NTT_MAX_LOG_L)
-    l1 = tn.l1;
-  l2 = log_n - l1;
-  const uint32_t N1 = 1u << l1, N2 = 1u << l2;
-  const uint32_t lo_cnt = 1u << std::min<uint32_t>(TWIST_LO_BITS, log_n);
-  const uint32_t hi_cnt = (uint32_t)std::max<uint64_t>(1, ((uint64_t)1 << log_n) >> TWIST_LO_BITS);
-  const uint32_t c1 = std::max(N1 / 2, 1u), c2 = std::max(N2 / 2, 1u);
-  const size_t entries = (size_t)c1 + c2 + lo_cnt + hi_cnt;
-  // powers in blst words into a temporary, then the tables as radix-2^29 records
-  void* tmp = nullptr;
-  MZK_HIP(hipMalloc(&tmp, entries * 32));
-  if (hipMalloc(&mem, entries * TW29 * 4) != hipSuccess) {
+  split(log_n, &np, lp);
+  const size_t need = records(log_n, np, lp);
+  if (need > cap_records) {
+    set_error("ntt plan 2^%u: %zu table records needed, %zu available", log_n, need, cap_records);
+    return MZK_ERR_INVALID;
+  }
+  uint32_t* p = tables;
+  for (uint32_t q = 0; q < np; q++) {
+    // omega_{L_q} = omega^(N / L_q)
+    host::Fr w = omega;
+    for (uint32_t i = lp[q]; i < log_n; i++) w = host::fr_mul(w, w);
+    const size_t cnt = std::max<size_t>(((size_t)1 << lp[q]) / 2, 1);
+    MZK_TRY(pow_records(st, p, cnt, w));
+    tw[q] = p;
+    p += TW29 * cnt;
+  }
+  if (np >= 2) {
+    const size_t lo_cnt = (size_t)1 << std::min<uint32_t>(TWIST_LO_BITS, log_n);
+    const size_t hi_cnt = std::max<size_t>(1, ((size_t)1 << log_n) >> TWIST_LO_BITS);
+    host::Fr whi = omega;
+    for (int i = 0; i < TWIST_LO_BITS; i++) whi = host::fr_mul(whi, whi);
+    MZK_TRY(pow_records(st, p, lo_cnt, omega));
+    twist_lo = p;
+    p += TW29 * lo_cnt;
+    MZK_TRY(pow_records(st, p, hi_cnt, whi));
+    twist_hi = p;
+  }
+  return MZK_OK;
+}
+
+int NttPlan::init(hipStream_t st, uint32_t log_n_, const host::Fr& omega) {
+  uint32_t np_, lp_[3];
+  split(log_n_, &np_, lp_);
+  const size_t recs = records(log_n_, np_, lp_);
+  if (hipMalloc(&mem, recs * TW29 * 4) != hipSuccess) {
     mem = nullptr;
-    (void)hipFree(tmp);
-    set_error("ntt plan 2^%u: out of device memory for twiddle tables", log_n);
+    set_error("ntt plan 2^%u: out of device memory for twiddle tables", log_n_);
     return MZK_ERR_NOMEM;
   }
-  uint32_t* t = (uint32_t*)tmp;
-  uint32_t* p = (uint32_t*)mem;
-  // omega_N1 = omega^N2, omega_N2 = omega^N1
-  host::Fr w1 = omega, w2 = omega;
-  for (uint32_t i = 0; i < l2; i++) w1 = host::fr_mul(w1, w1);
-  for (uint32_t i = 0; i < l1; i++) w2 = host::fr_mul(w2, w2);
-  host::Fr whi = omega;
-  for (int i = 0; i < TWIST_LO_BITS; i++) whi = host::fr_mul(whi, whi);
-  int rc = pow_table(st, t, c1, w1);
-  if (rc == MZK_OK) rc = pow_table(st, t + 8 * (size_t)c1, c2, w2);
-  if (rc == MZK_OK) rc = pow_table(st, t + 8 * (size_t)(c1 + c2), lo_cnt, omega);
-  if (rc == MZK_OK) rc = pow_table(st, t + 8 * (size_t)(c1 + c2 + lo_cnt), hi_cnt, whi);
-  if (rc == MZK_OK) {
-    hipLaunchKernelGGL(k_to_rec29, dim3(grid_for(entries, 256)), dim3(256), 0, st, t, (uint32_t)entries, p);
-    if (hipGetLastError() != hipSuccess) rc = MZK_ERR_HIP;
-  }
-  if (hipStreamSynchronize(st) != hipSuccess && rc == MZK_OK) rc = MZK_ERR_HIP;
-  (void)hipFree(tmp);
-  if (rc != MZK_OK) {
-    set_error("ntt plan 2^%u: twiddle tables failed", log_n);
-    return rc;
-  }
-  tw1 = p;
-  tw2 = p + TW29 * (size_t)c1;
-  twist_lo = p + TW29 * (size_t)(c1 + c2);
-  twist_hi = p + TW29 * (size_t)(c1 + c2 + lo_cnt);
-  return MZK_OK;
+  return init_at(st, log_n_, omega, (uint32_t*)mem, recs);
 }
 
 // 2^11 staged elements of 9 words take 72 KiB of LDS, above the 64 KiB a kernel gets
@@ -425,21 +449,47 @@ static uint32_t choose_c(uint32_t L, uint64_t nbatch, uint32_t groups) {
   return std::max(c, 1u);
 }
 
-static int launch_pass(mzk_ctx* c, const char* name, PassArgs a, uint64_t nbatch) {
+static uint32_t pass_threads(const PassArgs& a) {
   const uint32_t L = 1u << a.log_l;
-  const uint32_t threads = std::min<uint32_t>(NTT_THREADS, std::max<uint32_t>(64, a.C * L / 2));
-  const size_t lds = ntt_lds_bytes((size_t)a.C * L);
+  return std::min<uint32_t>(NTT_THREADS, std::max<uint32_t>(64, a.C * L / 2));
+}
+
+static int launch_pass(mzk_ctx* c, const char* name, const PassArgs& a, uint64_t groups) {
+  const size_t lds = ntt_lds_bytes((size_t)a.C << a.log_l);
   MZK_TRY(ntt_lds_attr());
-  MZK_LAUNCH(c, name, k_ntt_pass, dim3((unsigned)(nbatch / a.C)), dim3(threads), lds, a);
+  MZK_LAUNCH(c, name, k_ntt_pass, dim3((unsigned)groups), dim3(pass_threads(a)), lds, a);
   MZK_HIP(hipGetLastError());
   return MZK_OK;
 }
 
-// The passes of one transform: 1 (N <= 2^11, straight from src to dst) or 2 (through
-// tmp, which must hold N elements).  Returns the pass count; groups[i] = workgroups.
-static int build_passes(const NttPlan& plan, const uint32_t* src, uint32_t* dst, uint32_t* tmp, uint64_t n_real,
-                        int in_coset, int out_mode, const host::Fr& scale, uint64_t launch_elems, PassArgs pa[2],
-                        uint64_t groups[2]) {
+int ntt_launch_pass_raw(hipStream_t st, const PassArgs& a, uint64_t groups) {
+  const size_t lds = ntt_lds_bytes((size_t)a.C << a.log_l);
+  MZK_TRY(ntt_lds_attr());
+  hipLaunchKernelGGL(k_ntt_pass, dim3((unsigned)groups), dim3(pass_threads(a)), lds, st, a);
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
+}
+
+// contiguous batches: sub-DFT b starts at b * lo (the masks keep all of b, the shifts none)
+static void linear_addressing(PassArgs& a, uint64_t in_lo, uint64_t in_j, uint64_t out_lo, uint64_t out_k) {
+  a.in_mask = ~0ull;
+  a.in_shift = 63;
+  a.in_hi = 0;
+  a.in_lo = in_lo;
+  a.in_j = in_j;
+  a.out_mask = ~0ull;
+  a.out_shift = 63;
+  a.out_hi = 0;
+  a.out_lo = out_lo;
+  a.out_k = out_k;
+  a.tw_mask = ~0ull;
+  a.tw_shift = 63;
+  a.tw_kshift = 0;
+}
+
+int ntt_build_passes(const NttPlan& plan, const uint32_t* src, uint32_t* dst, uint32_t* tmp, uint64_t n_real,
+                     int in_coset, int out_mode, const host::Fr& scale, uint64_t launch_elems, PassArgs pa[3],
+                     uint64_t groups[3]) {
   const uint32_t log_n = plan.log_n;
   const uint64_t N = 1ull << log_n;
   host::Fr z = host::fr_const(host::kFrZeta);
@@ -451,17 +501,15 @@ static int build_passes(const NttPlan& plan, const uint32_t* src, uint32_t* dst,
   a.post[0] = words(scale);
   a.post[1] = words(host::fr_mul(scale, z2));
   a.post[2] = words(host::fr_mul(scale, z));
-  a.n_real = n_real;
-  a.in_coset = in_coset;
   a.pairs = log_n >= NTT_PAIRS_MIN_LOG && launch_elems >= (1ull << NTT_PAIRS_MIN_LAUNCH_LOG) ? 1 : 0;
-  if (log_n <= (uint32_t)NTT_MAX_LOG_L) {
+  const uint32_t groups_target = plan.np == 2 ? tune_ntt_get(log_n).groups : 0;
+  if (plan.np == 1) {
     a.src = src;
     a.dst = dst;
-    a.tw = plan.twist_lo;   // omega_N^i, i < N/2 is the full sub-DFT table here
-    a.in_bstride = 0;
-    a.in_jstride = 1;
-    a.out_bstride = 0;
-    a.out_kstride = 1;
+    a.tw = plan.tw[0];
+    linear_addressing(a, 0, 1, 0, 1);
+    a.n_real = n_real;
+    a.in_coset = in_coset;
     a.log_l = log_n;
     a.C = 1;
     a.out_mode = out_mode;
@@ -469,75 +517,156 @@ static int build_passes(const NttPlan& plan, const uint32_t* src, uint32_t* dst,
     groups[0] = 1;
     return 1;
   }
-  const uint64_t N1 = 1ull << plan.l1, N2 = 1ull << plan.l2;
-  // pass 1: columns (size N2), twist, out-of-place into tmp
+  if (plan.np == 2) {
+    const uint64_t N1 = 1ull << plan.lp[1], N2 = 1ull << plan.lp[0];
+    // pass 1: columns (size N2), twist omega_N^(n1 k2), out-of-place into tmp
+    a.src = src;
+    a.dst = tmp;
+    a.tw = plan.tw[0];
+    a.twist_lo = plan.twist_lo;
+    a.twist_hi = plan.twist_hi;
+    a.do_twist = 1;
+    linear_addressing(a, 1, N1, 1, N1);
+    a.n_real = n_real;
+    a.in_coset = in_coset;
+    a.log_l = plan.lp[0];
+    a.C = choose_c((uint32_t)N2, N1, groups_target);
+    a.out_mode = 0;
+    pa[0] = a;
+    groups[0] = N1 / a.C;
+    // pass 2: rows (size N1), natural-order output
+    PassArgs b = a;
+    b.src = tmp;
+    b.dst = dst;
+    b.tw = plan.tw[1];
+    b.do_twist = 0;
+    b.in_coset = 0;
+    b.n_real = N;
+    linear_addressing(b, N1, 1, 1, N2);
+    b.log_l = plan.lp[1];
+    b.C = choose_c((uint32_t)N1, N2, groups_target);
+    b.out_mode = out_mode;
+    pa[1] = b;
+    groups[1] = N2 / b.C;
+    return 2;
+  }
+  // three passes: N = A B C, input index nA + A nB + A B nC
+  const uint32_t lc = plan.lp[0], lb = plan.lp[1], la = plan.lp[2];
+  const uint64_t A = 1ull << la, B = 1ull << lb, Cs = 1ull << lc, AB = A * B, M = B * Cs;
+  // pass 1: size-C DFTs over nC, sub-DFT b = nA + A nB; twist omega_M^(nB kC) =
+  // omega_N^(A nB kC), i.e. e = (b & ~(A - 1)) k; out to tmp at the same positions
   a.src = src;
   a.dst = tmp;
-  a.tw = plan.tw2;
+  a.tw = plan.tw[0];
   a.twist_lo = plan.twist_lo;
   a.twist_hi = plan.twist_hi;
   a.do_twist = 1;
-  a.in_bstride = 1;
-  a.in_jstride = N1;
-  a.out_bstride = 1;
-  a.out_kstride = N1;
-  a.log_l = plan.l2;
-  const uint32_t groups_target = tune_ntt_get(log_n).groups;
-  a.C = choose_c((uint32_t)N2, N1, groups_target);
+  linear_addressing(a, 1, AB, 1, AB);
+  a.tw_mask = ~(A - 1);
+  a.n_real = n_real;
+  a.in_coset = in_coset;
+  a.log_l = lc;
+  a.C = choose_c((uint32_t)Cs, AB, 0);
   a.out_mode = 0;
   pa[0] = a;
-  groups[0] = N1 / a.C;
-  // pass 2: rows (size N1), natural-order output
+  groups[0] = AB / a.C;
+  // pass 2: size-B DFTs over nB, sub-DFT b = nA + A kC at nA + A B kC + A j, in place;
+  // twist omega_N^(nA (kC + C kB)), e = (b & (A - 1)) ((b >> la) + (k << lc))
   PassArgs b = a;
   b.src = tmp;
-  b.dst = dst;
-  b.tw = plan.tw1;
-  b.do_twist = 0;
-  b.in_coset = 0;
+  b.dst = tmp;
+  b.tw = plan.tw[1];
+  b.in_mask = A - 1;
+  b.in_lo = 1;
+  b.in_shift = la;
+  b.in_hi = AB;
+  b.in_j = A;
+  b.out_mask = A - 1;
+  b.out_lo = 1;
+  b.out_shift = la;
+  b.out_hi = AB;
+  b.out_k = A;
+  b.tw_mask = A - 1;
+  b.tw_shift = la;
+  b.tw_kshift = lc;
   b.n_real = N;
-  b.in_bstride = N1;
-  b.in_jstride = 1;
-  b.out_bstride = 1;
-  b.out_kstride = N2;
-  b.log_l = plan.l1;
-  b.C = choose_c((uint32_t)N1, N2, groups_target);
-  b.out_mode = out_mode;
+  b.in_coset = 0;
+  b.log_l = lb;
+  b.C = choose_c((uint32_t)B, A * Cs, 0);
   pa[1] = b;
-  groups[1] = N2 / b.C;
-  return 2;
+  groups[1] = A * Cs / b.C;
+  // pass 3: size-A DFTs over nA, sub-DFT b = kM = kC + C kB, a contiguous row of tmp at
+  // A B kC + A kB; natural-order output out[kM + M kA]
+  PassArgs d = b;
+  d.src = tmp;
+  d.dst = dst;
+  d.tw = plan.tw[2];
+  d.do_twist = 0;
+  linear_addressing(d, 0, 1, 1, M);
+  d.in_mask = Cs - 1;
+  d.in_lo = AB;
+  d.in_shift = lc;
+  d.in_hi = A;
+  d.log_l = la;
+  d.C = choose_c((uint32_t)A, M, 0);
+  d.out_mode = out_mode;
+  pa[2] = d;
+  groups[2] = M / d.C;
+  return 3;
+}
+
+static const char* pass_name(int np, int i) {
+  static const char* names[6] = {"k_ntt_pass_single", "k_ntt_pass1", "k_ntt_pass2",
+                                 "k_ntt_pass3a", "k_ntt_pass3b", "k_ntt_pass3c"};
+  return np == 1 ? names[0] : (np == 2 ? names[1 + i] : names[3 + i]);
 }
 
 int ntt_run(mzk_ctx* c, const NttPlan& plan, const uint32_t* src, uint32_t* dst, uint64_t n_real, int in_coset,
             int out_mode, const host::Fr& scale) {
   const uint64_t N = 1ull << plan.log_n;
   uint32_t* tmp = nullptr;
-  if (plan.log_n > (uint32_t)NTT_MAX_LOG_L) {
+  if (plan.np > 1) {
     MZK_TRY(c->scratch[SCR_NTT].reserve(N * 32));
     tmp = (uint32_t*)c->scratch[SCR_NTT].ptr;
   }
-  PassArgs pa[2];
-  uint64_t groups[2];
-  const int np = build_passes(plan, src, dst, tmp, n_real, in_coset, out_mode, scale, N, pa, groups);
-  static const char* names[3] = {"k_ntt_pass_single", "k_ntt_pass1", "k_ntt_pass2"};
-  for (int i = 0; i < np; i++) MZK_TRY(launch_pass(c, names[np == 1 ? 0 : 1 + i], pa[i], groups[i] * pa[i].C));
+  PassArgs pa[3];
+  uint64_t groups[3];
+  const int np = ntt_build_passes(plan, src, dst, tmp, n_real, in_coset, out_mode, scale, N, pa, groups);
+  for (int i = 0; i < np; i++) MZK_TRY(launch_pass(c, pass_name(np, i), pa[i], groups[i]));
   return MZK_OK;
 }
 
-// Cached plan for (log_n, inverse) on a context: omega = ROOT^(2^(32 - log_n)) or its inverse.
-const NttPlan* ctx_plan(mzk_ctx* c, uint32_t log_n, int inverse) {
-  // keyed by the split too: a plan made before a tuning run keeps its own tables
-  const int key = ((int)tune_ntt_get(log_n).l1 << 8) | ((int)log_n * 2 + (inverse ? 1 : 0));
-  auto it = c->ntt_plans.find(key);
+// Cached plans on a context, keyed by size, split and root.  Trimmed only at the start
+// of a call (ntt_plans_trim), so the plans a call looked up stay valid until it returns.
+static constexpr size_t MAX_CTX_PLANS = 64;
+
+static void ntt_plans_trim(mzk_ctx* c) {
+  if (c->ntt_plans.size() <= MAX_CTX_PLANS) return;
+  (void)hipStreamSynchronize(c->stream);   // queued passes may still read the tables
+  free_ntt_plans(c);
+}
+
+static const NttPlan* ctx_plan_root(mzk_ctx* c, uint32_t log_n, const host::Fr& omega) {
+  uint32_t np, lp[3];
+  NttPlan::split(log_n, &np, lp);
+  GraphKey k;
+  k.add(log_n).add(lp[0]).add(lp[1]).add(lp[2]).add(omega.l);
+  auto it = c->ntt_plans.find(k.s);
   if (it != c->ntt_plans.end()) return it->second;
-  host::Fr w = host::fr_root_of_unity(log_n);
-  if (inverse) w = host::fr_inv(w);
   NttPlan* p = new NttPlan();
-  if (p->init(c->stream, log_n, w) != MZK_OK) {
+  if (p->init(c->stream, log_n, omega) != MZK_OK) {
     delete p;
     return nullptr;
   }
-  c->ntt_plans[key] = p;
+  c->ntt_plans[k.s] = p;
   return p;
+}
+
+// omega = ROOT^(2^(32 - log_n)) or its inverse
+const NttPlan* ctx_plan(mzk_ctx* c, uint32_t log_n, int inverse) {
+  host::Fr w = host::fr_root_of_unity(log_n);
+  if (inverse) w = host::fr_inv(w);
+  return ctx_plan_root(c, log_n, w);
 }
 
 void free_ntt_plans(mzk_ctx* c) {
@@ -545,9 +674,11 @@ void free_ntt_plans(mzk_ctx* c) {
   c->ntt_plans.clear();
 }
 
-// Runs many transforms with two launches in all: every job's first pass, then
-// every two-pass job's second pass (each workgroup finds its job in a table).
+// Runs many transforms with up to three launches in all: every job's first pass, then
+// every multi-pass job's second pass, then every three-pass job's third pass (each
+// workgroup finds its job in a table).
 int ntt_run_batch(mzk_ctx* c, const mzk_ntt_job* jobs, size_t njobs) {
+  ntt_plans_trim(c);
   size_t tmp_elems = 0, all_elems = 0;
   for (size_t j = 0; j < njobs; j++) {
     if (jobs[j].log_n > (uint32_t)NTT_MAX_LOG_L) tmp_elems += 1ull << jobs[j].log_n;
@@ -555,8 +686,9 @@ int ntt_run_batch(mzk_ctx* c, const mzk_ntt_job* jobs, size_t njobs) {
   }
   MZK_TRY(c->scratch[SCR_NTT].reserve(tmp_elems * 32 + 64));
   uint32_t* tmp = (uint32_t*)c->scratch[SCR_NTT].ptr;
-  std::vector<PassArgs> t1, t2;
-  std::vector<uint32_t> f1(1, 0), f2(1, 0);
+  std::vector<PassArgs> t[3];
+  std::vector<uint32_t> f[3];
+  for (int p = 0; p < 3; p++) f[p].push_back(0);
   for (size_t j = 0; j < njobs; j++) {
     const mzk_ntt_job& jb = jobs[j];
     const int inverse = (jb.flags & MZK_NTT_INVERSE) != 0;
@@ -565,21 +697,21 @@ int ntt_run_batch(mzk_ctx* c, const mzk_ntt_job* jobs, size_t njobs) {
     MZK_REQUIRE(plan != nullptr, "ntt batch: plan for 2^%u failed", jb.log_n);
     const uint64_t N = 1ull << jb.log_n;
     host::Fr scale = inverse ? host::fr_inv(host::fr_from_u64(N)) : host::fr_one();
-    PassArgs pa[2];
-    uint64_t groups[2];
-    const int np = build_passes(*plan, (const uint32_t*)jb.d_in, (uint32_t*)jb.d_out, tmp,
-                                inverse ? N : jb.n_real, inverse ? 0 : coset,
-                                inverse ? (coset ? 2 : 1) : 0, scale, all_elems, pa, groups);
-    if (np == 2) tmp += 8 * N;
-    t1.push_back(pa[0]);
-    f1.push_back(f1.back() + (uint32_t)groups[0]);
-    if (np == 2) {
-      t2.push_back(pa[1]);
-      f2.push_back(f2.back() + (uint32_t)groups[1]);
+    PassArgs pa[3];
+    uint64_t groups[3];
+    const int np = ntt_build_passes(*plan, (const uint32_t*)jb.d_in, (uint32_t*)jb.d_out, tmp,
+                                    inverse ? N : jb.n_real, inverse ? 0 : coset, inverse ? (coset ? 2 : 1) : 0,
+                                    scale, all_elems, pa, groups);
+    if (np > 1) tmp += 8 * N;
+    for (int p = 0; p < np; p++) {
+      MZK_REQUIRE((uint64_t)f[p].back() + groups[p] < (1ull << 31), "ntt batch: too many workgroups");
+      t[p].push_back(pa[p]);
+      f[p].push_back(f[p].back() + (uint32_t)groups[p]);
     }
   }
   // tables: PassArgs and first-workgroup arrays, copied once into the batch scratch
-  const size_t bytes = (t1.size() + t2.size()) * sizeof(PassArgs) + (f1.size() + f2.size()) * 4 + 64;
+  size_t bytes = 64;
+  for (int p = 0; p < 3; p++) bytes += t[p].size() * sizeof(PassArgs) + f[p].size() * 4 + 32;
   MZK_TRY(c->scratch[SCR_IN1].reserve(bytes));
   char* d = (char*)c->scratch[SCR_IN1].ptr;
   std::vector<char> h(bytes);
@@ -590,20 +722,28 @@ int ntt_run_batch(mzk_ctx* c, const mzk_ntt_job* jobs, size_t njobs) {
     o = (o + n + 15) & ~(size_t)15;
     return d + at;
   };
-  PassArgs* d_t1 = (PassArgs*)put(t1.data(), t1.size() * sizeof(PassArgs));
-  PassArgs* d_t2 = (PassArgs*)put(t2.data(), t2.size() * sizeof(PassArgs));
-  uint32_t* d_f1 = (uint32_t*)put(f1.data(), f1.size() * 4);
-  uint32_t* d_f2 = (uint32_t*)put(f2.data(), f2.size() * 4);
+  PassArgs* d_t[3];
+  uint32_t* d_f[3];
+  for (int p = 0; p < 3; p++) {
+    d_t[p] = (PassArgs*)put(t[p].data(), t[p].size() * sizeof(PassArgs));
+    d_f[p] = (uint32_t*)put(f[p].data(), f[p].size() * 4);
+  }
   MZK_HIP(hipMemcpyAsync(d, h.data(), o, hipMemcpyHostToDevice, c->stream));
   const size_t lds = ntt_lds_bytes(NTT_LDS_ELEMS);
   MZK_TRY(ntt_lds_attr());
-  // the two launches as one cached graph (the pass tables are copied before it runs)
-  const uint32_t n1 = (uint32_t)t1.size(), n2 = (uint32_t)t2.size(), g1 = f1.back(), g2 = f2.back();
+  // the launches as one cached graph (the pass tables are copied before it runs)
+  uint32_t n[3], g[3];
   GraphKey gk;
-  gk.add('N').add(d_t1).add(d_f1).add(n1).add(g1).add(d_t2).add(d_f2).add(n2).add(g2).add(lds);
+  gk.add('N').add(lds);
+  for (int p = 0; p < 3; p++) {
+    n[p] = (uint32_t)t[p].size();
+    g[p] = f[p].back();
+    gk.add(d_t[p]).add(d_f[p]).add(n[p]).add(g[p]);
+  }
+  static const char* names[3] = {"k_ntt_pass_batch1", "k_ntt_pass_batch2", "k_ntt_pass_batch3"};
   return run_graph(c, gk.s, [&]() -> int {
-    if (n1) MZK_LAUNCH(c, "k_ntt_pass_batch1", k_ntt_pass_batch, dim3(g1), dim3(NTT_THREADS), lds, d_t1, d_f1, n1);
-    if (n2) MZK_LAUNCH(c, "k_ntt_pass_batch2", k_ntt_pass_batch, dim3(g2), dim3(NTT_THREADS), lds, d_t2, d_f2, n2);
+    for (int p = 0; p < 3; p++)
+      if (n[p]) MZK_LAUNCH(c, names[p], k_ntt_pass_batch, dim3(g[p]), dim3(NTT_THREADS), lds, d_t[p], d_f[p], n[p]);
     MZK_HIP(hipGetLastError());
     return MZK_OK;
   });
@@ -639,11 +779,7 @@ int mzk_domain_create(mzk_ctx* c, uint32_t j, uint32_t k, mzk_domain** out) {
   const uint64_t qd = (uint64_t)(j > 1 ? j - 1 : 0);
   uint32_t ek = k;
   while (ek < 40 && (1ull << ek) < n * qd) ek++;
-  MZK_REQUIRE(ek <= 32, "mzk_domain_create: extended_k %u exceeds S = 32", ek);
-  if (ek > 2 * NTT_MAX_LOG_L) {
-    set_error("mzk_domain_create: extended_k %u above the supported 2^%d", ek, 2 * NTT_MAX_LOG_L);
-    return MZK_ERR_UNSUPPORTED;
-  }
+  MZK_REQUIRE(ek <= (uint32_t)NTT_MAX_LOG_N, "mzk_domain_create: extended_k %u exceeds S = 32", ek);
   std::lock_guard<std::mutex> g(c->mu);
   MZK_HIP(hipSetDevice(c->device));
   mzk_domain* d = new mzk_domain();
@@ -781,15 +917,64 @@ int mzk_coeff_to_extended(mzk_domain* d, const uint64_t* in, uint64_t* out) {
 
 static int check_ntt_job(const mzk_ntt_job& j, size_t i) {
   MZK_REQUIRE(j.d_in != nullptr && j.d_out != nullptr, "ntt job %zu: null data", i);
-  MZK_REQUIRE(j.log_n >= 1 && j.log_n <= (uint32_t)(2 * NTT_MAX_LOG_L), "ntt job %zu: log_n %u outside [1, %d]", i,
-              j.log_n, 2 * NTT_MAX_LOG_L);
+  MZK_REQUIRE(j.log_n >= 1 && j.log_n <= (uint32_t)NTT_MAX_LOG_N, "ntt job %zu: log_n %u outside [1, %d]", i,
+              j.log_n, NTT_MAX_LOG_N);
   MZK_REQUIRE((j.flags & ~(uint32_t)(MZK_NTT_INVERSE | MZK_NTT_COSET)) == 0, "ntt job %zu: unknown flags", i);
   MZK_REQUIRE(j.n_real <= (1ull << j.log_n), "ntt job %zu: n_real above 2^log_n", i);
   MZK_REQUIRE(!(j.flags & MZK_NTT_INVERSE) || j.n_real == (1ull << j.log_n),
               "ntt job %zu: inverse transforms take 2^log_n inputs", i);
-  // the two-pass form reads all input before writing dst only through tmp; a coset or
-  // padded forward transform with d_in == d_out is still correct because pass 1 reads
-  // d_in and writes tmp, and pass 2 writes d_out after pass 1 completes
+  // in place (d_in == d_out) is fine: a multi-pass transform reads d_in only in its first
+  // pass and writes d_out only in its last; a single pass stages all of a sub-DFT in LDS
+  return MZK_OK;
+}
+
+// byte ranges of a job's input and output
+static void job_spans(const mzk_ntt_job& j, uintptr_t in[2], uintptr_t out[2]) {
+  const uint64_t n_in = (j.flags & MZK_NTT_INVERSE) ? (1ull << j.log_n) : j.n_real;
+  in[0] = (uintptr_t)j.d_in;
+  in[1] = in[0] + 32 * n_in;
+  out[0] = (uintptr_t)j.d_out;
+  out[1] = out[0] + 32 * (1ull << j.log_n);
+}
+
+static bool spans_meet(const uintptr_t a[2], const uintptr_t b[2]) { return a[0] < b[1] && b[0] < a[1]; }
+
+// A batch runs every job's first pass in one launch (single-pass jobs write their outputs
+// there) and its later passes in later launches, so no job's output may overlap another
+// job's input or output; a job's own d_in == d_out is allowed.  Sorted sweep: O(n log n).
+static int check_batch_overlaps(const mzk_ntt_job* jobs, size_t njobs) {
+  struct Span {
+    uintptr_t lo, hi;
+    size_t job;
+    bool out;
+  };
+  std::vector<Span> v;
+  v.reserve(2 * njobs);
+  for (size_t i = 0; i < njobs; i++) {
+    uintptr_t in[2], out[2];
+    job_spans(jobs[i], in, out);
+    MZK_REQUIRE(in[0] == out[0] || !spans_meet(in, out), "ntt batch: job %zu's input and output overlap", i);
+    if (in[1] > in[0] && in[0] != out[0]) v.push_back({in[0], in[1], i, false});
+    v.push_back({out[0], out[1], i, true});
+  }
+  std::sort(v.begin(), v.end(), [](const Span& x, const Span& y) { return x.lo < y.lo; });
+  // every pair of overlapping spans meets some span that starts earlier and is still open;
+  // track the open span reaching furthest (and, if it is an input, the furthest output)
+  size_t far = SIZE_MAX, far_out = SIZE_MAX;
+  for (size_t k = 0; k < v.size(); k++) {
+    const Span& s = v[k];
+    auto clash = [&](size_t o) {
+      return o != SIZE_MAX && v[o].hi > s.lo && v[o].job != s.job && (v[o].out || s.out);
+    };
+    if (clash(far) || clash(far_out)) {
+      const Span& o = clash(far) ? v[far] : v[far_out];
+      set_error("ntt batch: job %zu's %s overlaps job %zu's %s", s.job, s.out ? "output" : "input", o.job,
+                o.out ? "output" : "input");
+      return MZK_ERR_INVALID;
+    }
+    if (far == SIZE_MAX || s.hi > v[far].hi) far = k;
+    if (s.out && (far_out == SIZE_MAX || s.hi > v[far_out].hi)) far_out = k;
+  }
   return MZK_OK;
 }
 
@@ -798,6 +983,7 @@ int mzk_ntt_dev(mzk_ctx* c, const mzk_ntt_job* job) {
   MZK_TRY(check_ntt_job(*job, 0));
   std::lock_guard<std::mutex> g(c->mu);
   MZK_HIP(hipSetDevice(c->device));
+  ntt_plans_trim(c);
   const int inverse = (job->flags & MZK_NTT_INVERSE) != 0, coset = (job->flags & MZK_NTT_COSET) != 0;
   const NttPlan* plan = ctx_plan(c, job->log_n, inverse);
   MZK_REQUIRE(plan != nullptr, "mzk_ntt_dev: plan for 2^%u failed", job->log_n);
@@ -811,34 +997,127 @@ int mzk_ntt_dev(mzk_ctx* c, const mzk_ntt_job* job) {
 int mzk_ntt_batch_dev(mzk_ctx* c, const mzk_ntt_job* jobs, size_t njobs) {
   MZK_REQUIRE(c != nullptr && (jobs != nullptr || njobs == 0), "mzk_ntt_batch_dev: null argument");
   for (size_t i = 0; i < njobs; i++) MZK_TRY(check_ntt_job(jobs[i], i));
+  MZK_TRY(check_batch_overlaps(jobs, njobs));
   if (njobs == 0) return MZK_OK;
   std::lock_guard<std::mutex> g(c->mu);
   MZK_HIP(hipSetDevice(c->device));
   return ntt_run_batch(c, jobs, njobs);
 }
 
-// best_fft (fft.rs:49): DFT with a caller-supplied omega, in place on host memory.
-int mzk_best_fft(mzk_ctx* c, uint64_t* a, const uint64_t omega[4], uint32_t log_n) {
-  MZK_REQUIRE(c != nullptr && a != nullptr && omega != nullptr, "mzk_best_fft: null argument");
-  if (log_n > (uint32_t)(2 * NTT_MAX_LOG_L)) {
-    set_error("mzk_best_fft: log_n %u above the supported 2^%d", log_n, 2 * NTT_MAX_LOG_L);
-    return MZK_ERR_UNSUPPORTED;
+// The root a twiddle table [1, w, w^2, ..., w^(n/2 - 1)] (compute_twiddles, fft.rs:31-40)
+// was built from: w = twiddles[1] (n >= 4), -1 for n = 2.  The table is checked where
+// it is cheap: twiddles[0] = 1, w^(n/2) = -1 (the radix-2 butterflies of fft.rs:56-101
+// compute the DFT by w exactly when this holds) and the last entry = w^(n/2 - 1).
+static int root_from_twiddles(const uint64_t* tw, uint32_t log_n, host::Fr* w, const char* who) {
+  MZK_REQUIRE(tw != nullptr, "%s: null twiddles", who);
+  const host::Fr one = host::fr_one();
+  const host::Fr minus_one = host::fr_sub(host::fr_zero(), one);
+  host::Fr t0;
+  memcpy(t0.l, tw, 32);
+  MZK_REQUIRE(host::fr_eq(t0, one), "%s: twiddles[0] is not 1", who);
+  if (log_n == 1) {
+    *w = minus_one;
+    return MZK_OK;
   }
+  memcpy(w->l, tw + 4, 32);
+  host::Fr h = *w;
+  for (uint32_t i = 1; i < log_n; i++) h = host::fr_mul(h, h);   // w^(n/2)
+  MZK_REQUIRE(host::fr_eq(h, minus_one), "%s: twiddles[1] is not a primitive 2^%u-th root of unity", who, log_n);
+  const uint64_t last = (1ull << (log_n - 1)) - 1;
+  host::Fr tl;
+  memcpy(tl.l, tw + 4 * last, 32);
+  MZK_REQUIRE(host::fr_eq(tl, host::fr_pow(*w, last)), "%s: twiddles[%llu] is not twiddles[1]^%llu", who,
+              (unsigned long long)last, (unsigned long long)last);
+  return MZK_OK;
+}
+
+// DFT by omega of the first n_real host elements of a (zero beyond), in place, all of
+// 2^log_n elements written back.
+static int host_fft(mzk_ctx* c, uint64_t* a, const host::Fr& omega, uint32_t log_n, uint64_t n_real) {
   const size_t N = 1ull << log_n;
   std::lock_guard<std::mutex> g(c->mu);
   MZK_HIP(hipSetDevice(c->device));
-  if (log_n == 0) return MZK_OK;   // a DFT of length 1 is the identity
-  NttPlan plan;
-  host::Fr om;
-  memcpy(om.l, omega, 32);
-  MZK_TRY(plan.init(c->stream, log_n, om));
+  ntt_plans_trim(c);
+  const NttPlan* plan = ctx_plan_root(c, log_n, omega);
+  MZK_REQUIRE(plan != nullptr, "fft: plan for 2^%u failed", log_n);
   MZK_TRY(c->scratch[SCR_IN0].reserve(N * 32));
-  MZK_HIP(hipMemcpyAsync(c->scratch[SCR_IN0].ptr, a, N * 32, hipMemcpyHostToDevice, c->stream));
-  MZK_TRY(ntt_run(c, plan, (const uint32_t*)c->scratch[SCR_IN0].ptr, (uint32_t*)c->scratch[SCR_IN0].ptr, N, 0, 0,
-                  host::fr_one()));
-  MZK_HIP(hipMemcpyAsync(a, c->scratch[SCR_IN0].ptr, N * 32, hipMemcpyDeviceToHost, c->stream));
+  uint32_t* d = (uint32_t*)c->scratch[SCR_IN0].ptr;
+  if (n_real) MZK_HIP(hipMemcpyAsync(d, a, n_real * 32, hipMemcpyHostToDevice, c->stream));
+  MZK_TRY(ntt_run(c, *plan, d, d, n_real, 0, 0, host::fr_one()));
+  MZK_HIP(hipMemcpyAsync(a, d, N * 32, hipMemcpyDeviceToHost, c->stream));
   MZK_HIP(hipStreamSynchronize(c->stream));
   return MZK_OK;
+}
+
+// best_fft (fft.rs:49): DFT with a caller-supplied omega, in place on host memory.
+int mzk_best_fft(mzk_ctx* c, uint64_t* a, const uint64_t omega[4], uint32_t log_n) {
+  MZK_REQUIRE(c != nullptr && a != nullptr && omega != nullptr, "mzk_best_fft: null argument");
+  MZK_REQUIRE(log_n <= (uint32_t)NTT_MAX_LOG_N, "mzk_best_fft: log_n %u above %d", log_n, NTT_MAX_LOG_N);
+  if (log_n == 0) return MZK_OK;   // a DFT of length 1 is the identity
+  host::Fr om;
+  memcpy(om.l, omega, 32);
+  return host_fft(c, a, om, log_n, 1ull << log_n);
+}
+
+// compute_twiddles (fft.rs:31-40): out[i] = omega^i, i < 2^(log_n - 1), host memory.
+int mzk_compute_twiddles(mzk_ctx* c, const uint64_t omega[4], uint32_t log_n, uint64_t* out) {
+  MZK_REQUIRE(c != nullptr && omega != nullptr && out != nullptr, "mzk_compute_twiddles: null argument");
+  MZK_REQUIRE(log_n >= 1 && log_n <= (uint32_t)NTT_MAX_LOG_N, "mzk_compute_twiddles: log_n %u outside [1, %d]",
+              log_n, NTT_MAX_LOG_N);
+  const size_t half = 1ull << (log_n - 1);
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  MZK_TRY(c->scratch[SCR_IN0].reserve(half * 32));
+  host::Fr om;
+  memcpy(om.l, omega, 32);
+  hipLaunchKernelGGL(k_pow_table, dim3(grid_for(half, 256)), dim3(256), 0, c->stream,
+                     (uint32_t*)c->scratch[SCR_IN0].ptr, (uint64_t)half, pow_args(om));
+  MZK_HIP(hipGetLastError());
+  MZK_HIP(hipMemcpyAsync(out, c->scratch[SCR_IN0].ptr, half * 32, hipMemcpyDeviceToHost, c->stream));
+  MZK_HIP(hipStreamSynchronize(c->stream));
+  return MZK_OK;
+}
+
+// best_fft_with_twiddles (fft.rs:56-101): the DFT by the root the table was built from.
+int mzk_best_fft_with_twiddles(mzk_ctx* c, uint64_t* a, const uint64_t* twiddles, uint32_t log_n) {
+  MZK_REQUIRE(c != nullptr && a != nullptr, "mzk_best_fft_with_twiddles: null argument");
+  MZK_REQUIRE(log_n <= (uint32_t)NTT_MAX_LOG_N, "mzk_best_fft_with_twiddles: log_n %u above %d", log_n,
+              NTT_MAX_LOG_N);
+  if (log_n == 0) return MZK_OK;
+  host::Fr w;
+  MZK_TRY(root_from_twiddles(twiddles, log_n, &w, "mzk_best_fft_with_twiddles"));
+  return host_fft(c, a, w, log_n, 1ull << log_n);
+}
+
+// fft_coeff_to_extended (fft.rs:110-124): the same DFT of a whose entries from n_real on
+// are zero (the reference's pruned DIF relies on it; only a[0, n_real) is read here).
+int mzk_fft_coeff_to_extended(mzk_ctx* c, uint64_t* a, const uint64_t* twiddles, uint32_t log_n, size_t n_real) {
+  MZK_REQUIRE(c != nullptr && a != nullptr, "mzk_fft_coeff_to_extended: null argument");
+  MZK_REQUIRE(log_n >= 1 && log_n <= (uint32_t)NTT_MAX_LOG_N, "mzk_fft_coeff_to_extended: log_n %u outside [1, %d]",
+              log_n, NTT_MAX_LOG_N);
+  MZK_REQUIRE(n_real <= (1ull << log_n), "mzk_fft_coeff_to_extended: n_real %zu above 2^%u", n_real, log_n);
+  host::Fr w;
+  MZK_TRY(root_from_twiddles(twiddles, log_n, &w, "mzk_fft_coeff_to_extended"));
+  return host_fft(c, a, w, log_n, n_real);
+}
+
+// best_fft on device memory with a caller's root: d_out = DFT by omega of d_in's first
+// n_real elements, zero-padded to 2^log_n (in place allowed).
+int mzk_best_fft_dev(mzk_ctx* c, const void* d_in, void* d_out, const uint64_t omega[4], uint32_t log_n,
+                     uint64_t n_real) {
+  MZK_REQUIRE(c != nullptr && d_in != nullptr && d_out != nullptr && omega != nullptr,
+              "mzk_best_fft_dev: null argument");
+  MZK_REQUIRE(log_n >= 1 && log_n <= (uint32_t)NTT_MAX_LOG_N, "mzk_best_fft_dev: log_n %u outside [1, %d]", log_n,
+              NTT_MAX_LOG_N);
+  MZK_REQUIRE(n_real <= (1ull << log_n), "mzk_best_fft_dev: n_real above 2^log_n");
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  ntt_plans_trim(c);
+  host::Fr om;
+  memcpy(om.l, omega, 32);
+  const NttPlan* plan = ctx_plan_root(c, log_n, om);
+  MZK_REQUIRE(plan != nullptr, "mzk_best_fft_dev: plan for 2^%u failed", log_n);
+  return ntt_run(c, *plan, (const uint32_t*)d_in, (uint32_t*)d_out, n_real, 0, 0, host::fr_one());
 }
 
 }  // extern "C"

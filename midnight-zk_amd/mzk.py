@@ -4,8 +4,13 @@ Loads the in-tree gfx950 library libmzk_gpu.so (built by build.py) through ctype
 and exposes the reference's names:
 
   msm_best(coeffs, bases)            curves/src/msm.rs:460
+  msm_parallel(coeffs, bases)        curves/src/msm.rs:430
+  msm_serial(coeffs, bases, acc)     curves/src/msm.rs:338
   msm_specific(coeffs, bases)        proofs/src/poly/kzg/msm.rs:174
+  compute_twiddles(omega, log_n)     curves/src/fft.rs:31
   best_fft(a, omega, log_n)          curves/src/fft.rs:49
+  best_fft_with_twiddles(a, tw, log_n)          curves/src/fft.rs:56
+  fft_coeff_to_extended(a, tw, log_n, n_real)   curves/src/fft.rs:110
   EvaluationDomain(j, k)             proofs/src/poly/domain.rs:54
      .lagrange_to_coeff / .coeff_to_lagrange / .coeff_to_extended /
      .extended_to_coeff / .divide_by_vanishing_poly
@@ -124,6 +129,7 @@ SIGNATURES = {
     "mzk_ctx_destroy": ([_P], None),
     "mzk_ctx_stream": ([_P], _P),
     "mzk_ctx_synchronize": ([_P], _I),
+    "mzk_ctx_mem_info": ([_P, ctypes.POINTER(_S), ctypes.POINTER(_S)], _I),
     "mzk_msm_best": ([_P, _P, _P, _S, _P], _I),
     "mzk_msm_specific": ([_P, _P, _P, _S, _P], _I),
     "mzk_msm_dev": ([_P, _P, _P, _S, _P], _I),
@@ -139,6 +145,12 @@ SIGNATURES = {
     "mzk_ntt_dev": ([_P, _P], _I),
     "mzk_ntt_batch_dev": ([_P, _P, _S], _I),
     "mzk_best_fft": ([_P, _P, _P, _U32], _I),
+    "mzk_compute_twiddles": ([_P, _P, _U32, _P], _I),
+    "mzk_best_fft_with_twiddles": ([_P, _P, _P, _U32], _I),
+    "mzk_fft_coeff_to_extended": ([_P, _P, _P, _U32, _S], _I),
+    "mzk_best_fft_dev": ([_P, _P, _P, _P, _U32, _U64], _I),
+    "mzk_msm_parallel": ([_P, _P, _P, _S, _P], _I),
+    "mzk_msm_serial": ([_P, _P, _P, _S, _P], _I),
     "mzk_domain_create": ([_P, _U32, _U32, ctypes.POINTER(_P)], _I),
     "mzk_domain_destroy": ([_P], None),
     "mzk_domain_info": ([_P, ctypes.POINTER(_U32), ctypes.POINTER(_U32)], _I),
@@ -311,6 +323,12 @@ class Context:
     def synchronize(self):
         _check(load_library().mzk_ctx_synchronize(self.h))
 
+    def mem_info(self) -> tuple[int, int]:
+        """(free, total) bytes of the device's memory (hipMemGetInfo)."""
+        f, t = _S(), _S()
+        _check(load_library().mzk_ctx_mem_info(self.h, ctypes.byref(f), ctypes.byref(t)))
+        return f.value, t.value
+
     def set_graphs(self, on: bool = True):
         """Replay cached HIP graphs of the MSM / NTT launch sequences (default on)."""
         _check(load_library().mzk_ctx_set_graphs(self.h, 1 if on else 0))
@@ -360,6 +378,27 @@ class Context:
         _check(load_library().mzk_msm_specific(self.h, _np_ptr(coeffs), _np_ptr(bases), coeffs.shape[0], _np_ptr(out)))
         return out
 
+    def msm_parallel(self, coeffs: np.ndarray, bases: np.ndarray) -> np.ndarray:
+        coeffs, bases = self._msm_args(coeffs, bases, "msm_parallel")
+        out = np.zeros(18, dtype=np.uint64)
+        _check(load_library().mzk_msm_parallel(self.h, _np_ptr(coeffs), _np_ptr(bases), coeffs.shape[0], _np_ptr(out)))
+        return out
+
+    def msm_serial(self, coeffs: np.ndarray, bases: np.ndarray, acc: np.ndarray) -> np.ndarray:
+        """acc (18 Jacobian limbs) += sum coeffs[i] bases[i]; returns the updated acc."""
+        coeffs, bases = self._msm_args(coeffs, bases, "msm_serial")
+        acc = np.ascontiguousarray(acc, dtype=np.uint64).reshape(18).copy()
+        _check(load_library().mzk_msm_serial(self.h, _np_ptr(coeffs), _np_ptr(bases), coeffs.shape[0], _np_ptr(acc)))
+        return acc
+
+    @staticmethod
+    def _msm_args(coeffs, bases, who):
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.uint64).reshape(-1, 4)
+        bases = np.ascontiguousarray(bases, dtype=np.uint64).reshape(-1, 12)
+        if coeffs.shape[0] != bases.shape[0]:
+            raise MzkError(f"{who}: coeffs and bases differ in length")
+        return coeffs, bases
+
     @_ordered(_self_ctx)
     def msm_dev(self, d_coeffs, d_bases, n: int) -> np.ndarray:
         out = np.zeros(18, dtype=np.uint64)
@@ -374,6 +413,38 @@ class Context:
         om = np.ascontiguousarray(omega, dtype=np.uint64).reshape(4)
         _check(load_library().mzk_best_fft(self.h, _np_ptr(a), _np_ptr(om), log_n))
         return a
+
+    def compute_twiddles(self, omega: np.ndarray, log_n: int) -> np.ndarray:
+        out = np.zeros((1 << max(0, log_n - 1), 4), dtype=np.uint64)
+        om = np.ascontiguousarray(omega, dtype=np.uint64).reshape(4)
+        _check(load_library().mzk_compute_twiddles(self.h, _np_ptr(om), log_n, _np_ptr(out)))
+        return out
+
+    def best_fft_with_twiddles(self, a: np.ndarray, twiddles: np.ndarray, log_n: int) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=np.uint64).reshape(-1, 4).copy()
+        if a.shape[0] != (1 << log_n):
+            raise MzkError("best_fft_with_twiddles: len(a) != 2^log_n")
+        tw = np.ascontiguousarray(twiddles, dtype=np.uint64).reshape(-1, 4)
+        if tw.shape[0] < max(1, (1 << log_n) // 2):
+            raise MzkError("best_fft_with_twiddles: twiddle table shorter than 2^(log_n-1)")
+        _check(load_library().mzk_best_fft_with_twiddles(self.h, _np_ptr(a), _np_ptr(tw), log_n))
+        return a
+
+    def fft_coeff_to_extended(self, a: np.ndarray, twiddles: np.ndarray, log_n: int, n_real: int) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=np.uint64).reshape(-1, 4).copy()
+        if a.shape[0] != (1 << log_n):
+            raise MzkError("fft_coeff_to_extended: len(a) != 2^log_n")
+        tw = np.ascontiguousarray(twiddles, dtype=np.uint64).reshape(-1, 4)
+        if tw.shape[0] < max(1, (1 << log_n) // 2):
+            raise MzkError("fft_coeff_to_extended: twiddle table shorter than 2^(log_n-1)")
+        _check(load_library().mzk_fft_coeff_to_extended(self.h, _np_ptr(a), _np_ptr(tw), log_n, n_real))
+        return a
+
+    @_ordered(_self_ctx)
+    def best_fft_dev(self, d_in, d_out, omega: np.ndarray, log_n: int, n_real: int | None = None):
+        om = np.ascontiguousarray(omega, dtype=np.uint64).reshape(4)
+        n_real = (1 << log_n) if n_real is None else n_real
+        _check(load_library().mzk_best_fft_dev(self.h, _t_ptr(d_in), _t_ptr(d_out), _np_ptr(om), log_n, n_real))
 
     @_ordered(_self_ctx)
     def ntt(self, job: "NttJob"):

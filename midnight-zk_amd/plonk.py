@@ -722,6 +722,10 @@ class GpuBackend:
     def sync(self):
         self.ctx.synchronize()
 
+    def mem_info(self):
+        """(free, total) device bytes, for admission by HBM budget."""
+        return self.ctx.mem_info()
+
     def zeros(self, n):
         with self._stream_ctx():
             return self.torch.zeros((n, 4), dtype=self.torch.int64, device=self.dev)

@@ -5,6 +5,10 @@ of monomial SRS bases (2^max_k points) and, per supported k, the Lagrange bases,
 evaluation domain and the proving key resident on the device.  A request is admitted
 as soon as it has arrived, a worker has a free slot, and its estimated device footprint
 fits in the HBM budget beside the proofs already in flight (first come, first served).
+The budget is the smaller of the configured one and the device memory measured free
+(hipMemGetInfo) once the resident SRS and keys are in place, less a reserve; a request
+whose footprint alone exceeds the budget is rejected at once with a status, never
+started.
 Every in-flight proof is a prove_steps generator; each scheduling round of a worker
 collects the pending request of each of its in-flight proofs and answers all
 commitments with one batched MSM pipeline per SRS, all transforms with one batched NTT
@@ -48,6 +52,7 @@ class ProofResult:
     admitted: float
     done: float
     proof: bytes = field(repr=False, default=b"")
+    error: str = ""               # non-empty: the request was rejected (and not proven)
 
     @property
     def latency(self) -> float:
@@ -69,6 +74,9 @@ class ProofServer:
     `slots` in-flight proofs in batched rounds; the workers run in their own threads,
     so their device work overlaps on the GPU and their host waits overlap each other."""
 
+    # device memory left unclaimed by admission: scratch growth, plans and graphs
+    RESERVE_BYTES = 4 << 30
+
     def __init__(self, backends, ks=(12, 14, 16, 18, 20), tau: int = 0x5EEDC0DE, layout_seed: int = 1,
                  hbm_budget: int = 32 << 30, slots: int = 2):
         if not isinstance(backends, (list, tuple)):
@@ -87,6 +95,12 @@ class ProofServer:
             self.params[k], self.circuits[k] = p, c
             self.pks[k] = plonk.keygen(be, p, c)
         be.sync()
+        self.configured_budget = hbm_budget
+        self.measured_free = None
+        mem_info = getattr(be, "mem_info", None)
+        if mem_info is not None:
+            self.measured_free = mem_info()[0]
+            self.budget = min(self.budget, max(0, self.measured_free - self.RESERVE_BYTES))
 
     def prove_alone(self, req: ProofRequest, be=None) -> bytes:
         be = be or self.be
@@ -115,23 +129,36 @@ class ProofServer:
         slots = slots or self.slots
         queue = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
         lock = threading.Lock()
-        state = {"used": 0, "inflight": 0, "max_inflight": 0, "rounds": 0}
+        state = {"used": 0, "inflight": 0, "max_inflight": 0, "rounds": 0, "rejected": 0}
         done, errors = [], []
         t0 = time.perf_counter()
         clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
+
+        def reject_oversize(now):
+            """Requests at the head of the queue that no budget state could ever admit are
+            answered at once with an error (caller holds the lock)."""
+            while queue and queue[0].arrival <= now and footprint_bytes(queue[0].k) > self.budget:
+                req = queue.popleft()
+                t = time.perf_counter() - t0
+                done.append(ProofResult(req.rid, req.k, req.arrival, t, t, b"",
+                                        f"rejected: footprint {footprint_bytes(req.k)} B exceeds the HBM budget "
+                                        f"{self.budget} B"))
+                state["rejected"] += 1
 
         def admit(n_local):
             """Pops the requests this worker may start now (first come, first served)."""
             out = []
             with lock:
                 now = clock()
+                reject_oversize(now)
                 while (queue and queue[0].arrival <= now and n_local + len(out) < slots
-                       and (state["inflight"] == 0 or state["used"] + footprint_bytes(queue[0].k) <= self.budget)):
+                       and state["used"] + footprint_bytes(queue[0].k) <= self.budget):
                     req = queue.popleft()
                     state["used"] += footprint_bytes(req.k)
                     state["inflight"] += 1
                     state["max_inflight"] = max(state["max_inflight"], state["inflight"])
                     out.append(req)
+                    reject_oversize(now)
                 nxt = queue[0].arrival if queue else None
             return out, nxt
 
@@ -139,6 +166,7 @@ class ProofServer:
             """Native engine: the oldest arrived request and the arrived ones of its size."""
             with lock:
                 now = clock()
+                reject_oversize(now)
                 if not queue or queue[0].arrival > now:
                     return [], (queue[0].arrival if queue else None)
                 k = queue[0].k
@@ -148,8 +176,7 @@ class ProofServer:
                         break
                     if r.k != k:
                         continue
-                    if (state["inflight"] + len(picked) > 0
-                            and state["used"] + footprint_bytes(k) * (len(picked) + 1) > self.budget):
+                    if state["used"] + footprint_bytes(k) * (len(picked) + 1) > self.budget:
                         break
                     picked.append(r)
                 if not picked:
@@ -249,7 +276,8 @@ class ProofServer:
             raise errors[0]
         wall = time.perf_counter() - t0
         return done, {"rounds": state["rounds"], "max_inflight": state["max_inflight"], "wall_s": wall,
-                      "workers": len(bes), "slots": slots, "engine": engine}
+                      "workers": len(bes), "slots": slots, "engine": engine, "rejected": state["rejected"],
+                      "budget": self.budget}
 
 
 def poisson_trace(n: int, rate: float, ks, seed: int):
@@ -267,6 +295,7 @@ def poisson_trace(n: int, rate: float, ks, seed: int):
 
 
 def latency_stats(results) -> dict:
+    results = [r for r in results if not r.error]
     lat = sorted(r.latency for r in results)
     if not lat:
         return {}

@@ -631,6 +631,12 @@ void cpu_ref_msm_parallel(const u64 *scalars, const u64 *bases, size_t n, u64 *o
   msm_parallel(reprs, (const g1a_t *)bases, n, (g1j_t *)out);
   free(reprs);
 }
+/* msm_serial (msm.rs:338-423): *acc += the MSM, acc Jacobian in and out. */
+void cpu_ref_msm_serial(const u64 *scalars, const u64 *bases, size_t n, u64 *acc) {
+  uint8_t *reprs = scalars_to_reprs(scalars, n);
+  msm_serial(reprs, (const g1a_t *)bases, n, (g1j_t *)acc);
+  free(reprs);
+}
 /* msm_specific (kzg/msm.rs:174-201): filter zero scalars, then msm_best (the
  * blst branch below 2^19 non-zero terms is third-party code not present here;
  * it computes the same unique point). */

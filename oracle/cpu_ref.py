@@ -44,6 +44,7 @@ def lib():
             "cpu_ref_window_size": ([S], ctypes.c_int),
             "cpu_ref_msm_best": ([P, P, S, P], None),
             "cpu_ref_msm_parallel": ([P, P, S, P], None),
+            "cpu_ref_msm_serial": ([P, P, S, P], None),
             "cpu_ref_msm_specific": ([P, P, S, P], None),
             "cpu_ref_g1_to_affine": ([P, P], None),
             "cpu_ref_compute_twiddles": ([P, U, P], None),
@@ -111,6 +112,13 @@ def msm_parallel(scalars: np.ndarray, bases: np.ndarray) -> np.ndarray:
     return out
 
 
+def msm_serial(scalars: np.ndarray, bases: np.ndarray, acc: np.ndarray) -> np.ndarray:
+    """acc + the MSM (msm.rs:338-423), Jacobian."""
+    out = np.ascontiguousarray(acc, dtype=np.uint64).reshape(18).copy()
+    lib().cpu_ref_msm_serial(_p(scalars), _p(bases), scalars.shape[0], _p(out))
+    return out
+
+
 def msm_specific(scalars: np.ndarray, bases: np.ndarray) -> np.ndarray:
     out = np.zeros(18, dtype=np.uint64)
     lib().cpu_ref_msm_specific(_p(scalars), _p(bases), scalars.shape[0], _p(out))
@@ -133,6 +141,45 @@ def poly_eval(coeffs: np.ndarray, x_mont: np.ndarray) -> np.ndarray:
     out = np.zeros(4, dtype=np.uint64)
     lib().cpu_ref_poly_eval(_p(coeffs), coeffs.shape[0], _p(np.ascontiguousarray(x_mont, dtype=np.uint64)), _p(out))
     return out
+
+
+def root_of_unity(log_n: int) -> np.ndarray:
+    """ROOT^(2^(32 - log_n)), Montgomery."""
+    out = np.zeros(4, dtype=np.uint64)
+    lib().cpu_ref_root_of_unity(log_n, _p(out))
+    return out
+
+
+def fr_mul(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    out = np.zeros(4, dtype=np.uint64)
+    lib().cpu_ref_fr_mul(_p(_c(a)), _p(_c(b)), _p(out))
+    return out
+
+
+def fr_inv(a: np.ndarray) -> np.ndarray:
+    out = np.zeros(4, dtype=np.uint64)
+    lib().cpu_ref_fr_inv(_p(_c(a)), _p(out))
+    return out
+
+
+def compute_twiddles(omega: np.ndarray, log_n: int) -> np.ndarray:
+    """[1, w, ..., w^(n/2 - 1)] (fft.rs:31-40)."""
+    out = np.zeros((1 << (log_n - 1), 4), dtype=np.uint64)
+    lib().cpu_ref_compute_twiddles(_p(_c(omega)), log_n, _p(out))
+    return out
+
+
+def best_fft_with_twiddles(a: np.ndarray, twiddles: np.ndarray, log_n: int) -> np.ndarray:
+    b = np.ascontiguousarray(a, dtype=np.uint64).copy()
+    lib().cpu_ref_best_fft_with_twiddles(_p(b), _p(_c(twiddles)), log_n)
+    return b
+
+
+def fft_coeff_to_extended(a: np.ndarray, twiddles: np.ndarray, log_n: int, n_real: int) -> np.ndarray:
+    """The pruned DIF of fft.rs:110-186 (a's entries from n_real on must be zero)."""
+    b = np.ascontiguousarray(a, dtype=np.uint64).copy()
+    lib().cpu_ref_fft_coeff_to_extended(_p(b), _p(_c(twiddles)), log_n, n_real)
+    return b
 
 
 def ntt(a: np.ndarray, log_n: int) -> np.ndarray:

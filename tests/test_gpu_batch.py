@@ -134,6 +134,6 @@ def test_ntt_job_validation(ctx):
     with pytest.raises(mzk.MzkError):
         ctx.ntt(mzk.NttJob.make(a, a, 4, mzk.NTT_INVERSE, 8))       # inverse needs 2^log_n inputs
     with pytest.raises(mzk.MzkError):
-        ctx.ntt(mzk.NttJob.make(a, a, 23, 0, 16))                    # beyond 2^22
+        ctx.ntt(mzk.NttJob.make(a, a, 33, 0, 16))                    # beyond 2^32 (S = 32)
     with pytest.raises(mzk.MzkError):
         ctx.ntt(mzk.NttJob.make(a, a, 4, 8, 16))                     # unknown flag

@@ -75,3 +75,27 @@ def test_native_proofs_with_and_without_graphs(ctx):
         print(f"\n[graphs] native group {rep}", flush=True)
         assert plonk.create_proofs_native(be, params, pk, wits, [4, 5]) == eager
     assert ctx.graph_stats()["replayed"] > 0
+
+
+def test_working_set_above_cache_size(ctx):
+    # 60 shapes cycled four times: more shapes than the 48-graph cache holds.  Results stay
+    # the eager ones, the cache keeps its first 48 shapes (no recapture on every call) and
+    # the shapes beyond them run eagerly.
+    tau = C.fr_from_seed(8, 0, 1)[0]
+    srs = ctx.srs_upload(C.srs_bases(tau, 1 << 15))
+    sizes = [1000 + 37 * i for i in range(60)]
+    sc = [to_dev(C.fr_from_seed(600 + i, 1, n)) for i, n in enumerate(sizes)]
+    ctx.set_graphs(False)
+    want = [aff(srs.commit(d, n)) for d, n in zip(sc, sizes)]
+    ctx.set_graphs(True)          # frees every graph: counters below start from this state
+    base = ctx.graph_stats()
+    for rep in range(4):
+        print(f"\n[graphs] 60-shape cycle {rep}", flush=True)
+        for d, n, w in zip(sc, sizes, want):
+            assert aff(srs.commit(d, n)) == w, (rep, n)
+    st = ctx.graph_stats()
+    captured = st["captured"] - base["captured"]
+    replayed = st["replayed"] - base["replayed"]
+    assert st["held"] <= 48 and captured <= 48, st
+    assert replayed >= 2 * 48, st        # cycles 3 and 4 replay the cached shapes
+    srs.close()

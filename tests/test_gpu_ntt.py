@@ -120,4 +120,6 @@ def test_best_fft_with_given_omega(ctx, log_n):
 
 def test_domain_rejects_oversize(ctx):
     with pytest.raises(mzk.MzkError):
-        ctx.domain(5, 23)   # extended_k 25 is beyond this build's two-pass limit
+        ctx.domain(5, 31)   # extended_k 33 exceeds S = 32 (domain.rs:69-70)
+    with pytest.raises(mzk.MzkError):
+        ctx.domain(17, 29)  # 2^29 * 16 needs extended_k 33

@@ -192,3 +192,52 @@ def test_bases_from_scalars_match_python():
     s = [O.fr_from_seed(6, 1, i) for i in range(20)] + [0, 1, O.R_MOD - 1]
     rows = C.g1_mul_generator_many(C.ints_to_fr_rows(s))
     assert aff_rows_to_points(rows) == [O.g1_mul(O.G1_GEN, v) for v in s]
+
+
+def test_msm_serial_adds_into_accumulator():
+    rng = random.Random(12)
+    n = 30
+    pts = [O.g1_mul(O.G1_GEN, rng.randrange(1, O.R_MOD)) for _ in range(n)]
+    sc = [rng.randrange(O.R_MOD) for _ in range(n)]
+    bases = np.array([O.g1_affine_mont_limbs(P) for P in pts], dtype=np.uint64)
+    acc_pt = O.g1_mul(O.G1_GEN, 987654321)
+    acc = C.msm_parallel(C.ints_to_fr_rows([987654321]), np.array([O.g1_affine_mont_limbs(O.G1_GEN)],
+                                                                    dtype=np.uint64))
+    assert O.g1_jacobian_from_mont_limbs(acc) == acc_pt
+    # msm.rs:364-367 doubles the incoming accumulator c times per window before adding the
+    # window's buckets: acc_out = [2^(c W)] acc_in + MSM, W = 8 (highest non-zero byte) / c + 1
+    sc[5] = (1 << 200) + 3                   # the top byte among the scalars decides W
+    out = C.msm_serial(C.ints_to_fr_rows(sc), bases, acc)
+    c = O.msm_window_size(n)
+    top = max((s.bit_length() + 7) // 8 for s in sc)
+    W = top * 8 // c + 1
+    assert O.g1_jacobian_from_mont_limbs(out) == O.g1_add(O.g1_mul(acc_pt, pow(2, c * W, O.R_MOD)),
+                                                          O.msm_naive(sc, pts))
+    # from the identity it is the MSM itself; all-zero scalars leave acc untouched (:360-362)
+    zero = np.zeros(18, dtype=np.uint64)
+    assert O.g1_jacobian_from_mont_limbs(C.msm_serial(C.ints_to_fr_rows(sc), bases, zero)) == \
+        O.msm_naive(sc, pts)
+    assert O.g1_jacobian_from_mont_limbs(C.msm_serial(C.ints_to_fr_rows([0] * n), bases, acc)) == acc_pt
+
+
+def _dft_textbook(a, w):
+    n = len(a)
+    return [sum(a[i] * pow(w, i * j, O.R_MOD) for i in range(n)) % O.R_MOD for j in range(n)]
+
+
+@pytest.mark.parametrize("log_n", [1, 2, 3, 5])
+def test_twiddle_table_ffts_with_any_primitive_root(log_n):
+    # omega^3 is another primitive 2^log_n-th root (3 is odd): the table forms follow it
+    n = 1 << log_n
+    w = pow(O.root_of_unity(log_n), 3, O.R_MOD)
+    tw = C.compute_twiddles(C.ints_to_fr_rows([w])[0], log_n)
+    assert C.fr_rows_to_ints(tw) == [pow(w, i, O.R_MOD) for i in range(n // 2)]
+    rng = random.Random(log_n)
+    a = [rng.randrange(O.R_MOD) for _ in range(n)]
+    assert C.fr_rows_to_ints(C.best_fft_with_twiddles(C.ints_to_fr_rows(a), tw, log_n)) == _dft_textbook(a, w)
+    for n_real in sorted({0, 1, max(1, n // 2), n // 2 + 1, n} - {n + 1}):
+        if n_real > n:
+            continue
+        b = a[:n_real] + [0] * (n - n_real)
+        got = C.fft_coeff_to_extended(C.ints_to_fr_rows(b), tw, log_n, n_real)
+        assert C.fr_rows_to_ints(got) == _dft_textbook(b, w), n_real

@@ -39,6 +39,24 @@ def test_budget_limits_inflight(server):
         server.budget = 1 << 30
 
 
+def test_oversize_request_rejected_even_when_idle(server):
+    # a request whose footprint alone exceeds the budget is answered with an error at once
+    # and never started; the others are served as usual
+    server.budget = serve.footprint_bytes(6)
+    try:
+        reqs = [serve.ProofRequest(rid=0, k=5, witness_seed=1), serve.ProofRequest(rid=1, k=7, witness_seed=2),
+                serve.ProofRequest(rid=2, k=6, witness_seed=3)]
+        results, stats = server.run(reqs, realtime=False, workers=1, slots=4)
+        by = {r.rid: r for r in results}
+        assert sorted(by) == [0, 1, 2] and stats["rejected"] == 1
+        assert by[1].error.startswith("rejected") and by[1].proof == b""
+        for rid in (0, 2):
+            assert not by[rid].error and by[rid].proof == server.prove_alone(reqs[rid])
+        assert serve.latency_stats(results)["proofs"] == 2
+    finally:
+        server.budget = 1 << 30
+
+
 def test_unknown_size_rejected(server):
     with pytest.raises(ValueError):
         server.run([serve.ProofRequest(rid=0, k=9, witness_seed=1)], realtime=False)
