@@ -217,6 +217,15 @@ int mzk_ntt_dev(mzk_ctx *ctx, const mzk_ntt_job *job);
  * overlap another job's input or output (a job's own d_in == d_out is allowed):
  * MZK_ERR_INVALID names the first such pair. */
 int mzk_ntt_batch_dev(mzk_ctx *ctx, const mzk_ntt_job *jobs, size_t njobs);
+/* No counterpart in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  The same transform as mzk_ntt_dev, with
+ * the same results, calling no hipMalloc / hipFree / allocating routine: the pass buffer
+ * and twiddle tables live in a fixed device array the library holds per device (32 MiB +
+ * 192 KiB) until the process ends.  log_n <= MZK_NTT_NOALLOC_MAX_LOG, else MZK_ERR_INVALID.
+ * Every call on one device shares that array: calls that overlap on a device race, so
+ * order them (one stream).  No graph is captured and no kernel timing is recorded. */
+#define MZK_NTT_NOALLOC_MAX_LOG 20
+int mzk_ntt_dev_noalloc(mzk_ctx *ctx, const mzk_ntt_job *job);
 /* Host-pointer forms of the four transforms. */
 int mzk_lagrange_to_coeff(mzk_domain *d, uint64_t *a);
 int mzk_coeff_to_extended(mzk_domain *d, const uint64_t *in, uint64_t *out);

@@ -235,6 +235,7 @@ int mzk_ctx_create(int device, mzk_ctx** out) {
   MZK_REQUIRE(device >= 0 && device < ndev, "mzk_ctx_create: device %d out of range (%d devices)",
               device, ndev);
   MZK_HIP(hipSetDevice(device));
+  MZK_TRY(noalloc_setup(device));
   mzk_ctx* c = new mzk_ctx();
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

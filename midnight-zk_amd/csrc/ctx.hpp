@@ -116,6 +116,9 @@ void free_ntt_plans(mzk_ctx* c);
 // With graphs off or the kernel profiler on, `launch` simply runs.  Caller holds c->mu.
 int run_graph(mzk_ctx* c, const std::string& key, const std::function<int()>& launch);
 void free_graphs(mzk_ctx* c);
+// Per-device set-up of the allocation-free NTT's device array (noalloc.hip); called with
+// the device current.
+int noalloc_setup(int device);
 
 }  // namespace mzk
 

@@ -915,7 +915,10 @@ int mzk_coeff_to_extended(mzk_domain* d, const uint64_t* in, uint64_t* out) {
   return run_host(d, in, out, 1ull << d->k, 1ull << d->ext_k, c2e_impl);
 }
 
-static int check_ntt_job(const mzk_ntt_job& j, size_t i) {
+}  // extern "C"
+
+namespace mzk {
+int ntt_check_job(const mzk_ntt_job& j, size_t i) {
   MZK_REQUIRE(j.d_in != nullptr && j.d_out != nullptr, "ntt job %zu: null data", i);
   MZK_REQUIRE(j.log_n >= 1 && j.log_n <= (uint32_t)NTT_MAX_LOG_N, "ntt job %zu: log_n %u outside [1, %d]", i,
               j.log_n, NTT_MAX_LOG_N);
@@ -927,6 +930,9 @@ static int check_ntt_job(const mzk_ntt_job& j, size_t i) {
   // pass and writes d_out only in its last; a single pass stages all of a sub-DFT in LDS
   return MZK_OK;
 }
+}  // namespace mzk
+
+extern "C" {
 
 // byte ranges of a job's input and output
 static void job_spans(const mzk_ntt_job& j, uintptr_t in[2], uintptr_t out[2]) {
@@ -980,7 +986,7 @@ static int check_batch_overlaps(const mzk_ntt_job* jobs, size_t njobs) {
 
 int mzk_ntt_dev(mzk_ctx* c, const mzk_ntt_job* job) {
   MZK_REQUIRE(c != nullptr && job != nullptr, "mzk_ntt_dev: null argument");
-  MZK_TRY(check_ntt_job(*job, 0));
+  MZK_TRY(ntt_check_job(*job, 0));
   std::lock_guard<std::mutex> g(c->mu);
   MZK_HIP(hipSetDevice(c->device));
   ntt_plans_trim(c);
@@ -996,7 +1002,7 @@ int mzk_ntt_dev(mzk_ctx* c, const mzk_ntt_job* job) {
 
 int mzk_ntt_batch_dev(mzk_ctx* c, const mzk_ntt_job* jobs, size_t njobs) {
   MZK_REQUIRE(c != nullptr && (jobs != nullptr || njobs == 0), "mzk_ntt_batch_dev: null argument");
-  for (size_t i = 0; i < njobs; i++) MZK_TRY(check_ntt_job(jobs[i], i));
+  for (size_t i = 0; i < njobs; i++) MZK_TRY(ntt_check_job(jobs[i], i));
   MZK_TRY(check_batch_overlaps(jobs, njobs));
   if (njobs == 0) return MZK_OK;
   std::lock_guard<std::mutex> g(c->mu);

@@ -77,6 +77,9 @@ int ntt_build_passes(const NttPlan& plan, const uint32_t* src, uint32_t* dst, ui
 // Launches one pass on stream st without the context's profiler or graphs.
 int ntt_launch_pass_raw(hipStream_t st, const PassArgs& a, uint64_t groups);
 
+// Validates one transform job (sizes, flags, pointers); job index i names it in the error.
+int ntt_check_job(const mzk_ntt_job& j, size_t i);
+
 // DFT of src into dst as ntt_build_passes describes; src may equal dst.  Enqueued on the
 // context stream (scratch tmp from the context); caller holds the context lock.
 int ntt_run(mzk_ctx* c, const NttPlan& plan, const uint32_t* src, uint32_t* dst, uint64_t n_real, int in_coset,

@@ -46,6 +46,7 @@ class MsmJob(ctypes.Structure):
 
 NTT_INVERSE = 1
 NTT_COSET = 2
+NTT_NOALLOC_MAX_LOG = 20
 
 
 class NttJob(ctypes.Structure):
@@ -144,6 +145,7 @@ SIGNATURES = {
     "mzk_g1_jacobian_to_affine": ([_P, _P], _I),
     "mzk_ntt_dev": ([_P, _P], _I),
     "mzk_ntt_batch_dev": ([_P, _P, _S], _I),
+    "mzk_ntt_dev_noalloc": ([_P, _P], _I),
     "mzk_best_fft": ([_P, _P, _P, _U32], _I),
     "mzk_compute_twiddles": ([_P, _P, _U32, _P], _I),
     "mzk_best_fft_with_twiddles": ([_P, _P, _P, _U32], _I),
@@ -449,6 +451,12 @@ class Context:
     @_ordered(_self_ctx)
     def ntt(self, job: "NttJob"):
         _check(load_library().mzk_ntt_dev(self.h, ctypes.byref(job)))
+
+    @_ordered(_self_ctx)
+    def ntt_noalloc(self, job: "NttJob"):
+        """mzk_ntt_dev_noalloc: the same transform from the library's fixed per-device
+        array (log_n <= NTT_NOALLOC_MAX_LOG), no device allocation."""
+        _check(load_library().mzk_ntt_dev_noalloc(self.h, ctypes.byref(job)))
 
     @_ordered(_self_ctx)
     def ntt_batch(self, jobs):
