@@ -126,6 +126,8 @@ size_t mzk_srs_len(const mzk_srs *srs);
 const void *mzk_srs_bases_dev(const mzk_srs *srs);
 /* Copies bases[offset, offset+count) to host memory. */
 int mzk_srs_download(mzk_srs *srs, size_t offset, size_t count, uint64_t *out);
+/* Waits for all work on the SRS's device, then frees the bases (valid also after the
+ * creating context is destroyed). */
 void mzk_srs_destroy(mzk_srs *srs);
 /* KZG commit against resident bases (kzg/mod.rs:82-108 commit_many -> msm_specific):
  * sum_i coeffs[i] * g[offset + i], coeffs device-resident.  The bases are read-only after
@@ -180,6 +182,8 @@ int mzk_best_fft_dev(mzk_ctx *ctx, const void *d_in, void *d_out, const uint64_t
  * resident on device. */
 typedef struct mzk_domain mzk_domain;
 int mzk_domain_create(mzk_ctx *ctx, uint32_t j, uint32_t k, mzk_domain **out);
+/* Waits for all work on the domain's device, then frees its tables (valid also after the
+ * creating context is destroyed). */
 void mzk_domain_destroy(mzk_domain *d);
 int mzk_domain_info(const mzk_domain *d, uint32_t *k, uint32_t *extended_k);
 /* domain.rs:249-264 lagrange_to_coeff: in place on 2^k device elements. */

@@ -753,7 +753,8 @@ int ntt_run_batch(mzk_ctx* c, const mzk_ntt_job* jobs, size_t njobs) {
 
 // ----------------------------------------------------------------------- domain
 struct mzk_domain {
-  mzk_ctx* ctx = nullptr;
+  mzk_ctx* ctx = nullptr;   // runs the transforms; never read at destroy
+  int device = 0;
   uint32_t k = 0, ext_k = 0;
   mzk::NttPlan fwd, inv, ext_fwd, ext_inv;
   mzk::host::Fr n_inv, ext_n_inv;
@@ -784,6 +785,7 @@ int mzk_domain_create(mzk_ctx* c, uint32_t j, uint32_t k, mzk_domain** out) {
   MZK_HIP(hipSetDevice(c->device));
   mzk_domain* d = new mzk_domain();
   d->ctx = c;
+  d->device = c->device;
   d->k = k;
   d->ext_k = ek;
   host::Fr w_ext = host::fr_root_of_unity(ek);
@@ -822,12 +824,11 @@ int mzk_domain_create(mzk_ctx* c, uint32_t j, uint32_t k, mzk_domain** out) {
   return MZK_OK;
 }
 
+// The creating context may already be destroyed: wait for the device, touching no context.
 void mzk_domain_destroy(mzk_domain* d) {
   if (!d) return;
-  if (d->ctx) {
-    (void)hipSetDevice(d->ctx->device);
-    (void)hipStreamSynchronize(d->ctx->stream);
-  }
+  (void)hipSetDevice(d->device);
+  (void)hipDeviceSynchronize();
   if (d->t_evals) (void)hipFree(d->t_evals);
   delete d;
 }

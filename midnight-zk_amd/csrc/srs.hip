@@ -341,6 +341,7 @@ static int srs_finish(mzk_ctx* c, mzk_srs* s) {
 static int srs_alloc(mzk_ctx* c, size_t n, mzk_srs** out) {
   mzk_srs* s = new mzk_srs();
   s->ctx = c;
+  s->device = c->device;
   s->n = n;
   hipError_t e = hipMalloc(&s->bases, n * 96);
   if (e != hipSuccess) {
@@ -463,12 +464,12 @@ int mzk_srs_download(mzk_srs* s, size_t offset, size_t count, uint64_t* out) {
   return MZK_OK;
 }
 
+// Any context on the device may have queued commits reading the bases, and the creating
+// context may already be destroyed: wait for the whole device, touching no context.
 void mzk_srs_destroy(mzk_srs* s) {
   if (!s) return;
-  if (s->ctx) {
-    (void)hipSetDevice(s->ctx->device);
-    (void)hipStreamSynchronize(s->ctx->stream);
-  }
+  (void)hipSetDevice(s->device);
+  (void)hipDeviceSynchronize();
   if (s->bases) (void)hipFree(s->bases);
   if (s->shifted) (void)hipFree(s->shifted);
   delete s;

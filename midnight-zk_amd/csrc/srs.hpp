@@ -5,7 +5,8 @@
 #define MZK_SRS_REC_WORDS 32
 
 struct mzk_srs {
-  mzk_ctx* ctx = nullptr;
+  mzk_ctx* ctx = nullptr;    // the creating context (used by mzk_srs_download); never read at destroy
+  int device = 0;            // the device holding the bases
   size_t n = 0;
   void* bases = nullptr;     // n x 24 u32 affine (blst_p1_affine layout)
   void* shifted = nullptr;   // optional window-shifted copies 2^(c w) g_i, as accumulation

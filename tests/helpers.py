@@ -27,3 +27,18 @@ def from_dev(t):
 def empty_dev(rows, cols):
     import torch
     return torch.zeros((rows, cols), dtype=torch.int64, device="cuda")
+
+
+# Host builds of the native check programs: plain (-O2), and with AddressSanitizer and
+# UndefinedBehaviorSanitizer (any report aborts the program, so the test fails).
+SANITIZE_FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+                  "-fno-sanitize-recover=all"]
+BUILD_MODES = ["plain", "asan_ubsan"]
+
+
+def build_native(compiler: str, src: str, out: str, mode: str, lang_flags=("-std=c++17", "-x", "c++"),
+                 libs=()):
+    import subprocess
+    opt = SANITIZE_FLAGS if mode == "asan_ubsan" else ["-O2"]
+    subprocess.run([compiler, *opt, *lang_flags, src, "-o", out, *libs], check=True)
+    return out

@@ -10,18 +10,18 @@ import subprocess
 
 import pytest
 
+from tests.helpers import BUILD_MODES, build_native
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-@pytest.fixture(scope="module")
-def exe(tmp_path_factory):
+@pytest.fixture(scope="module", params=BUILD_MODES)
+def exe(tmp_path_factory, request):
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("g++ not available")
-    out = str(tmp_path_factory.mktemp("b2b") / "blake2b_check")
-    subprocess.run([gxx, "-O2", "-std=c++17", "-x", "c++", os.path.join(HERE, "native", "blake2b_check.cpp"),
-                    "-o", out], check=True)
-    return out
+    out = str(tmp_path_factory.mktemp("b2b_" + request.param) / "blake2b_check")
+    return build_native(gxx, os.path.join(HERE, "native", "blake2b_check.cpp"), out, request.param)
 
 
 def digests(exe, cases):

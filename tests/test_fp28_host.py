@@ -10,20 +10,20 @@ import subprocess
 import numpy as np
 import pytest
 
+from tests.helpers import BUILD_MODES, build_native
+
 from oracle import bls12_381 as O
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-@pytest.fixture(scope="module")
-def exe(tmp_path_factory):
+@pytest.fixture(scope="module", params=BUILD_MODES)
+def exe(tmp_path_factory, request):
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("g++ not available")
-    out = str(tmp_path_factory.mktemp("f28") / "f28_check")
-    subprocess.run([gxx, "-O2", "-std=c++17", "-x", "c++", os.path.join(HERE, "native", "f28_check.cpp"), "-o", out],
-                   check=True)
-    return out
+    out = str(tmp_path_factory.mktemp("f28_" + request.param) / "f28_check")
+    return build_native(gxx, os.path.join(HERE, "native", "f28_check.cpp"), out, request.param)
 
 
 def run(exe, cmd, n, payload: np.ndarray) -> np.ndarray:

@@ -10,6 +10,8 @@ import subprocess
 import numpy as np
 import pytest
 
+from tests.helpers import BUILD_MODES, build_native
+
 from oracle import bls12_381 as O
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -17,15 +19,13 @@ R = O.R_MOD
 RINV = pow(1 << 256, -1, R)
 
 
-@pytest.fixture(scope="module")
-def exe(tmp_path_factory):
+@pytest.fixture(scope="module", params=BUILD_MODES)
+def exe(tmp_path_factory, request):
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("g++ not available")
-    out = str(tmp_path_factory.mktemp("fr29") / "fr29_check")
-    subprocess.run([gxx, "-O2", "-std=c++17", "-x", "c++", os.path.join(HERE, "native", "fr29_check.cpp"), "-o", out],
-                   check=True)
-    return out
+    out = str(tmp_path_factory.mktemp("fr29_" + request.param) / "fr29_check")
+    return build_native(gxx, os.path.join(HERE, "native", "fr29_check.cpp"), out, request.param)
 
 
 def words(v):

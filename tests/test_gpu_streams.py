@@ -30,3 +30,22 @@ def test_library_call_orders_after_and_before_torch_work():
         assert np.array_equal(got[1:], want[1:])
     finally:
         ctx.close()
+
+
+def test_srs_and_domain_destroyed_after_their_context():
+    # an SRS and a domain may outlive the context that created them (the bases are shared
+    # by every context on the device): destroying them afterwards touches no context
+    from oracle import bls12_381 as O
+    c1, c2 = mzk.Context(0), mzk.Context(0)
+    tau = C.fr_from_seed(3, 0, 1)[0]
+    bases = C.srs_bases(tau, 200)
+    srs = c1.srs_upload(bases)
+    dom = c1.domain(3, 8)
+    sc = C.fr_from_seed(4, 1, 200)
+    d = torch.from_numpy(sc.view(np.int64)).cuda()
+    c1.close()
+    out = srs.commit_batch([(d, 200, 0)], ctx=c2)[0]     # another context still commits
+    assert O.g1_jacobian_from_mont_limbs(out) == O.g1_jacobian_from_mont_limbs(C.msm_best(sc, bases))
+    srs.close()
+    dom.close()
+    c2.close()
