@@ -283,6 +283,13 @@ typedef struct mzk_kate_job {
   void *d_q;
 } mzk_kate_job;
 int mzk_fr_kate_div_batch(mzk_ctx *ctx, const mzk_kate_job *jobs, size_t njobs);
+/* Delta encodings of a Lagrange vector for commits in the delta bases (poly/mod.rs:290-335,
+ * Polynomial::to_delta / to_double_delta): order 1, out_0 = a_0, out_i = a_i - a_(i-1);
+ * order 2, out_0 = a_0, out_1 = a_1 - 2 a_0, out_i = a_i - 2 a_(i-1) + a_(i-2).  Committed
+ * against the suffix-sum bases (mzk_srs_suffix_sum of g_lagrange, once or twice) they give
+ * the Lagrange commitment, with zero entries for constant (or linear) runs.  out must not
+ * overlap a. */
+int mzk_fr_to_delta_dev(mzk_ctx *ctx, const void *d_a, void *d_out, size_t n, int order);
 /* out_i = x^i for i < n (powers, kzg/mod.rs:151-167 and the domain's omega tables). */
 int mzk_fr_powers(mzk_ctx *ctx, const uint64_t x[4], size_t n, void *d_out);
 
@@ -357,6 +364,10 @@ typedef struct {
   const void *l0, *l_last, *l_active, *coset_x;  /* extended-domain vectors */
   const void *omega_powers;                      /* omega^i, i < n */
   uint64_t transcript_repr[4];                   /* VK hash (Montgomery) */
+  /* suffix sums of g_lagrange (2^k): when set, the permutation products z are committed as
+   * z.to_delta() against them (permutation/prover.rs:236-245) -- the same points, with the
+   * constant runs of z skipped; null commits z densely against g_lagrange */
+  const mzk_srs *g_lagrange_delta;
 } mzk_plonk_key;
 int mzk_plonk_prove(mzk_ctx *ctx, const mzk_plonk_key *key, size_t nproofs, const void *const *advice,
                     const uint64_t *rng_seeds, uint8_t *proofs, double *phase_ms);

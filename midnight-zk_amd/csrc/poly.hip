@@ -44,6 +44,23 @@ __global__ void k_vec_binop(int op, const uint32_t* __restrict__ a, const uint32
   store<FrP>(out + 8 * i, r);
 }
 
+// Delta encodings of a Lagrange vector (poly/mod.rs:290-335): order 1, b_0 = a_0 and
+// b_i = a_i - a_(i-1); order 2, c_0 = a_0, c_1 = a_1 - 2 a_0, c_i = a_i - 2 a_(i-1) + a_(i-2).
+__global__ void k_to_delta(const uint32_t* __restrict__ a, uint32_t* __restrict__ out, size_t n, int order) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  fr_t r = load<FrP>(a + 8 * i);
+  if (i >= 1) {
+    const fr_t p = load<FrP>(a + 8 * (i - 1));
+    r = sub(r, p);
+    if (order == 2) {
+      r = sub(r, p);
+      if (i >= 2) r = add(r, load<FrP>(a + 8 * (i - 2)));
+    }
+  }
+  store<FrP>(out + 8 * i, r);
+}
+
 // op 0: out = a * s; 1: out = a + s; 2: out = out + a * s
 __global__ void k_vec_scalar(int op, const uint32_t* __restrict__ a, FrArg s, uint32_t* __restrict__ out, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -583,6 +600,24 @@ int mzk_fr_vec_op(mzk_ctx* c, int op, const void* a, const void* b, void* out, s
   std::lock_guard<std::mutex> g(c->mu);
   MZK_LAUNCH(c, "k_vec_binop", k_vec_binop, dim3(grid_for(n, 256)), dim3(256), 0, op, (const uint32_t*)a,
              (const uint32_t*)b, (uint32_t*)out, n);
+  MZK_HIP(hipGetLastError());
+  return MZK_OK;
+}
+
+}  // extern "C"
+static bool spans_overlap(const void* p, size_t pb, const void* q, size_t qb);
+extern "C" {
+
+int mzk_fr_to_delta_dev(mzk_ctx* c, const void* a, void* out, size_t n, int order) {
+  MZK_REQUIRE(c != nullptr, "null context");
+  MZK_REQUIRE(order == 1 || order == 2, "mzk_fr_to_delta_dev: order %d is not 1 or 2", order);
+  if (n == 0) return MZK_OK;
+  MZK_REQUIRE(a && out, "mzk_fr_to_delta_dev: null data");
+  MZK_REQUIRE(!spans_overlap(a, n * 32, out, n * 32), "mzk_fr_to_delta_dev: output overlaps input");
+  MZK_TRY(vec_ready(c));
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_LAUNCH(c, "k_to_delta", k_to_delta, dim3(grid_for(n, 256)), dim3(256), 0, (const uint32_t*)a, (uint32_t*)out,
+             n, order);
   MZK_HIP(hipGetLastError());
   return MZK_OK;
 }

@@ -5,8 +5,8 @@
 // It follows create_proof (proofs/src/plonk/prover.rs:432-474 -> compute_trace :66-299
 // -> finalise_proof :302-423) and multi_open (proofs/src/poly/kzg/mod.rs:110-289) step
 // for step, with the same deliberate differences as the Python form of this prover
-// (midnight-zk_amd/plonk.py): seeded random draws, Lagrange-basis permutation commits,
-// a descriptor string in the VK hash (the caller passes the hash).  Every phase issues
+// (midnight-zk_amd/plonk.py): seeded random draws and a descriptor string in the VK hash
+// (the caller passes the hash).  Every phase issues
 // one batched call for the whole group (commitments through mzk_msm_srs_batch_dev,
 // transforms through mzk_ntt_batch_dev, evaluations through mzk_fr_eval_batch), so a
 // group of proofs shares each pipeline; everything else is the library's vector
@@ -290,6 +290,8 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
   const size_t n = (size_t)1 << k, N = 4 * n, u = n - BF - 1;
   MZK_REQUIRE(key->g && key->g_lagrange && key->g->n >= n && key->g_lagrange->n >= n,
               "mzk_plonk_prove: SRS shorter than 2^%u", k);
+  MZK_REQUIRE(!key->g_lagrange_delta || key->g_lagrange_delta->n == n,
+              "mzk_plonk_prove: delta bases must hold exactly 2^%u points", k);
   const void* need[] = {key->l0, key->l_last, key->l_active, key->coset_x, key->omega_powers};
   for (const void* p : need) MZK_REQUIRE(p, "mzk_plonk_prove: key vector missing");
   for (int j = 0; j < NFIX; j++) MZK_REQUIRE(key->fixed_polys[j] && key->fixed_cosets[j], "mzk_plonk_prove: fixed %d", j);
@@ -445,10 +447,23 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
     }
   }
   {
+    // z has long constant runs (rows outside every copy cycle): committed as z.to_delta()
+    // against the suffix-sum bases (permutation/prover.rs:236-245), the zero deltas give the
+    // MSM no window digits.  The deltas go where the numerators were (free by now).
     std::vector<std::pair<const void*, size_t>> jobs;
+    const bool delta = key->g_lagrange_delta != nullptr;
+    uint32_t* deltas = arena + 8 * per * np;
     for (size_t p = 0; p < np; p++)
-      for (int s = 0; s < NSETS; s++) jobs.push_back({B[p].z[s], n});
-    PV_TRY(msm_batch(c, key->g_lagrange, jobs, jac));
+      for (int s = 0; s < NSETS; s++) {
+        if (delta) {
+          uint32_t* d = deltas + 8 * (p * NSETS + s) * n;
+          PV_TRY(mzk_fr_to_delta_dev(c, B[p].z[s], d, n, 1));
+          jobs.push_back({d, n});
+        } else {
+          jobs.push_back({B[p].z[s], n});
+        }
+      }
+    PV_TRY(msm_batch(c, delta ? key->g_lagrange_delta : key->g_lagrange, jobs, jac));
     for (size_t p = 0; p < np; p++)
       for (int s = 0; s < NSETS; s++) {
         uint8_t cb[48];

@@ -109,7 +109,7 @@ class PlonkKey(ctypes.Structure):
                 ("sigma_cosets", ctypes.c_void_p * 5),
                 ("l0", ctypes.c_void_p), ("l_last", ctypes.c_void_p), ("l_active", ctypes.c_void_p),
                 ("coset_x", ctypes.c_void_p), ("omega_powers", ctypes.c_void_p),
-                ("transcript_repr", ctypes.c_uint64 * 4)]
+                ("transcript_repr", ctypes.c_uint64 * 4), ("g_lagrange_delta", ctypes.c_void_p)]
 
 
 PLONK_PROOF_BYTES = 1360
@@ -197,6 +197,7 @@ SIGNATURES = {
     "mzk_fr_kate_div_batch": ([_P, _P, _S], _I),
     "mzk_fr_prefix_product_batch": ([_P, _P, _S], _I),
     "mzk_fr_powers": ([_P, _P, _S, _P], _I),
+    "mzk_fr_to_delta_dev": ([_P, _P, _P, _S, _I], _I),
 }
 
 

@@ -214,8 +214,10 @@ class SyntheticCircuit:
 
 # ---------------------------------------------------------------- keys
 class Params:
-    """ParamsKZG from unsafe_setup (params.rs:154-210): monomial bases g_i = [tau^i]G
-    and Lagrange bases [L_i(tau)]G, L_i(tau) = (tau^n - 1)/n * omega^i / (tau - omega^i)."""
+    """ParamsKZG from unsafe_setup (params.rs:154-210): monomial bases g_i = [tau^i]G,
+    Lagrange bases [L_i(tau)]G, L_i(tau) = (tau^n - 1)/n * omega^i / (tau - omega^i), and
+    their suffix sums g_lagrange_delta (params.rs:47-57, 186), the bases of the permutation
+    commitments."""
 
     def __init__(self, be, k: int, tau: int, g=None):
         """g: monomial bases of the same tau with at least 2^k points to share (commits
@@ -230,6 +232,7 @@ class Params:
         den = be.batch_invert(be.add_scalar(be.neg(w), self.tau))
         c = (pow(self.tau, n, R_MOD) - 1) * pow(n, -1, R_MOD) % R_MOD
         self.g_lagrange = be.srs_from_scalars(be.scale(be.mul(w, den), c))
+        self.g_lagrange_delta = be.srs_suffix_sum(self.g_lagrange)
 
 
 class ProvingKey:
@@ -422,7 +425,9 @@ def prove_steps(be, params: Params, pk: ProvingKey, advice_values, rng_seed: int
     for s in range(1, len(zs)):
         corr = last_uncorrected[s - 1] if corr is None else be.mul(corr, last_uncorrected[s - 1])
         zs[s] = be.mul(zs[s], be.broadcast(corr, n))
-    for c in (yield ("commit", [(g_lag, z) for z in zs])):
+    # z has long constant runs: committed as z.to_delta() against the suffix-sum bases
+    # (permutation/prover.rs:236-245), the same point with the runs' zero deltas skipped
+    for c in (yield ("commit", [(params.g_lagrange_delta, be.to_delta(z)) for z in zs])):
         tr.write_point(g1_compress(c))
     mark("permutation_commit")
     trash = tr.squeeze()  # noqa: F841  (no trashcans)
@@ -543,6 +548,8 @@ def create_proofs_native(be, params: Params, pk: ProvingKey, advices, rng_seeds,
     key.l0, key.l_last, key.l_active = p(pk.l0), p(pk.l_last), p(pk.l_active_row)
     key.coset_x, key.omega_powers = p(pk.coset_x), p(pk.omega_pows)
     key.transcript_repr[:] = [int(w) for w in mont_limbs(pk.transcript_repr)]
+    gd = params.g_lagrange_delta.h
+    key.g_lagrange_delta = gd.value if hasattr(gd, "value") else gd
     advices = list(advices)
     m = len(advices)
     cols = []
@@ -962,6 +969,16 @@ class GpuBackend:
         h = ctypes.c_void_p()
         self.mzk._check(self._L.mzk_srs_from_scalars_dev(self.ctx.h, self._p(s), s.shape[0], ctypes.byref(h)))
         return self.mzk.ResidentSrs(self.ctx, h)
+
+    def srs_suffix_sum(self, srs):
+        """suffix_sum (kzg/params.rs:47-57): the delta bases of srs."""
+        return srs.suffix_sum()
+
+    def to_delta(self, v, order: int = 1):
+        """Polynomial::to_delta / to_double_delta (poly/mod.rs:290-335)."""
+        out = self._empty(v.shape[0])
+        self.mzk._check(self._L.mzk_fr_to_delta_dev(self.ctx.h, self._p(v), self._p(out), v.shape[0], order))
+        return out
 
     def divide_by_vanishing(self, v, k: int, ek: int):
         """divide_by_vanishing_poly (domain.rs:361-380) on this backend's stream."""

@@ -1086,6 +1086,57 @@ void cpu_ref_fr_powers(const u64 *x, size_t n, u64 *out_raw) {
   for (size_t i = 0; i < n; i++) { out[i] = acc; fr_mul(&acc, &acc, (const fr_t *)x); }
 }
 /* [s_i]G for n scalars (the Lagrange bases of unsafe_setup, params.rs:174-186), affine out */
+/* suffix_sum (params.rs:47-57): out_i = sum_(j >= i) in_j, affine; one running Jacobian
+ * sum from the end, then all points normalised with one inversion (Montgomery's trick). */
+void cpu_ref_g1_suffix_sum(const u64 *in, size_t n, u64 *out) {
+  g1j_t *acc = (g1j_t *)malloc(sizeof(g1j_t) * (n + 1));
+  fp_t *pre = (fp_t *)malloc(sizeof(fp_t) * (n + 1));
+  g1j_t run;
+  g1j_set_inf(&run);
+  for (size_t i = n; i-- > 0;) {
+    g1j_add_affine(&run, &run, (const g1a_t *)(in + 12 * i));
+    acc[i] = run;
+  }
+  fp_t p;
+  memcpy(p.l, FP_ONE, 48);
+  for (size_t i = 0; i < n; i++) {          /* prefix products of the non-zero z */
+    pre[i] = p;
+    if (!g1j_is_inf(&acc[i])) fp_mul(&p, &p, &acc[i].z);
+  }
+  fp_t inv;
+  fp_inv(&inv, &p);
+  for (size_t i = n; i-- > 0;) {
+    g1a_t *r = (g1a_t *)(out + 12 * i);
+    if (g1j_is_inf(&acc[i])) { memset(r, 0, sizeof *r); continue; }
+    fp_t zi, zi2, zi3;
+    fp_mul(&zi, &inv, &pre[i]);               /* 1 / z_i */
+    fp_mul(&inv, &inv, &acc[i].z);
+    fp_sqr(&zi2, &zi);
+    fp_mul(&zi3, &zi2, &zi);
+    fp_mul(&r->x, &acc[i].x, &zi2);
+    fp_mul(&r->y, &acc[i].y, &zi3);
+  }
+  free(acc);
+  free(pre);
+}
+
+/* Polynomial::to_delta / to_double_delta (poly/mod.rs:290-335). */
+void cpu_ref_fr_to_delta(const u64 *a_raw, size_t n, int order, u64 *out_raw) {
+  const fr_t *a = (const fr_t *)a_raw;
+  fr_t *o = (fr_t *)out_raw;
+  for (size_t i = 0; i < n; i++) {
+    fr_t r = a[i];
+    if (i >= 1) {
+      fr_sub(&r, &r, &a[i - 1]);
+      if (order == 2) {
+        fr_sub(&r, &r, &a[i - 1]);
+        if (i >= 2) fr_add(&r, &r, &a[i - 2]);
+      }
+    }
+    o[i] = r;
+  }
+}
+
 void cpu_ref_g1_mul_generator_many(const u64 *s_mont, size_t n, u64 *out_aff) {
   comb_build();
   srs_ctx_t ctx = {NULL, n, (g1a_t *)out_aff, (const fr_t *)s_mont};

@@ -70,6 +70,8 @@ def lib():
             "cpu_ref_fr_kate_div": ([P, S, P, P], None),
             "cpu_ref_fr_powers": ([P, S, P], None),
             "cpu_ref_g1_mul_generator_many": ([P, S, P], None),
+            "cpu_ref_g1_suffix_sum": ([P, S, P], None),
+            "cpu_ref_fr_to_delta": ([P, S, ctypes.c_int, P], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -256,6 +258,22 @@ def fr_kate_div(a: np.ndarray, b_mont: np.ndarray) -> np.ndarray:
 def fr_powers(x_mont: np.ndarray, n: int) -> np.ndarray:
     out = np.zeros((n, 4), dtype=np.uint64)
     lib().cpu_ref_fr_powers(_p(_c(x_mont)), n, _p(out))
+    return out
+
+
+def g1_suffix_sum(bases: np.ndarray) -> np.ndarray:
+    """suffix_sum (params.rs:47-57) of affine bases (n x 12)."""
+    b = _c(bases)
+    out = np.zeros_like(b)
+    lib().cpu_ref_g1_suffix_sum(_p(b), b.shape[0], _p(out))
+    return out
+
+
+def fr_to_delta(a: np.ndarray, order: int = 1) -> np.ndarray:
+    """Polynomial::to_delta (order 1) / to_double_delta (order 2), poly/mod.rs:290-335."""
+    a = _c(a)
+    out = np.empty_like(a)
+    lib().cpu_ref_fr_to_delta(_p(a), a.shape[0], order, _p(out))
     return out
 
 

@@ -146,6 +146,12 @@ class CpuBackend:
     def srs_from_scalars(self, s):
         return CpuSrs(C.g1_mul_generator_many(s))
 
+    def srs_suffix_sum(self, srs):
+        return CpuSrs(C.g1_suffix_sum(srs.bases))
+
+    def to_delta(self, v, order: int = 1):
+        return C.fr_to_delta(v, order)
+
     def divide_by_vanishing(self, v, k, ek):
         return C.divide_by_vanishing_poly(v, k, ek)
 

@@ -247,3 +247,35 @@ def test_perm_products(be, n, ncols):
     assert L.mzk_plonk_perm_products_dev(be.ctx.h, dw_.data_ptr(), n, ctypes.cast(arr, ctypes.c_void_p), 2) == 0
     for (o_num, o_den), (num, den) in zip(outs, want):
         assert np.array_equal(host(be, o_num), num) and np.array_equal(host(be, o_den), den)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 257, (1 << 18) + 1])
+def test_delta_encodings(be, n):
+    a = C.fr_from_seed(8, n, n)
+    if n > 100:
+        a[50:90] = a[50]                                  # a constant run
+    d = to_dev(a)
+    for order in (1, 2):
+        assert np.array_equal(host(be, be.to_delta(d, order)), C.fr_to_delta(a, order)), order
+    with pytest.raises(mzk.MzkError, match="overlaps"):
+        be.mzk._check(be._L.mzk_fr_to_delta_dev(be.ctx.h, be._p(d), be._p(d), n, 1))
+
+
+def test_delta_basis_commit_equals_lagrange_commit(be):
+    # a z-like vector (constant runs with a few steps) committed as its deltas against the
+    # suffix-sum bases gives its Lagrange commitment (permutation/prover.rs:236-245)
+    k = 16
+    params = plonk.Params(be, k, 0xDE17A)
+    n = 1 << k
+    rng = np.random.default_rng(3)
+    steps = np.sort(rng.choice(n, 900, replace=False))
+    vals = C.fr_from_seed(9, 9, len(steps) + 1)
+    z = np.repeat(vals, np.diff(np.concatenate([[0], steps, [n]])), axis=0)
+    dz = to_dev(np.ascontiguousarray(z))
+    plain = be.commit_jobs([(params.g_lagrange, dz)])[0]
+    delta = be.commit_jobs([(params.g_lagrange_delta, be.to_delta(dz))])[0]
+    assert np.array_equal(plain, delta)
+    double = params.g_lagrange_delta.suffix_sum()
+    assert np.array_equal(be.commit_jobs([(double, be.to_delta(dz, 2))])[0], plain)
+    # the delta bases themselves against the C restatement
+    assert np.array_equal(params.g_lagrange_delta.download(), C.g1_suffix_sum(params.g_lagrange.download()))

@@ -241,3 +241,30 @@ def test_twiddle_table_ffts_with_any_primitive_root(log_n):
         b = a[:n_real] + [0] * (n - n_real)
         got = C.fft_coeff_to_extended(C.ints_to_fr_rows(b), tw, log_n, n_real)
         assert C.fr_rows_to_ints(got) == _dft_textbook(b, w), n_real
+
+
+def test_suffix_sum_and_delta_commitment_identity():
+    # params.rs:47-57: delta bases are suffix sums; committing a.to_delta() against them
+    # gives the commitment of a against the plain bases (sum_i b_i sum_(j>=i) G_j =
+    # sum_j a_j G_j), for the single and the double delta encodings
+    rng = random.Random(21)
+    n = 24
+    pts = [O.g1_mul(O.G1_GEN, rng.randrange(1, O.R_MOD)) for _ in range(n)]
+    pts[7] = None                                           # an identity base
+    pts[11] = O.g1_neg(pts[10])                             # a suffix that cancels
+    rows = np.array([O.g1_affine_mont_limbs(P) if P is not None else [0] * 12 for P in pts], dtype=np.uint64)
+    suf = C.g1_suffix_sum(rows)
+    for i in range(n):
+        expect = O.g1_sum([P for P in pts[i:] if P is not None])
+        assert O.g1_affine_from_mont_limbs(suf[i]) == expect, i
+    a = [rng.randrange(O.R_MOD) for _ in range(n)]
+    a[3:9] = [a[3]] * 6                                     # a constant run: zero deltas
+    d1 = C.fr_rows_to_ints(C.fr_to_delta(C.ints_to_fr_rows(a), 1))
+    assert d1 == [a[0]] + [(a[i] - a[i - 1]) % O.R_MOD for i in range(1, n)]
+    assert d1[4:9] == [0] * 5
+    d2 = C.fr_rows_to_ints(C.fr_to_delta(C.ints_to_fr_rows(a), 2))
+    assert d2 == [d1[0]] + [(d1[i] - d1[i - 1]) % O.R_MOD for i in range(1, n)]
+    plain = O.msm_naive(a, [P for P in pts])
+    assert O.msm_naive(d1, [O.g1_affine_from_mont_limbs(r) for r in suf]) == plain
+    suf2 = C.g1_suffix_sum(suf)
+    assert O.msm_naive(d2, [O.g1_affine_from_mont_limbs(r) for r in suf2]) == plain
