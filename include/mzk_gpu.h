@@ -344,6 +344,40 @@ int mzk_plonk_perm_products_dev(mzk_ctx *ctx, const void *omega_powers, size_t n
 int mzk_fr_lincomb_dev(mzk_ctx *ctx, const void *const *d_polys, const size_t *lens, const uint64_t *scalars,
                        size_t npolys, void *d_out, size_t out_len);
 
+/* ------------------------------------------------------------------ LogUp lookups
+ * compute_multiplicities (proofs/src/plonk/logup/prover.rs:411-477): for the first
+ * usable_rows rows, m_i = (number of selected input rows, over all ninputs compressed input
+ * columns, whose value is table_i) / (number of table rows holding table_i); rows from
+ * usable_rows on take blinding[0 .. n - usable_rows) (host memory).  Inputs are counted where
+ * the selector is non-zero.  A selected input value missing from the table gives
+ * MZK_ERR_INVALID naming its column and row (the reference panics).  All vectors hold n
+ * device Montgomery values; d_m must not overlap the others.  Up to 32 input columns. */
+int mzk_logup_multiplicities_dev(mzk_ctx *ctx, const void *d_selector, const void *const *d_inputs, size_t ninputs,
+                                 const void *d_table, size_t n, size_t usable_rows, const uint64_t *blinding,
+                                 void *d_m);
+/* compute_logderivative (logup/prover.rs:189-310): with 1/(v + beta) for every input and
+ * table value (one batch inversion, zeros kept), helper column k = sum over chunk k's
+ * inputs of 1/(f + beta); per row l_i = selector_i sum_k helper_k,i - m_i / (table_i + beta);
+ * aggregator Z_0 = 0, Z_j = l_0 + ... + l_(j-1) for j < n - blinding_factors, then the
+ * blinding values.  Outputs must not overlap inputs or each other. */
+typedef struct {
+  size_t n;                        /* rows */
+  size_t blinding_factors;         /* Z's last rows take the blinding values */
+  const void *selector, *multiplicities, *table;
+  const void *const *inputs;       /* compressed input columns, chunk after chunk (<= 32) */
+  size_t ninputs;
+  const uint32_t *chunk_sizes;     /* input columns per chunk; they sum to ninputs */
+  size_t nchunks;
+  uint64_t beta[4];
+  const uint64_t *blinding;        /* host memory, blinding_factors values */
+  void *const *helpers;            /* out: one column of n values per chunk */
+  void *aggregator;                /* out: n values */
+} mzk_logup_args;
+int mzk_logup_logderivative_dev(mzk_ctx *ctx, const mzk_logup_args *args);
+/* Inclusive running sums out_i = a_0 + ... + a_i (n <= 2^24; the LogUp aggregator's scan,
+ * logup/prover.rs:269-278).  d_out must not alias d_a. */
+int mzk_fr_prefix_sum(mzk_ctx *ctx, const void *d_a, void *d_out, size_t n);
+
 /* ------------------------------------------------------------------ native prover
  * create_proof (proofs/src/plonk/prover.rs:432-474) with multi_open
  * (proofs/src/poly/kzg/mod.rs:110-289) for the synthetic circuit (advice a..e, fixed

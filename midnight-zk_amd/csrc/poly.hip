@@ -119,21 +119,26 @@ __global__ void k_binv_down(const uint32_t* x, size_t m, const uint32_t* __restr
 static constexpr int SCAN_T = 256;
 static constexpr int SCAN_E = 16;   // elements per thread
 
-// Exclusive block scan of one field element per thread under multiplication.
-__device__ fr_t block_excl_prod(fr_t v, uint32_t* lds, fr_t& total) {
+// The scans run under multiplication (running products) or addition (running sums), a
+// uniform choice per launch.
+__device__ __forceinline__ fr_t scan_op(bool sum, const fr_t& x, const fr_t& y) { return sum ? add(x, y) : mul(x, y); }
+__device__ __forceinline__ fr_t scan_id(bool sum) { return sum ? fr_t::zero() : fr_t::one(); }
+
+// Exclusive block scan of one field element per thread.
+__device__ fr_t block_excl_scan(fr_t v, uint32_t* lds, fr_t& total, bool sum) {
   const int t = threadIdx.x;
   store<FrP>(lds + 8 * t, v);
   __syncthreads();
   fr_t x = v;
   for (int d = 1; d < SCAN_T; d <<= 1) {
-    fr_t y = t >= d ? load<FrP>(lds + 8 * (t - d)) : fr_t::one();
+    fr_t y = t >= d ? load<FrP>(lds + 8 * (t - d)) : scan_id(sum);
     __syncthreads();
-    x = mul(x, y);
+    x = scan_op(sum, x, y);
     store<FrP>(lds + 8 * t, x);
     __syncthreads();
   }
   total = load<FrP>(lds + 8 * (SCAN_T - 1));
-  fr_t excl = t > 0 ? load<FrP>(lds + 8 * (t - 1)) : fr_t::one();
+  fr_t excl = t > 0 ? load<FrP>(lds + 8 * (t - 1)) : scan_id(sum);
   __syncthreads();
   return excl;
 }
@@ -153,6 +158,7 @@ struct ProdJob {
 
 struct ProdBatch {
   uint32_t njobs;
+  uint32_t sum;   // 1: running sums, 0: running products
   uint32_t first_tile[PROD_MAXJ + 1];
   ProdJob j[PROD_MAXJ];
 };
@@ -174,14 +180,15 @@ __global__ void __launch_bounds__(SCAN_T) k_prod_tiles(const ProdBatch B) {
   const ProdJob& J = B.j[jb];
   const uint32_t tile = blockIdx.x - B.first_tile[jb];
   const size_t base = ((size_t)tile * SCAN_T + threadIdx.x) * SCAN_E;
-  fr_t p = fr_t::one();
+  const bool sum = B.sum != 0;
+  fr_t p = scan_id(sum);
   for (int k = 0; k < SCAN_E; k++)
-    if (base + k < J.n) p = mul(p, load<FrP>(J.a + 8 * (base + k)));
+    if (base + k < J.n) p = scan_op(sum, p, load<FrP>(J.a + 8 * (base + k)));
   fr_t total;
-  fr_t run = block_excl_prod(p, lds, total);
+  fr_t run = block_excl_scan(p, lds, total, sum);
   for (int k = 0; k < SCAN_E; k++) {
     if (base + k >= J.n) break;
-    run = mul(run, load<FrP>(J.a + 8 * (base + k)));
+    run = scan_op(sum, run, load<FrP>(J.a + 8 * (base + k)));
     store<FrP>(J.out + 8 * (base + k), run);
   }
   if (threadIdx.x == 0) store<FrP>(J.totals + 8 * tile, total);
@@ -194,16 +201,17 @@ __global__ void __launch_bounds__(SCAN_T) k_prod_totals(const ProdBatch B) {
   const ProdJob& J = B.j[blockIdx.x];
   const size_t tiles = B.first_tile[blockIdx.x + 1] - B.first_tile[blockIdx.x];
   const size_t base = (size_t)threadIdx.x * SCAN_E;
-  fr_t p = fr_t::one();
+  const bool sum = B.sum != 0;
+  fr_t p = scan_id(sum);
   for (int k = 0; k < SCAN_E; k++)
-    if (base + k < tiles) p = mul(p, load<FrP>(J.totals + 8 * (base + k)));
+    if (base + k < tiles) p = scan_op(sum, p, load<FrP>(J.totals + 8 * (base + k)));
   fr_t total;
-  fr_t run = block_excl_prod(p, lds, total);
+  fr_t run = block_excl_scan(p, lds, total, sum);
   for (int k = 0; k < SCAN_E; k++) {
     if (base + k >= tiles) break;
     fr_t v = load<FrP>(J.totals + 8 * (base + k));
     store<FrP>(J.totals + 8 * (base + k), run);
-    run = mul(run, v);
+    run = scan_op(sum, run, v);
   }
 }
 
@@ -217,7 +225,7 @@ __global__ void __launch_bounds__(256) k_prod_apply(const ProdBatch B) {
   if (tile == 0) return;
   const size_t i = (size_t)tile * PROD_TILE + (blockIdx.x % per_tile) * 256 + threadIdx.x;
   if (i >= J.n) return;
-  store<FrP>(J.out + 8 * i, mul(load<FrP>(J.out + 8 * i), load<FrP>(J.totals + 8 * tile)));
+  store<FrP>(J.out + 8 * i, scan_op(B.sum != 0, load<FrP>(J.out + 8 * i), load<FrP>(J.totals + 8 * tile)));
 }
 
 // ---------------------------------------------------------------- evaluation
@@ -687,14 +695,36 @@ static bool spans_overlap(const void* p, size_t pb, const void* q, size_t qb) {
   return a < b + qb && b < a + pb;
 }
 
+static int prefix_scan_batch(mzk_ctx* c, const mzk_prefix_job* jobs, size_t njobs, bool sum);
+
 int mzk_fr_prefix_product_batch(mzk_ctx* c, const mzk_prefix_job* jobs, size_t njobs) {
-  MZK_REQUIRE(c != nullptr && (jobs != nullptr || njobs == 0), "mzk_fr_prefix_product_batch: null argument");
+  return prefix_scan_batch(c, jobs, njobs, false);
+}
+
+int mzk_fr_prefix_sum(mzk_ctx* c, const void* a, void* out, size_t n) {
+  MZK_REQUIRE(c != nullptr, "null context");
+  if (n == 0) return MZK_OK;
+  MZK_REQUIRE(a && out, "mzk_fr_prefix_sum: null data");
+  mzk_prefix_job j;
+  j.d_a = a;
+  j.d_out = out;
+  j.n = n;
+  return prefix_scan_batch(c, &j, 1, true);
+}
+
+}  // extern "C"
+
+// Inclusive running products (sum = false) or sums (sum = true) of every job, each phase
+// one launch for up to PROD_MAXJ jobs.
+static int prefix_scan_batch(mzk_ctx* c, const mzk_prefix_job* jobs, size_t njobs, bool sum) {
+  const char* who = sum ? "mzk_fr_prefix_sum" : "mzk_fr_prefix_product_batch";
+  MZK_REQUIRE(c != nullptr && (jobs != nullptr || njobs == 0), "%s: null argument", who);
   std::vector<mzk_prefix_job> live;
   for (size_t i = 0; i < njobs; i++) {
     const mzk_prefix_job& j = jobs[i];
     if (j.n == 0) continue;
-    MZK_REQUIRE(j.d_a && j.d_out, "mzk_fr_prefix_product_batch: job %zu: null data", i);
-    MZK_REQUIRE(j.n <= (size_t)PROD_TILE * PROD_TILE, "mzk_fr_prefix_product_batch: job %zu: n above %zu", i,
+    MZK_REQUIRE(j.d_a && j.d_out, "%s: job %zu: null data", who, i);
+    MZK_REQUIRE(j.n <= (size_t)PROD_TILE * PROD_TILE, "%s: job %zu: n above %zu", who, i,
                 (size_t)PROD_TILE * PROD_TILE);
     live.push_back(j);
   }
@@ -702,9 +732,9 @@ int mzk_fr_prefix_product_batch(mzk_ctx* c, const mzk_prefix_job* jobs, size_t n
   for (size_t i = 0; i < live.size(); i++)
     for (size_t k = 0; k < live.size(); k++) {
       MZK_REQUIRE(!spans_overlap(live[i].d_out, live[i].n * 32, live[k].d_a, live[k].n * 32),
-                  "mzk_fr_prefix_product_batch: an output overlaps an input");
+                  "%s: an output overlaps an input", who);
       MZK_REQUIRE(i == k || !spans_overlap(live[i].d_out, live[i].n * 32, live[k].d_out, live[k].n * 32),
-                  "mzk_fr_prefix_product_batch: two outputs overlap");
+                  "%s: two outputs overlap", who);
     }
   if (live.empty()) return MZK_OK;
   MZK_TRY(vec_ready(c));
@@ -718,6 +748,7 @@ int mzk_fr_prefix_product_batch(mzk_ctx* c, const mzk_prefix_job* jobs, size_t n
     ProdBatch B;
     memset(&B, 0, sizeof B);
     B.njobs = (uint32_t)nj;
+    B.sum = sum ? 1 : 0;
     uint32_t t = 0;
     for (size_t i = 0; i < nj; i++) {
       const mzk_prefix_job& j = live[i0 + i];
@@ -738,6 +769,8 @@ int mzk_fr_prefix_product_batch(mzk_ctx* c, const mzk_prefix_job* jobs, size_t n
   MZK_HIP(hipGetLastError());
   return MZK_OK;
 }
+
+extern "C" {
 
 int mzk_fr_prefix_product(mzk_ctx* c, const void* a, void* out, size_t n) {
   MZK_REQUIRE(c != nullptr, "null context");

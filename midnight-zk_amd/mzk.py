@@ -101,6 +101,15 @@ class KateJob(ctypes.Structure):
                 ("d_q", ctypes.c_void_p)]
 
 
+class LogupArgs(ctypes.Structure):
+    """mzk_logup_args: the log-derivative pass of one LogUp argument."""
+    _fields_ = [("n", ctypes.c_size_t), ("blinding_factors", ctypes.c_size_t),
+                ("selector", ctypes.c_void_p), ("multiplicities", ctypes.c_void_p), ("table", ctypes.c_void_p),
+                ("inputs", ctypes.c_void_p), ("ninputs", ctypes.c_size_t), ("chunk_sizes", ctypes.c_void_p),
+                ("nchunks", ctypes.c_size_t), ("beta", ctypes.c_uint64 * 4), ("blinding", ctypes.c_void_p),
+                ("helpers", ctypes.c_void_p), ("aggregator", ctypes.c_void_p)]
+
+
 class PlonkKey(ctypes.Structure):
     """mzk_plonk_key: the native prover's view of a proving key (device pointers)."""
     _fields_ = [("k", ctypes.c_uint32), ("g", ctypes.c_void_p), ("g_lagrange", ctypes.c_void_p),
@@ -198,6 +207,9 @@ SIGNATURES = {
     "mzk_fr_prefix_product_batch": ([_P, _P, _S], _I),
     "mzk_fr_powers": ([_P, _P, _S, _P], _I),
     "mzk_fr_to_delta_dev": ([_P, _P, _P, _S, _I], _I),
+    "mzk_fr_prefix_sum": ([_P, _P, _P, _S], _I),
+    "mzk_logup_multiplicities_dev": ([_P, _P, _P, _S, _P, _S, _S, _P, _P], _I),
+    "mzk_logup_logderivative_dev": ([_P, _P], _I),
 }
 
 
@@ -473,6 +485,44 @@ class Context:
 
     def srs_upload(self, bases: np.ndarray) -> "ResidentSrs":
         return ResidentSrs.upload(self, bases)
+
+    # ------------------------------------------------------------------ LogUp
+    @_ordered(_self_ctx)
+    def logup_multiplicities(self, d_selector, d_inputs, d_table, usable_rows: int, blinding: np.ndarray, d_out):
+        """compute_multiplicities (logup/prover.rs:411-477) into d_out; blinding: host
+        (n - usable_rows, 4) Montgomery rows."""
+        n = d_table.shape[0]
+        ins = (ctypes.c_void_p * max(1, len(d_inputs)))(*[_t_ptr(d) for d in d_inputs])
+        bl = np.ascontiguousarray(blinding, dtype=np.uint64).reshape(-1, 4)
+        if bl.shape[0] != n - usable_rows:
+            raise MzkError("logup_multiplicities: one blinding value per row from usable_rows on")
+        _check(load_library().mzk_logup_multiplicities_dev(
+            self.h, _t_ptr(d_selector), ctypes.cast(ins, ctypes.c_void_p), len(d_inputs), _t_ptr(d_table), n,
+            usable_rows, _np_ptr(bl) if bl.shape[0] else None, _t_ptr(d_out)))
+
+    @_ordered(_self_ctx)
+    def logup_logderivative(self, d_selector, d_m, d_table, d_inputs, chunk_sizes, beta_mont: np.ndarray,
+                            blinding: np.ndarray, d_helpers, d_aggregator):
+        """compute_logderivative (logup/prover.rs:189-310): helper columns and aggregator."""
+        a = LogupArgs()
+        bl = np.ascontiguousarray(blinding, dtype=np.uint64).reshape(-1, 4)
+        ins = (ctypes.c_void_p * max(1, len(d_inputs)))(*[_t_ptr(d) for d in d_inputs])
+        hs = (ctypes.c_void_p * max(1, len(d_helpers)))(*[_t_ptr(d) for d in d_helpers])
+        cs = np.asarray(chunk_sizes, dtype=np.uint32)
+        a.n = d_table.shape[0]
+        a.blinding_factors = bl.shape[0]
+        a.selector, a.multiplicities, a.table = _t_ptr(d_selector), _t_ptr(d_m), _t_ptr(d_table)
+        a.inputs, a.ninputs = ctypes.cast(ins, ctypes.c_void_p), len(d_inputs)
+        a.chunk_sizes, a.nchunks = cs.ctypes.data, len(cs)
+        a.beta[:] = [int(w) for w in np.asarray(beta_mont, dtype=np.uint64).reshape(4)]
+        a.blinding = bl.ctypes.data if bl.shape[0] else None
+        a.helpers, a.aggregator = ctypes.cast(hs, ctypes.c_void_p), _t_ptr(d_aggregator)
+        _check(load_library().mzk_logup_logderivative_dev(self.h, ctypes.byref(a)))
+
+    @_ordered(_self_ctx)
+    def fr_prefix_sum(self, d_a, d_out, n: int | None = None):
+        n = d_a.shape[0] if n is None else n
+        _check(load_library().mzk_fr_prefix_sum(self.h, _t_ptr(d_a), _t_ptr(d_out), n))
 
     # ------------------------------------------------------------------ inputs
     @_ordered(_self_ctx)

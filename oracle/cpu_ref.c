@@ -1120,6 +1120,110 @@ void cpu_ref_g1_suffix_sum(const u64 *in, size_t n, u64 *out) {
   free(pre);
 }
 
+/* ------------------------------------------------------------------ LogUp (logup/prover.rs) */
+typedef struct { fr_t v; size_t idx; } keyed_t;
+static int fr_cmp(const fr_t *a, const fr_t *b) {
+  for (int i = 3; i >= 0; i--)
+    if (a->l[i] != b->l[i]) return a->l[i] < b->l[i] ? -1 : 1;
+  return 0;
+}
+static int keyed_cmp(const void *x, const void *y) {
+  const keyed_t *a = (const keyed_t *)x, *b = (const keyed_t *)y;
+  int c = fr_cmp(&a->v, &b->v);
+  return c ? c : (a->idx < b->idx ? -1 : a->idx > b->idx);
+}
+/* compute_multiplicities (:411-477): counts of the usable table rows' values among the
+ * selected usable input rows, over the table counts; blinding values on the other rows.
+ * Returns 0, or 1 + (c n + i) for the first selected input value (column c, row i; columns
+ * outer, as the reference iterates) missing from the table, where the reference panics. */
+long long cpu_ref_logup_multiplicities(const u64 *sel, const u64 *const *inputs, size_t ninputs, const u64 *table,
+                                       size_t n, size_t usable, const u64 *blinding, u64 *m_out) {
+  keyed_t *k = (keyed_t *)malloc(sizeof(keyed_t) * (usable + 1));
+  size_t *run_of = (size_t *)malloc(sizeof(size_t) * (usable + 1));   /* sorted position -> run */
+  size_t *pos_of = (size_t *)malloc(sizeof(size_t) * (usable + 1));   /* table row -> sorted position */
+  u64 *tcnt = (u64 *)calloc(usable + 1, 8), *icnt = (u64 *)calloc(usable + 1, 8);
+  for (size_t i = 0; i < usable; i++) { memcpy(k[i].v.l, table + 4 * i, 32); k[i].idx = i; }
+  qsort(k, usable, sizeof(keyed_t), keyed_cmp);
+  size_t runs = 0;
+  for (size_t j = 0; j < usable; j++) {
+    if (j > 0 && fr_cmp(&k[j].v, &k[j - 1].v) != 0) runs++;
+    run_of[j] = runs;
+    pos_of[k[j].idx] = j;
+    tcnt[runs]++;
+  }
+  long long bad = 0;
+  for (size_t c = 0; c < ninputs && !bad; c++)
+    for (size_t i = 0; i < usable; i++) {
+      const u64 *s = sel + 4 * i;
+      if (!(s[0] | s[1] | s[2] | s[3])) continue;
+      fr_t v;
+      memcpy(v.l, inputs[c] + 4 * i, 32);
+      size_t lo = 0, hi = usable;   /* first position with value >= v */
+      while (lo < hi) {
+        size_t mid = (lo + hi) / 2;
+        if (fr_cmp(&k[mid].v, &v) < 0) lo = mid + 1; else hi = mid;
+      }
+      if (lo == usable || fr_cmp(&k[lo].v, &v) != 0) { bad = 1 + (long long)(c * n + i); break; }
+      icnt[run_of[lo]]++;
+    }
+  if (!bad) {
+    u64 *den = (u64 *)malloc(32 * (n + 1));
+    for (size_t i = 0; i < n; i++) {
+      if (i < usable) {
+        size_t r = run_of[pos_of[i]];
+        fr_from_u64((fr_t *)(m_out + 4 * i), icnt[r]);
+        fr_from_u64((fr_t *)(den + 4 * i), tcnt[r]);
+      } else {
+        memcpy(m_out + 4 * i, blinding + 4 * (i - usable), 32);
+        memcpy(den + 4 * i, FR_ONE, 32);
+      }
+    }
+    cpu_ref_fr_batch_invert(den, n);
+    for (size_t i = 0; i < usable; i++) fr_mul((fr_t *)(m_out + 4 * i), (fr_t *)(m_out + 4 * i), (fr_t *)(den + 4 * i));
+    free(den);
+  }
+  free(k); free(run_of); free(pos_of); free(tcnt); free(icnt);
+  return bad;
+}
+
+/* compute_logderivative (:189-310): helpers per chunk (sums of batch-inverted input
+ * denominators), the row log-derivatives and the aggregator with its blinding rows. */
+void cpu_ref_logup_logderivative(size_t n, size_t bf, const u64 *sel, const u64 *m, const u64 *table,
+                                 const u64 *const *inputs, size_t ninputs, const uint32_t *chunk_sizes, size_t nchunks,
+                                 const u64 *beta, const u64 *blinding, u64 *const *helpers, u64 *z) {
+  (void)ninputs;
+  u64 *tden = (u64 *)malloc(32 * n), *den = (u64 *)malloc(32 * n), *logd = (u64 *)malloc(32 * n);
+  for (size_t i = 0; i < n; i++) fr_add((fr_t *)(tden + 4 * i), (const fr_t *)beta, (const fr_t *)(table + 4 * i));
+  cpu_ref_fr_batch_invert(tden, n);
+  size_t col = 0;
+  for (size_t k = 0; k < nchunks; k++) {
+    fr_t *h = (fr_t *)helpers[k];
+    memset(h, 0, 32 * n);
+    for (uint32_t c = 0; c < chunk_sizes[k]; c++, col++) {
+      for (size_t i = 0; i < n; i++) fr_add((fr_t *)(den + 4 * i), (const fr_t *)beta, (const fr_t *)(inputs[col] + 4 * i));
+      cpu_ref_fr_batch_invert(den, n);
+      for (size_t i = 0; i < n; i++) fr_add(&h[i], &h[i], (const fr_t *)(den + 4 * i));
+    }
+  }
+  for (size_t i = 0; i < n; i++) {
+    fr_t sum, a, b;
+    memset(&sum, 0, 32);
+    for (size_t k = 0; k < nchunks; k++) fr_add(&sum, &sum, (const fr_t *)(helpers[k] + 4 * i));
+    fr_mul(&a, (const fr_t *)(sel + 4 * i), &sum);
+    fr_mul(&b, (const fr_t *)(m + 4 * i), (const fr_t *)(tden + 4 * i));
+    fr_sub((fr_t *)(logd + 4 * i), &a, &b);
+  }
+  fr_t state;
+  memset(&state, 0, 32);
+  memset(z, 0, 32);
+  for (size_t j = 0; j + 1 < n - bf; j++) {
+    fr_add(&state, &state, (const fr_t *)(logd + 4 * j));
+    memcpy(z + 4 * (j + 1), state.l, 32);
+  }
+  for (size_t t = 0; t < bf; t++) memcpy(z + 4 * (n - bf + t), blinding + 4 * t, 32);
+  free(tden); free(den); free(logd);
+}
+
 /* Polynomial::to_delta / to_double_delta (poly/mod.rs:290-335). */
 void cpu_ref_fr_to_delta(const u64 *a_raw, size_t n, int order, u64 *out_raw) {
   const fr_t *a = (const fr_t *)a_raw;

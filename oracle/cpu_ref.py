@@ -72,6 +72,8 @@ def lib():
             "cpu_ref_g1_mul_generator_many": ([P, S, P], None),
             "cpu_ref_g1_suffix_sum": ([P, S, P], None),
             "cpu_ref_fr_to_delta": ([P, S, ctypes.c_int, P], None),
+            "cpu_ref_logup_multiplicities": ([P, P, S, P, S, S, P, P], ctypes.c_longlong),
+            "cpu_ref_logup_logderivative": ([S, S, P, P, P, P, S, P, S, P, P, P, P], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -275,6 +277,40 @@ def fr_to_delta(a: np.ndarray, order: int = 1) -> np.ndarray:
     out = np.empty_like(a)
     lib().cpu_ref_fr_to_delta(_p(a), a.shape[0], order, _p(out))
     return out
+
+
+def _ptrs(arrays):
+    return (ctypes.c_void_p * max(1, len(arrays)))(*[a.ctypes.data for a in arrays])
+
+
+def logup_multiplicities(selector, inputs, table, usable: int, blinding) -> np.ndarray:
+    """compute_multiplicities (logup/prover.rs:411-477).  Raises ValueError naming the
+    first selected input value (column, row) missing from the table."""
+    sel, tab = _c(selector), _c(table)
+    ins = [_c(x) for x in inputs]
+    n = tab.shape[0]
+    bl = _c(blinding) if len(blinding) else np.zeros((1, 4), dtype=np.uint64)
+    out = np.zeros((n, 4), dtype=np.uint64)
+    bad = lib().cpu_ref_logup_multiplicities(_p(sel), _ptrs(ins), len(ins), _p(tab), n, usable, _p(bl), _p(out))
+    if bad:
+        c, i = divmod(bad - 1, n)
+        raise ValueError(f"input {c} row {i}: value not found in the lookup table")
+    return out
+
+
+def logup_logderivative(selector, m, table, inputs, chunk_sizes, beta_mont, blinding, bf: int):
+    """compute_logderivative (logup/prover.rs:189-310): (helper columns, aggregator)."""
+    sel, mm, tab = _c(selector), _c(m), _c(table)
+    n = tab.shape[0]
+    ins = [_c(x) for x in inputs]
+    cs = np.asarray(chunk_sizes, dtype=np.uint32)
+    helpers = [np.zeros((n, 4), dtype=np.uint64) for _ in chunk_sizes]
+    z = np.zeros((n, 4), dtype=np.uint64)
+    bl = _c(blinding) if bf else np.zeros((1, 4), dtype=np.uint64)
+    lib().cpu_ref_logup_logderivative(n, bf, _p(sel), _p(mm), _p(tab), _ptrs(ins), len(ins),
+                                      cs.ctypes.data_as(ctypes.c_void_p), len(cs), _p(_c(beta_mont)), _p(bl),
+                                      _ptrs(helpers), _p(z))
+    return helpers, z
 
 
 def g1_mul_generator_many(s_mont: np.ndarray) -> np.ndarray:

@@ -112,6 +112,32 @@ static void vector_checks(size_t n) {
   free(a); free(b); free(o); free(inv); free(q); free(pw); free(aff);
 }
 
+static void logup_checks(void) {
+  enum { N = 64, BF = 5, U = N - BF - 1, NC = 3 };
+  u64 *table = (u64 *)malloc(32 * N), *sel = (u64 *)malloc(32 * N), *m = (u64 *)malloc(32 * N);
+  u64 *in[NC], *h[2], *z = (u64 *)malloc(32 * N), bl[(N - U) * 4], beta[4];   /* m takes N - U */
+  cpu_ref_fr_from_seed(31, 1, 0, N, table);
+  for (int i = U / 4; i < U; i++) memcpy(table + 4 * i, table + 4 * (i % (U / 4)), 32);   /* duplicates */
+  for (int i = 0; i < N; i++) { memset(sel + 4 * i, 0, 32); if (i % 3) cpu_ref_fr_powers(beta, 1, sel + 4 * i); }
+  for (int c = 0; c < NC; c++) {
+    in[c] = (u64 *)malloc(32 * N);
+    for (int i = 0; i < N; i++) memcpy(in[c] + 4 * i, table + 4 * ((i * 7 + c) % U), 32);
+  }
+  cpu_ref_fr_from_seed(31, 2, 0, N - U, bl);
+  cpu_ref_fr_from_seed(31, 3, 0, 1, beta);
+  CHECK(cpu_ref_logup_multiplicities(sel, (const u64 *const *)in, NC, table, N, U, bl, m) == 0, "logup multiplicities");
+  uint32_t cs[2] = {2, 1};
+  h[0] = (u64 *)malloc(32 * N);
+  h[1] = (u64 *)malloc(32 * N);
+  cpu_ref_logup_logderivative(N, BF, sel, m, table, (const u64 *const *)in, NC, cs, 2, beta, bl, h, z);
+  CHECK(!(z[4 * U] | z[4 * U + 1] | z[4 * U + 2] | z[4 * U + 3]), "logup balance");
+  memcpy(in[1] + 4 * 2, bl, 32);   /* a value outside the table on a selected row */
+  CHECK(cpu_ref_logup_multiplicities(sel, (const u64 *const *)in, NC, table, N, U, bl, m) == 1 + N + 2,
+        "logup missing value");
+  for (int c = 0; c < NC; c++) free(in[c]);
+  free(table); free(sel); free(m); free(z); free(h[0]); free(h[1]);
+}
+
 int main(void) {
   cpu_ref_set_threads(4);
   u64 tau[4];
@@ -120,6 +146,7 @@ int main(void) {
   for (size_t i = 0; i < sizeof sizes / sizeof sizes[0]; i++) msm_checks(sizes[i], tau);
   for (unsigned ln = 1; ln <= 12; ln += 1) fft_checks(ln);
   vector_checks(1000);
+  logup_checks();
   u64 g[12];
   cpu_ref_g1_generator(g);
   uint8_t rep[32] = {0xff, 0x01, 0x80};
