@@ -1087,6 +1087,19 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
     for (auto& j : jobs) ref_end = std::max<uint64_t>(ref_end, (uint64_t)j.ref_offset + j.n);
     if (!p.per_window) ref_end += (uint64_t)(p.W - 1) * ref_stride;
     const uint32_t ref_bits = (uint32_t)ceil_log2(std::max<uint64_t>(ref_end, 2));
+    // fewer low bits (more top-level bins) when that makes the word fit: one word per
+    // digit through the scatter and the bins, and more, shorter bins for k_sort_bins
+    // (5 x 2^18 shared sets against a 2^20 SRS: 320 bins of 2^9 -> 1280 of 2^7,
+    // k_sort_bins 0.39 -> 0.13 ms)
+    if (sa.l2 + ref_bits + 1 > 32 && ref_bits + 1 <= 32) {
+      const uint32_t l2 = 31 - ref_bits;
+      const size_t D1 = ((size_t)NB + (1u << l2) - 1) >> l2;
+      const size_t hn_tiles = (D1 * sa.G + SCAN_TILE - 1) / SCAN_TILE;
+      if (D1 <= SCAT_MAX_D1 && hn_tiles <= (size_t)SCAN_TILE) {
+        sa.l2 = l2;
+        sa.D1 = (uint32_t)D1;
+      }
+    }
     sa.pack = sa.l2 + ref_bits + 1 <= 32 ? ref_bits + 1 : 0;
   }
   const size_t HN = (size_t)sa.D1 * sa.G;   // histogram entries
