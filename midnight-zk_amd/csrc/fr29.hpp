@@ -100,19 +100,21 @@ MZK_HD29 fr29 fr29_add(const fr29& a, const fr29& b) {
   return o;
 }
 
-// a + K r - b for b < K r (signed limb differences, carries propagated)
+// a + K r - b for b < K r (signed limb differences, carries propagated).  K <= 2: every
+// limb sum a_i + K r_i - b_i + c lies in (-2^29 - 2, 2^29 + 2^30 + 4), so 32-bit signed
+// arithmetic holds it (four instructions a limb instead of a 64-bit chain).
 template <int K>
 MZK_HD29 fr29 fr29_sub_kr(const fr29& a, const fr29& b) {
+  static_assert(K >= 0 && K <= 2, "sub_kr: K r must keep the limb sums within 32 bits");
   fr29 o;
-  int64_t c = 0;
+  int32_t c = 0;
 #pragma unroll
   for (int i = 0; i < r29c::N - 1; i++) {
-    const int64_t t = (int64_t)a.l[i] + (int64_t)K * r29c::R(i) - (int64_t)b.l[i] + c;
+    const int32_t t = (int32_t)(a.l[i] + (uint32_t)K * r29c::R(i) - b.l[i]) + c;
     o.l[i] = (uint32_t)t & r29c::M29;
     c = t >> 29;   // arithmetic: the borrow is -1, 0 or a small carry
   }
-  o.l[r29c::N - 1] = (uint32_t)((int64_t)a.l[r29c::N - 1] + (int64_t)K * r29c::R(r29c::N - 1) -
-                                (int64_t)b.l[r29c::N - 1] + c);
+  o.l[r29c::N - 1] = a.l[r29c::N - 1] + (uint32_t)K * r29c::R(r29c::N - 1) - b.l[r29c::N - 1] + (uint32_t)c;
   return o;
 }
 
