@@ -1055,9 +1055,17 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   // tiles of at most ~2048 scalars, never crossing a job
   std::vector<JobDesc> jd(nj);
   std::vector<TileDesc> td;
-  // ~1024-1536 tiles plus at most one partial tile per job keeps the (bin, tile)
-  // histogram within the two-level scan (2048 bins x 2048 tiles)
-  const size_t tile = std::max<size_t>(256, (ntot + 1535) / 1536);
+  // top-level bins: 512 when the second level's 2^l2 counters allow it (longer runs per
+  // bin in the scatter), more for large key spaces (the second level keeps <= 2^11)
+  const int nb_bits = ceil_log2(NB);
+  const uint32_t l2_top = (uint32_t)std::min(11, std::max(0, nb_bits - 9));
+  const size_t D1_top = ((size_t)NB + (1u << l2_top) - 1) >> l2_top;
+  // ~1024-1536 tiles plus at most one partial tile per job, fewer when the bins are many:
+  // the (bin, tile) histogram stays within the two-level scan (2048 x 2048 entries)
+  const size_t max_tiles = (size_t)SCAN_TILE * SCAN_TILE / std::max<size_t>(D1_top, 1);
+  MZK_REQUIRE(max_tiles > nj + 1, "msm: %zu bucket sets too many for the sort", sets);
+  const size_t tile_target = std::min<size_t>(1536, max_tiles - nj - 1);
+  const size_t tile = std::max<size_t>(256, (ntot + tile_target - 1) / tile_target);
   MZK_REQUIRE(nj < 512, "msm: at most 511 jobs per batch (%zu given)", nj);
   size_t canon_off = 0;
   for (size_t j = 0; j < nj; j++) {
@@ -1076,11 +1084,8 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   sa.c = p.c;
   sa.W = p.W;
   sa.per_window = p.per_window;
-  // top-level bins: 512 when the second level's 2^l2 counters allow it (longer runs per
-  // bin in the scatter), more for large key spaces (the second level keeps <= 2^11)
-  const int nb_bits = ceil_log2(NB);
-  sa.l2 = (uint32_t)std::min(11, std::max(0, nb_bits - 9));
-  sa.D1 = (uint32_t)((NB + (1u << sa.l2) - 1) >> sa.l2);
+  sa.l2 = l2_top;
+  sa.D1 = (uint32_t)D1_top;
   sa.ref_stride = ref_stride;
   {   // one word per sorted digit when the low key bits, the reference and its sign fit
     uint64_t ref_end = 0;   // references are below this

@@ -95,6 +95,29 @@ def test_few_large_commits_against_2p20_srs(ctx):
 
 
 @pytest.mark.timeout(600)
+def test_many_shared_sets_commits(ctx):
+    """200 commits of 2^15 terms in one batch: 200 shared bucket sets, whose sort bins
+    need fewer, larger scalar tiles to keep the (bin, tile) histogram within the scan;
+    each against the closed form."""
+    n_srs, n, nj = 1 << 16, 1 << 15, 200
+    tau = C.fr_from_seed(0xC6, 0, 1)[0]
+    tau_i = C.fr_rows_to_ints([tau])[0]
+    srs = ctx.srs_generate(tau, n_srs)
+    buf = empty_dev(nj * n, 4)
+    ctx.fr_from_seed_dev(98, 7, nj * n, buf)
+    ctx.synchronize()
+    offs = [(i * 331) % (n_srs - n + 1) for i in range(nj)]
+    jobs = [(buf[i * n:(i + 1) * n], n, off) for i, off in enumerate(offs)]
+    outs = srs.commit_batch(jobs)
+    sc = from_dev(buf).reshape(-1, 4)
+    for i, ((_, _, off), out) in enumerate(zip(jobs, outs)):
+        s = C.fr_rows_to_ints([C.poly_eval(sc[i * n:(i + 1) * n], tau)])[0] * pow(tau_i, off, O.R_MOD) % O.R_MOD
+        expect = O.g1_affine_from_mont_limbs(C.g1_mul_generator(C.ints_to_fr_rows([s])[0]))
+        assert O.g1_jacobian_from_mont_limbs(out) == expect, (i, off)
+    srs.close()
+
+
+@pytest.mark.timeout(600)
 def test_config4_coset_ntts(ctx):
     sizes, _ = config4_jobs(1 << 20)
     print(f"\n[config 4: 96 coset NTTs, {sum(sizes)} coefficients]", flush=True)

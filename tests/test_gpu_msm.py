@@ -125,6 +125,35 @@ def test_resident_srs_shared_bucket_path_with_offset(ctx, srs16):
     srs.close()
 
 
+def test_shared_bucket_path_edge_scalars(ctx, srs16):
+    """Edge scalars through the shared bucket set (every window's digits of one scalar
+    in one set): values on both sides of (r - 1) / 2, r - 1, r - 2, 0, 1, 2, 2^253,
+    2^254 and window-boundary values scattered among random scalars, the same scalars
+    negated, and all scalars in the upper half, against msm_best."""
+    tau, bases = srs16
+    srs = ctx.srs_upload(bases[:40000])
+    n = 20000
+    sc = C.fr_from_seed(13, 1, n)
+    R = O.R_MOD
+    half = (R - 1) // 2
+    edge = [0, 1, 2, half - 1, half, half + 1, half + 2, R - 1, R - 2, 1 << 253, 1 << 254,
+            (1 << 254) - 1, R - (1 << 253), (1 << 17) - 1, 1 << 16, R - (1 << 16)]
+    rows = C.ints_to_fr_rows(edge)
+    for k, row in enumerate(rows):
+        sc[k * 1237 % n] = row
+        sc[n - 1 - k] = row
+    d = to_dev(sc)
+    assert aff(srs.commit(d, n)) == aff(C.msm_best(sc, bases[:n]))
+    # the same scalars negated: each term's point negated, the sum too
+    neg = C.ints_to_fr_rows([(R - v) % R for v in C.fr_rows_to_ints(sc)])
+    assert aff(srs.commit(to_dev(neg), n)) == aff(C.msm_best(neg, bases[:n]))
+    # all scalars in the upper half at once
+    up = C.ints_to_fr_rows([half + 1 + (v % half) for v in C.fr_rows_to_ints(sc)])
+    assert aff(srs.commit(to_dev(up), n)) == aff(C.msm_best(up, bases[:n]))
+    assert ctx.msm_last_plan()["bucket_sets"] == 1
+    srs.close()
+
+
 def test_msm_2_20_closed_form(ctx):
     """The metric configuration: n = 2^20, bases [tau^i]G, checked against
     [poly_s(tau)] G (one Horner pass and one scalar multiplication)."""
