@@ -95,15 +95,26 @@ struct JobDesc {
   uint32_t set_base;
 };
 
-// A tile of one job's scalars, handled by one sort workgroup.
+// A tile of one job's scalars, handled by one sort workgroup.  Jobs are sorted in groups
+// of consecutive jobs (a group's buckets are [kbase, kbase + NBg)); the tile's histogram
+// column is hist[hbase + bin * hstride] (hbase = D1 * the group's first tile + the tile's
+// index in its group, hstride = the group's tile count).
 struct TileDesc {
   uint32_t job, lo, hi;
+  uint32_t hbase, hstride, kbase;
+};
+
+// A sort group: its first tile and tile count (its histogram block starts at D1 * tile0).
+struct GroupDesc {
+  uint32_t tile0, G;
 };
 
 struct SortArgs {
   const JobDesc* jobs;
   const TileDesc* tiles;
+  const GroupDesc* groups;
   uint32_t G;            // tiles
+  uint32_t NBg;          // buckets per sort group (the last group may have fewer)
   int c, W, per_window;
   uint32_t l2;           // low key bits sorted in k_sort_bins
   uint32_t D1;           // bins of the top key bits
@@ -159,11 +170,11 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_hist(SortArgs a, uint32_t* 
     uint32_t carry = 0;
     for (int w = 0; w < a.W; w++) {
       int d = window_digit(s.v, w, a.c, carry);
-      if (d != 0) atomicAdd(&cnt[bucket_key(a, j, w, d) >> a.l2], 1u);
+      if (d != 0) atomicAdd(&cnt[(bucket_key(a, j, w, d) - tl.kbase) >> a.l2], 1u);
     }
   }
   __syncthreads();
-  for (uint32_t b = threadIdx.x; b < a.D1; b += SORT_BLOCK) hist[(size_t)b * a.G + blockIdx.x] = cnt[b];
+  for (uint32_t b = threadIdx.x; b < a.D1; b += SORT_BLOCK) hist[tl.hbase + (size_t)b * tl.hstride] = cnt[b];
 }
 
 // 3. scatter into bins: tmp_key[pos] = key & (2^l2 - 1), tmp_ref[pos] = reference | sign.
@@ -174,6 +185,9 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_hist(SortArgs a, uint32_t* 
 static constexpr int SCAT_WG = 16;   // 16 windows per round: 2^20 scatter 0.196 -> 0.187 ms against 8
 static constexpr uint32_t SCAT_MAX_D1 = 8192;   // LDS: (3 D1 + 3 SCAT_SLOTS + SORT_BLOCK) words <= 148.5 KiB
 static constexpr int SCAT_SLOTS = SORT_BLOCK * SCAT_WG;
+// buckets per sort group (batches of many bucket sets sort a group of jobs at a time:
+// 96 shared sets of 2^15 in one group made k_sort_bins 10 ms of a 55 ms batch)
+static constexpr size_t SORT_GROUP_BUCKETS = 1u << 18;
 
 __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __restrict__ canon, SortArgs a,
                                                              const uint32_t* __restrict__ hoff,
@@ -188,9 +202,9 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __r
   uint32_t* sref = skey + SCAT_SLOTS;   // staged references, bin order
   uint32_t* psum = sref + SCAT_SLOTS;   // SORT_BLOCK partial sums of the offset scan
   const uint32_t tid = threadIdx.x;
-  for (uint32_t b = tid; b < a.D1; b += SORT_BLOCK) cur[b] = hoff[(size_t)b * a.G + blockIdx.x];
-  const uint32_t mask2 = (1u << a.l2) - 1u;
   const TileDesc tl = a.tiles[blockIdx.x];
+  for (uint32_t b = tid; b < a.D1; b += SORT_BLOCK) cur[b] = hoff[tl.hbase + (size_t)b * tl.hstride];
+  const uint32_t mask2 = (1u << a.l2) - 1u;
   const JobDesc j = a.jobs[tl.job];
   const uint32_t per_t = (a.D1 + SORT_BLOCK - 1) / SORT_BLOCK;   // bins per thread in the scan
   const uint32_t b0 = tid * per_t, b1 = b0 + per_t < a.D1 ? b0 + per_t : a.D1;
@@ -209,7 +223,7 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __r
         const int w = w0 + wl;
         if (!live || w >= a.W) break;
         const int d = window_digit(s.v, w, a.c, carry);
-        if (d != 0) rank[tid * SCAT_WG + wl] = atomicAdd(&cnt[bucket_key(a, j, w, d) >> a.l2], 1u);
+        if (d != 0) rank[tid * SCAT_WG + wl] = atomicAdd(&cnt[(bucket_key(a, j, w, d) - tl.kbase) >> a.l2], 1u);
       }
       __syncthreads();
       // exclusive scan of the counts: per-thread ranges, then a block scan of the partials
@@ -229,7 +243,7 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __r
         if (!live || w >= a.W) break;
         const int d = window_digit(s.v, w, a.c, carry);
         if (d != 0) {
-          const uint32_t key = bucket_key(a, j, w, d);
+          const uint32_t key = bucket_key(a, j, w, d) - tl.kbase;   // within the sort group
           uint32_t ref = j.ref_offset + i;
           if (!a.per_window) ref += (uint32_t)w * a.ref_stride;
           const uint32_t p = offs[key >> a.l2] + rank[tid * SCAT_WG + wl];
@@ -259,17 +273,21 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __r
   }
 }
 
-// 4. per top-bit bin: counting sort on the low bits; writes bucket offsets off[]
-//    (off[NB] = M) and the references in bucket order.
+// 4. per top-bit bin of a sort group (workgroup g D1 + bin): counting sort on the low
+//    bits; writes bucket offsets off[] (off[NB] = M) and the references in bucket order.
 __global__ void __launch_bounds__(SORT_BLOCK) k_sort_bins(const uint32_t* __restrict__ hoff, SortArgs a, uint32_t NB,
                                                           const uint32_t* __restrict__ tmp_key,
                                                           const uint32_t* __restrict__ tmp_ref,
                                                           uint32_t* __restrict__ refs, uint32_t* __restrict__ off) {
   extern __shared__ uint32_t c2[];   // 2^l2 counters, then scan scratch
-  const uint32_t bin = blockIdx.x;
+  const uint32_t g = blockIdx.x / a.D1, bin = blockIdx.x % a.D1;
+  const GroupDesc gr = a.groups[g];
   const uint32_t R = 1u << a.l2;
-  const uint32_t lo = hoff[(size_t)bin * a.G];
-  const uint32_t hi = hoff[(size_t)(bin + 1) * a.G];   // hoff has D1 * G + 1 entries
+  // the group's block of hoff starts at D1 tile0 (bin-major); its last bin ends where the
+  // next group's block starts (hoff has D1 G + 1 entries)
+  const size_t h0 = (size_t)a.D1 * gr.tile0;
+  const uint32_t lo = hoff[h0 + (size_t)bin * gr.G];
+  const uint32_t hi = hoff[h0 + (size_t)(bin + 1) * gr.G];
   for (uint32_t d = threadIdx.x; d < R; d += SORT_BLOCK) c2[d] = 0;
   __syncthreads();
   const uint32_t sh = a.pack;   // 0: keys alone in tmp_key (a shift by 0 keeps them)
@@ -288,12 +306,13 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_bins(const uint32_t* __rest
       c2[d] = run;
       run += v;
     }
-    if (bin == 0 && threadIdx.x == 0) off[NB] = hoff[(size_t)a.D1 * a.G];
+    if (blockIdx.x == 0 && threadIdx.x == 0) off[NB] = hoff[(size_t)a.D1 * a.G];
   }
   __syncthreads();
   for (uint32_t d = threadIdx.x; d < R; d += SORT_BLOCK) {
-    const uint32_t b = bin * R + d;
-    if (b < NB) off[b] = c2[d];
+    const uint32_t lb = bin * R + d;   // bucket within the group
+    const size_t b = (size_t)g * a.NBg + lb;
+    if (lb < a.NBg && b < NB) off[b] = c2[d];
   }
   __syncthreads();
   if (sh) {
@@ -1055,11 +1074,18 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   // tiles of at most ~2048 scalars, never crossing a job
   std::vector<JobDesc> jd(nj);
   std::vector<TileDesc> td;
+  // sort groups: consecutive jobs whose bucket sets hold at most SORT_GROUP_BUCKETS
+  // buckets together, each group sorted on its own key range (one histogram block, one
+  // set of bins per group) so that bins stay short with few second-level counters
+  const size_t job_buckets = (size_t)spj * p.B;
+  const size_t GJ = NB <= SORT_GROUP_BUCKETS ? nj : std::max<size_t>(1, SORT_GROUP_BUCKETS / job_buckets);
+  const size_t ngroups = (nj + GJ - 1) / GJ;
+  const uint32_t NBg = (uint32_t)std::min<size_t>(NB, GJ * job_buckets);
   // top-level bins: 512 when the second level's 2^l2 counters allow it (longer runs per
   // bin in the scatter), more for large key spaces (the second level keeps <= 2^11)
-  const int nb_bits = ceil_log2(NB);
+  const int nb_bits = ceil_log2(NBg);
   const uint32_t l2_top = (uint32_t)std::min(11, std::max(0, nb_bits - 9));
-  const size_t D1_top = ((size_t)NB + (1u << l2_top) - 1) >> l2_top;
+  const size_t D1_top = ((size_t)NBg + (1u << l2_top) - 1) >> l2_top;
   // ~1024-1536 tiles plus at most one partial tile per job, fewer when the bins are many:
   // the (bin, tile) histogram stays within the two-level scan (2048 x 2048 entries)
   const size_t max_tiles = (size_t)SCAN_TILE * SCAN_TILE / std::max<size_t>(D1_top, 1);
@@ -1068,16 +1094,22 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   const size_t tile = std::max<size_t>(256, (ntot + tile_target - 1) / tile_target);
   MZK_REQUIRE(nj < 512, "msm: at most 511 jobs per batch (%zu given)", nj);
   size_t canon_off = 0;
+  std::vector<GroupDesc> gd(ngroups);
   for (size_t j = 0; j < nj; j++) {
     jd[j].coeffs = (const uint32_t*)jobs[j].d_coeffs;
     jd[j].n = (uint32_t)jobs[j].n;
     jd[j].ref_offset = jobs[j].ref_offset;
     jd[j].canon_off = (uint32_t)canon_off;
     jd[j].set_base = (uint32_t)(j * spj);
+    const size_t g = j / GJ;
+    if (j % GJ == 0) gd[g].tile0 = (uint32_t)td.size();
     for (size_t lo = 0; lo < jobs[j].n; lo += tile)
-      td.push_back(TileDesc{(uint32_t)j, (uint32_t)lo, (uint32_t)std::min(lo + tile, jobs[j].n)});
+      td.push_back(TileDesc{(uint32_t)j, (uint32_t)lo, (uint32_t)std::min(lo + tile, jobs[j].n), 0, 0,
+                            (uint32_t)(g * NBg)});
     canon_off += jobs[j].n;
   }
+  for (size_t g = 0; g < ngroups; g++)
+    gd[g].G = (uint32_t)((g + 1 < ngroups ? gd[g + 1].tile0 : td.size()) - gd[g].tile0);
   SortArgs sa;
   memset(&sa, 0, sizeof sa);   // padding too: the struct is part of the graph key
   sa.G = (uint32_t)td.size();
@@ -1098,7 +1130,7 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
     // k_sort_bins 0.39 -> 0.13 ms)
     if (sa.l2 + ref_bits + 1 > 32 && ref_bits + 1 <= 32) {
       const uint32_t l2 = 31 - ref_bits;
-      const size_t D1 = ((size_t)NB + (1u << l2) - 1) >> l2;
+      const size_t D1 = ((size_t)NBg + (1u << l2) - 1) >> l2;
       const size_t hn_tiles = (D1 * sa.G + SCAN_TILE - 1) / SCAN_TILE;
       if (D1 <= SCAT_MAX_D1 && hn_tiles <= (size_t)SCAN_TILE) {
         sa.l2 = l2;
@@ -1107,6 +1139,12 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
     }
     sa.pack = sa.l2 + ref_bits + 1 <= 32 ? ref_bits + 1 : 0;
   }
+  sa.NBg = NBg;
+  for (size_t g = 0; g < ngroups; g++)   // each tile's histogram column within its group's block
+    for (uint32_t t = gd[g].tile0; t < gd[g].tile0 + gd[g].G; t++) {
+      td[t].hbase = sa.D1 * gd[g].tile0 + (t - gd[g].tile0);
+      td[t].hstride = gd[g].G;
+    }
   const size_t HN = (size_t)sa.D1 * sa.G;   // histogram entries
   const size_t tiles = (HN + SCAN_TILE - 1) / SCAN_TILE;
   MZK_REQUIRE(tiles <= (size_t)SCAN_TILE, "msm: sort histogram too large (%zu)", HN);
@@ -1140,7 +1178,8 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
                        T <= 256 * ST_LEAVES;
 
   // scratch A: descriptors | canonical scalars | hist | hoff (HN+1) | tile sums | off (NB+1)
-  const size_t desc_words = (nj * sizeof(JobDesc) + td.size() * sizeof(TileDesc) + 15) / 4 + 8;
+  const size_t desc_words =
+      (nj * sizeof(JobDesc) + td.size() * sizeof(TileDesc) + ngroups * sizeof(GroupDesc) + 31) / 4 + 8;
   size_t a_words = desc_words + 8 * ntot + HN + (HN + 1) + tiles + ((size_t)NB + 1) + 64;
   MZK_TRY(c->scratch[SCR_MSM_A].reserve(a_words * 4));
   uint32_t* base = (uint32_t*)c->scratch[SCR_MSM_A].ptr;
@@ -1154,17 +1193,21 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   // descriptors through pinned memory: [jobs | pad | tiles] as laid out in scratch A, one
   // asynchronous copy; the channel sums come back through the same buffer after it
   const size_t tiles_at = (size_t)((char*)d_tiles - (char*)d_jobs);
-  const size_t desc_bytes = tiles_at + td.size() * sizeof(TileDesc);
+  const size_t groups_at = (tiles_at + td.size() * sizeof(TileDesc) + 15) & ~(size_t)15;
+  GroupDesc* d_groups = (GroupDesc*)((char*)d_jobs + groups_at);
+  const size_t desc_bytes = groups_at + ngroups * sizeof(GroupDesc);
   const size_t res_at = (desc_bytes + 255) & ~(size_t)255;
   const size_t res_words = std::max(nchan, sets) * 48 + 4;
   MZK_TRY(c->pinned.reserve(res_at + res_words * 4));
   char* pin = (char*)c->pinned.ptr;
   memcpy(pin, jd.data(), nj * sizeof(JobDesc));
   memcpy(pin + tiles_at, td.data(), td.size() * sizeof(TileDesc));
+  memcpy(pin + groups_at, gd.data(), ngroups * sizeof(GroupDesc));
   MZK_HIP(hipMemcpyAsync(d_jobs, pin, desc_bytes, hipMemcpyHostToDevice, c->stream));
   uint32_t* res = (uint32_t*)(pin + res_at);
   sa.jobs = d_jobs;
   sa.tiles = d_tiles;
+  sa.groups = d_groups;
   // scratch B: tmp keys | tmp refs | sorted refs
   MZK_TRY(c->scratch[SCR_MSM_B].reserve(3 * M_max * 4 + 64));
   uint32_t* tmp_key = (uint32_t*)c->scratch[SCR_MSM_B].ptr;
@@ -1197,7 +1240,7 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   gk.add('M').add(sa).add(canon).add(hist).add(hoff).add(HN).add(tsums).add(tiles).add(tmp_key).add(tmp_ref);
   gk.add(refs).add(off).add(NB).add(K).add(nchunks).add(records).add(d_bases).add(partials).add(small_sets);
   gk.add(p.B).add(sets).add(chan).add(segA).add(segR).add(L).add(nseg).add(T).add(nch).add(parts).add(chan_parts);
-  gk.add(nchan).add(tree).add(segtree);
+  gk.add(nchan).add(tree).add(segtree).add(ngroups);
   MZK_TRY(run_graph(c, gk.s, [&]() -> int {
     MZK_LAUNCH(c, "k_sort_hist", k_sort_hist, dim3(sa.G), dim3(SORT_BLOCK), sa.D1 * 4, sa, canon, hist);
     MZK_LAUNCH(c, "k_scan_tiles", k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, hist, hoff, HN, tsums);
@@ -1205,7 +1248,8 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
     MZK_LAUNCH(c, "k_scan_add", k_scan_add, dim3(grid_for(HN, 256)), dim3(256), 0, hoff, HN, tsums);
     MZK_LAUNCH(c, "k_sort_scatter", k_sort_scatter, dim3(sa.G), dim3(SORT_BLOCK), scat_lds, canon, sa, hoff, tmp_key,
                tmp_ref);
-    MZK_LAUNCH(c, "k_sort_bins", k_sort_bins, dim3(sa.D1), dim3(SORT_BLOCK), (1u << sa.l2) * 4, hoff, sa, NB,
+    MZK_LAUNCH(c, "k_sort_bins", k_sort_bins, dim3((unsigned)(ngroups * sa.D1)), dim3(SORT_BLOCK), (1u << sa.l2) * 4,
+               hoff, sa, NB,
                tmp_key, tmp_ref, refs, off);
     if (records)
       MZK_LAUNCH(c, "k_msm_accum", k_msm_accum<true>, dim3(grid_for(nchunks, 64)), dim3(64), 0, refs, off, NB, K,
