@@ -388,7 +388,12 @@ int mzk_fr_prefix_sum(mzk_ctx *ctx, const void *d_a, void *d_out, size_t n);
  * values, or 2^(k+2) for the coset forms), as the Python keygen builds them.  advice
  * holds nproofs x 5 device pointers to the Lagrange columns (rows n - 7.. are replaced
  * by blinding values drawn from the seeded stream of rng_seeds[p]).  Writes nproofs x
- * MZK_PLONK_PROOF_BYTES.  phase_ms, if not null, receives 8 cumulative times (ms). */
+ * MZK_PLONK_PROOF_BYTES.  The columns' transforms to coefficients and to the extended
+ * coset run on a second stream of the context (created on first use, released with the
+ * context), overlapping the advice and permutation commitments.  phase_ms, if not null,
+ * receives 8 cumulative times (ms) at the ends of: advice commitments, permutation
+ * commitments, the wait for the columns' transforms, (the same point again), the quotient
+ * numerator, the quotient commitments, the evaluations, the multi-open. */
 #define MZK_PLONK_PROOF_BYTES 1360
 typedef struct {
   uint32_t k;

@@ -254,6 +254,7 @@ int mzk_ctx_create(int device, mzk_ctx** out) {
 
 void mzk_ctx_destroy(mzk_ctx* c) {
   if (!c) return;
+  if (c->aux) mzk_ctx_destroy(c->aux);
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   free_ntt_plans(c);

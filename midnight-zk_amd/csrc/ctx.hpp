@@ -143,4 +143,7 @@ struct mzk_ctx {
   uint64_t last_msm[6] = {0, 0, 0, 0, 0, 0};   // c, W, sets, K, M, n of the last MSM
   std::map<std::string, mzk::NttPlan*> ntt_plans;   // (log_n, split, root) -> resident twiddle tables
   mzk::GraphCache graphs;
+  // a second context (own stream and scratch) on the same device for work the prover can
+  // overlap with this context's stream; created on first use under prove_mu
+  mzk_ctx* aux = nullptr;
 };

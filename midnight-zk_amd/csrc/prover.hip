@@ -274,6 +274,55 @@ void intermediate_sets(const std::vector<std::pair<int, Fr>>& q, std::vector<int
   }
 }
 
+// Cross-stream ordering for one prover call: ev[0] is recorded on the main stream when
+// the advice columns are in place and ev[1] on the auxiliary stream after their
+// transforms; ev[2] / ev[3] the same for the permutation products z.  On every exit the
+// auxiliary stream is drained (it writes into the call's arena) and the events released.
+struct AuxStream {
+  mzk_ctx* aux = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool used = false;
+  ~AuxStream() {
+    if (used) (void)hipStreamSynchronize(aux->stream);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+// The transforms of columns cols[] of every proof on the auxiliary stream, after the main
+// stream reaches this point: to coefficients (src -> polys), then to the extended coset
+// (polys -> cos); `done` is recorded after them.
+static int aux_transforms(mzk_ctx* c, AuxStream& ax, hipEvent_t ready, hipEvent_t done,
+                          const std::vector<std::pair<const void*, std::pair<uint32_t*, uint32_t*>>>& cols,
+                          uint32_t k) {
+  {
+    std::lock_guard<std::mutex> g(c->mu);
+    MZK_HIP(hipSetDevice(c->device));
+    MZK_HIP(hipEventRecord(ready, c->stream));
+    MZK_HIP(hipStreamWaitEvent(ax.aux->stream, ready, 0));
+  }
+  ax.used = true;
+  std::vector<mzk_ntt_job> jobs(cols.size());
+  for (size_t i = 0; i < cols.size(); i++) {
+    jobs[i].d_in = cols[i].first;
+    jobs[i].d_out = cols[i].second.first;
+    jobs[i].n_real = (uint64_t)1 << k;
+    jobs[i].log_n = k;
+    jobs[i].flags = MZK_NTT_INVERSE;
+  }
+  MZK_TRY(mzk_ntt_batch_dev(ax.aux, jobs.data(), jobs.size()));
+  for (size_t i = 0; i < cols.size(); i++) {
+    jobs[i].d_in = cols[i].second.first;
+    jobs[i].d_out = cols[i].second.second;
+    jobs[i].log_n = k + 2;
+    jobs[i].flags = MZK_NTT_COSET;
+  }
+  MZK_TRY(mzk_ntt_batch_dev(ax.aux, jobs.data(), jobs.size()));
+  std::lock_guard<std::mutex> g(ax.aux->mu);
+  MZK_HIP(hipEventRecord(done, ax.aux->stream));
+  return MZK_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -359,6 +408,21 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
       MZK_HIP(hipMemcpyAsync(B[p].adv[j] + 8 * u, B[p].draws + 8 * (size_t)(j * (BF + 1)), (BF + 1) * 32,
                              hipMemcpyDeviceToDevice, st));
     }
+  }
+  // the advice columns' transforms (to coefficients, to the extended coset) depend on the
+  // advice alone: they run on the auxiliary stream while this stream commits the advice
+  // and builds the permutation products (they fill the commitments' sort and reduction
+  // phases, where most SIMDs wait); the numerator waits for them
+  AuxStream ax;
+  {
+    if (!c->aux) PV_TRY(mzk_ctx_create(c->device, &c->aux));
+    ax.aux = c->aux;
+    MZK_HIP(hipSetDevice(c->device));
+    for (hipEvent_t& e : ax.ev) MZK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    std::vector<std::pair<const void*, std::pair<uint32_t*, uint32_t*>>> cols;
+    for (size_t p = 0; p < np; p++)
+      for (int j = 0; j < NADV; j++) cols.push_back({B[p].adv[j], {B[p].polys[j], B[p].cos[j]}});
+    PV_TRY(aux_transforms(c, ax, ax.ev[0], ax.ev[1], cols, k));
   }
   std::vector<uint64_t> jac;
   {
@@ -446,6 +510,13 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
       }
     }
   }
+  {   // z's transforms likewise overlap its commitment (few non-zero deltas: mostly the
+      // MSM's sort and reduction phases)
+    std::vector<std::pair<const void*, std::pair<uint32_t*, uint32_t*>>> cols;
+    for (size_t p = 0; p < np; p++)
+      for (int s = 0; s < NSETS; s++) cols.push_back({B[p].z[s], {B[p].polys[NADV + s], B[p].cos[NADV + s]}});
+    PV_TRY(aux_transforms(c, ax, ax.ev[2], ax.ev[3], cols, k));
+  }
   {
     // z has long constant runs (rows outside every copy cycle): committed as z.to_delta()
     // against the suffix-sum bases (permutation/prover.rs:236-245), the zero deltas give the
@@ -477,35 +548,15 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
     y[p] = T[p].squeeze();
   }
 
-  // ---- to coefficients, to the extended coset, numerator / (X^n - 1), back (prover.rs:634-772)
+  // ---- to coefficients, to the extended coset, numerator / (X^n - 1), back (prover.rs:634-772):
+  // the columns' transforms ran on the auxiliary stream; the numerator waits for them
   {
-    std::vector<mzk_ntt_job> jobs;
-    for (size_t p = 0; p < np; p++)
-      for (int j = 0; j < NADV + NSETS; j++) {
-        mzk_ntt_job t;
-        t.d_in = j < NADV ? (const void*)B[p].adv[j] : (const void*)B[p].z[j - NADV];
-        t.d_out = B[p].polys[j];
-        t.n_real = n;
-        t.log_n = k;
-        t.flags = MZK_NTT_INVERSE;
-        jobs.push_back(t);
-      }
-    PV_TRY(mzk_ntt_batch_dev(c, jobs.data(), jobs.size()));
-    mark(2);
-    jobs.clear();
-    for (size_t p = 0; p < np; p++)
-      for (int j = 0; j < NADV + NSETS; j++) {
-        mzk_ntt_job t;
-        t.d_in = B[p].polys[j];
-        t.d_out = B[p].cos[j];
-        t.n_real = n;
-        t.log_n = k + 2;
-        t.flags = MZK_NTT_COSET;
-        jobs.push_back(t);
-      }
-    PV_TRY(mzk_ntt_batch_dev(c, jobs.data(), jobs.size()));
-    mark(3);
+    std::lock_guard<std::mutex> g(c->mu);
+    MZK_HIP(hipStreamWaitEvent(st, ax.ev[1], 0));
+    MZK_HIP(hipStreamWaitEvent(st, ax.ev[3], 0));
   }
+  mark(2);
+  mark(3);
   std::vector<Fr> te(4);
   {
     const Fr wext = host::fr_root_of_unity(k + 2), zeta = host::fr_const(host::kFrZeta);
