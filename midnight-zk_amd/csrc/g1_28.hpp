@@ -45,6 +45,21 @@ MZK_HD g1x28 g1x28_from_affine(const f28& x, const f28& y, bool neg) {
   return r;
 }
 
+// The same for an affine input already below 2p and normalised (the resident SRS's
+// accumulation records): no product, so a lane that opens a bucket costs its wave a
+// few selects instead of two products.
+MZK_HD g1x28 g1x28_from_reduced(const f28& x, const f28& y, bool neg) {
+  g1x28 r;
+  const f28 one = f28_one();
+  r.X = x;
+  f28 yy = y;
+  if (neg) yy = f28_norm(f28_sub<4, 1>(f28_zero(), y));   // 4p - y < 4p
+  r.Y = yy;
+  r.ZZ = one;
+  r.ZZZ = one;
+  return r;
+}
+
 // 2 (x, +-y) for an affine point (mdbl-2008-s-1); coordinates as from_affine makes
 // them (x < 2p, y < 4p, normalised).
 MZK_HD g1x28 g1x28_dbl_affine(const f28& x, const f28& y) {
@@ -66,10 +81,14 @@ MZK_HD g1x28 g1x28_dbl_affine(const f28& x, const f28& y) {
   return r;
 }
 
-// acc + (x, +-y) (madd-2008-s).  x, y from from_std (< 2^389); the affine identity
-// is the caller's to skip.
+// acc + (x, +-y) (madd-2008-s).  x, y from from_std (< 2^389), or below 2p and
+// normalised when REDUCED; the affine identity is the caller's to skip.
+template <bool REDUCED = false>
 MZK_HD g1x28 g1x28_madd(const g1x28& p, const f28& x, const f28& y, bool neg) {
-  if (g1x28_is_inf(p)) return g1x28_from_affine(x, y, neg);
+  if (g1x28_is_inf(p)) {
+    if constexpr (REDUCED) return g1x28_from_reduced(x, y, neg);
+    else return g1x28_from_affine(x, y, neg);
+  }
   const f28 U2 = f28_mul(x, p.ZZ);                 // 512 p^2 -> < 2p
   const f28 S2 = f28_mul(y, p.ZZZ);                // < 2p
   // P = U2 - X < 18p (X < 10p; 16p's top limb minus 1 covers X's)
@@ -78,8 +97,13 @@ MZK_HD g1x28 g1x28_madd(const g1x28& p, const f28& x, const f28& y, bool neg) {
   const f28 R = neg ? f28_neg_sum<16, 2>(S2, p.Y) : f28_sub<8, 1>(S2, p.Y);
   if (f28_is_zero_mod_p<18>(P)) {
     if (f28_is_zero_mod_p<16>(R)) {
-      const g1x28 q = g1x28_from_affine(x, y, neg);
-      return g1x28_dbl_affine(q.X, q.Y);
+      if constexpr (REDUCED) {
+        const g1x28 q = g1x28_from_reduced(x, y, neg);
+        return g1x28_dbl_affine(q.X, q.Y);
+      } else {
+        const g1x28 q = g1x28_from_affine(x, y, neg);
+        return g1x28_dbl_affine(q.X, q.Y);
+      }
     }
     return g1x28_inf();
   }

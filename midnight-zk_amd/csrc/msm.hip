@@ -273,6 +273,8 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __r
   }
 }
 
+static constexpr int BINS_UNROLL = 8;
+
 // 4. per top-bit bin of a sort group (workgroup g D1 + bin): counting sort on the low
 //    bits; writes bucket offsets off[] (off[NB] = M) and the references in bucket order.
 __global__ void __launch_bounds__(SORT_BLOCK) k_sort_bins(const uint32_t* __restrict__ hoff, SortArgs a, uint32_t NB,
@@ -291,7 +293,19 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_bins(const uint32_t* __rest
   for (uint32_t d = threadIdx.x; d < R; d += SORT_BLOCK) c2[d] = 0;
   __syncthreads();
   const uint32_t sh = a.pack;   // 0: keys alone in tmp_key (a shift by 0 keeps them)
-  for (uint32_t e = lo + threadIdx.x; e < hi; e += SORT_BLOCK) atomicAdd(&c2[tmp_key[e] >> sh], 1u);
+  // BINS_UNROLL loads in flight per thread before their LDS atomics (one at a time, each
+  // atomic waited for its load: the kernel was bound by load latency)
+  for (uint32_t e = lo + threadIdx.x; e < hi; e += BINS_UNROLL * SORT_BLOCK) {
+    uint32_t v[BINS_UNROLL];
+#pragma unroll
+    for (int u = 0; u < BINS_UNROLL; u++) {
+      const uint32_t x = e + (uint32_t)u * SORT_BLOCK;
+      v[u] = x < hi ? tmp_key[x] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < BINS_UNROLL; u++)
+      if (e + (uint32_t)u * SORT_BLOCK < hi) atomicAdd(&c2[v[u] >> sh], 1u);
+  }
   __syncthreads();
   {   // exclusive scan of the R counters from lo: per-thread ranges, then a block scan
     __shared__ uint32_t wtot[SORT_BLOCK / 64];
@@ -317,15 +331,37 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_bins(const uint32_t* __rest
   __syncthreads();
   if (sh) {
     const uint32_t idx_mask = (1u << (sh - 1)) - 1u;
-    for (uint32_t e = lo + threadIdx.x; e < hi; e += SORT_BLOCK) {
-      const uint32_t v = tmp_key[e];
-      const uint32_t pos = atomicAdd(&c2[v >> sh], 1u);
-      refs[pos] = (v & idx_mask) | (((v >> (sh - 1)) & 1u) ? NEG_FLAG : 0u);
+    for (uint32_t e = lo + threadIdx.x; e < hi; e += BINS_UNROLL * SORT_BLOCK) {
+      uint32_t v[BINS_UNROLL];
+#pragma unroll
+      for (int u = 0; u < BINS_UNROLL; u++) {
+        const uint32_t x = e + (uint32_t)u * SORT_BLOCK;
+        v[u] = x < hi ? tmp_key[x] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < BINS_UNROLL; u++) {
+        if (e + (uint32_t)u * SORT_BLOCK < hi) {
+          const uint32_t pos = atomicAdd(&c2[v[u] >> sh], 1u);
+          refs[pos] = (v[u] & idx_mask) | (((v[u] >> (sh - 1)) & 1u) ? NEG_FLAG : 0u);
+        }
+      }
     }
   } else {
-    for (uint32_t e = lo + threadIdx.x; e < hi; e += SORT_BLOCK) {
-      const uint32_t pos = atomicAdd(&c2[tmp_key[e]], 1u);
-      refs[pos] = tmp_ref[e];
+    for (uint32_t e = lo + threadIdx.x; e < hi; e += BINS_UNROLL * SORT_BLOCK) {
+      uint32_t v[BINS_UNROLL], r[BINS_UNROLL];
+#pragma unroll
+      for (int u = 0; u < BINS_UNROLL; u++) {
+        const uint32_t x = e + (uint32_t)u * SORT_BLOCK;
+        v[u] = x < hi ? tmp_key[x] : 0u;
+        r[u] = x < hi ? tmp_ref[x] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < BINS_UNROLL; u++) {
+        if (e + (uint32_t)u * SORT_BLOCK < hi) {
+          const uint32_t pos = atomicAdd(&c2[v[u]], 1u);
+          refs[pos] = r[u];
+        }
+      }
     }
   }
 }
@@ -411,7 +447,8 @@ __global__ void k_scan_add(uint32_t* __restrict__ out, size_t n, const uint32_t*
 //
 // The point of reference r: blst affine (24 words, converted here) for caller bases, or
 // a 128-byte record of a resident SRS's shifted tables (already in the radix-2^28 form,
-// identity flagged): one cache line per gathered point instead of 1.5 on average.
+// below 2p, identity flagged): one cache line per gathered point instead of 1.5 on
+// average, and a lane that opens a bucket takes the record's coordinates as they are.
 // The loads are issued one reference ahead (raw words held in registers) so that each
 // gather's latency overlaps the previous mixed addition.
 template <bool REC>
@@ -499,7 +536,7 @@ __global__ void __launch_bounds__(64) k_msm_accum(const uint32_t* __restrict__ r
         next = off[cb + 1];
       } while (next <= e);
     }
-    if (live) acc = g1x28_madd(acc, x, y, (r & NEG_FLAG) != 0);
+    if (live) acc = g1x28_madd<REC>(acc, x, y, (r & NEG_FLAG) != 0);
   }
   store_x28_raw(partials + MZK_X28_WORDS * (size_t)(k + cb), acc);
 }

@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(64) k_shift_window_xyzz(const uint32_t* __rest
 }
 
 // Affine points (blst layout) -> 128-byte records for the bucket accumulation: x and y
-// in the radix-2^28 form (words 0..13, 14..27; the values from_std gives), word 28 = 1
+// in the radix-2^28 form, below 2p and normalised (words 0..13, 14..27), word 28 = 1
 // for the identity, words 29..31 zero.  One record is one 128-byte line.
 __global__ void __launch_bounds__(64) k_to_records(const uint32_t* __restrict__ aff, size_t m,
                                                    uint32_t* __restrict__ rec) {
@@ -235,7 +235,10 @@ __global__ void __launch_bounds__(64) k_to_records(const uint32_t* __restrict__ 
 #pragma unroll
   for (int k = 0; k < MZK_SRS_REC_WORDS; k++) w[k] = 0;
   if (any) {
-    const f28 x = f28_from_std(a), y = f28_from_std(a + 12);
+    // below 2p and normalised (a product by one each): the accumulation opens a bucket
+    // with these coordinates as they are
+    const f28 one = f28_one();
+    const f28 x = f28_mul(f28_from_std(a), one), y = f28_mul(f28_from_std(a + 12), one);
 #pragma unroll
     for (int k = 0; k < 14; k++) {
       w[k] = x.l[k];
