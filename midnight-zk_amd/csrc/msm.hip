@@ -183,12 +183,21 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_hist(SortArgs a, uint32_t* 
 //    scattered 4-byte store per digit would make this kernel bound by partial-line
 //    HBM writes).
 static constexpr int SCAT_WG = 16;   // 16 windows per round: 2^20 scatter 0.196 -> 0.187 ms against 8
-static constexpr uint32_t SCAT_MAX_D1 = 8192;   // LDS: (3 D1 + 3 SCAT_SLOTS + SORT_BLOCK) words <= 148.5 KiB
+static constexpr uint32_t SCAT_MAX_D1 = 8192;   // LDS: at most (3 D1 + 3 SCAT_SLOTS + SORT_BLOCK) words = 145 KiB
 static constexpr int SCAT_SLOTS = SORT_BLOCK * SCAT_WG;
 // buckets per sort group (batches of many bucket sets sort a group of jobs at a time:
 // 96 shared sets of 2^15 in one group made k_sort_bins 10 ms of a 55 ms batch)
 static constexpr size_t SORT_GROUP_BUCKETS = 1u << 18;
 
+// LDS words of the scatter: cur, cnt, offs (D1 each), the partial sums, and the staging.
+// Unpacked: rank, key and reference per slot (3 SCAT_SLOTS).  Packed (key low bits,
+// reference and sign in one word): a 16-bit rank and bin per slot plus the packed word
+// (2 SCAT_SLOTS), 39 KiB at 512 bins, so four workgroups share a CU instead of two.
+__host__ __device__ constexpr size_t scatter_lds_words(uint32_t D1, bool packed) {
+  return 3 * (size_t)D1 + (packed ? 2 : 3) * (size_t)SCAT_SLOTS + SORT_BLOCK;
+}
+
+template <bool PACKED>
 __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __restrict__ canon, SortArgs a,
                                                              const uint32_t* __restrict__ hoff,
                                                              uint32_t* __restrict__ tmp_key,
@@ -197,10 +206,16 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __r
   uint32_t* cur = sm;                   // D1: next output position of each bin
   uint32_t* cnt = cur + a.D1;           // D1: entries of the round per bin
   uint32_t* offs = cnt + a.D1;          // D1: LDS offset of each bin's entries in the round
-  uint32_t* rank = offs + a.D1;         // SCAT_SLOTS: rank of each (thread, window) slot in its bin
-  uint32_t* skey = rank + SCAT_SLOTS;   // staged keys, bin order
-  uint32_t* sref = skey + SCAT_SLOTS;   // staged references, bin order
-  uint32_t* psum = sref + SCAT_SLOTS;   // SORT_BLOCK partial sums of the offset scan
+  uint32_t* psum = offs + a.D1;         // SORT_BLOCK partial sums of the offset scan
+  uint32_t* stage = psum + SORT_BLOCK;
+  // unpacked: rank, key, reference (SCAT_SLOTS words each); packed: the packed word, then
+  // the 16-bit ranks and bins (ranks < SCAT_SLOTS, bins < SCAT_MAX_D1)
+  uint32_t* rank = stage;                                   // unpacked
+  uint32_t* skey = stage + SCAT_SLOTS;                      // unpacked: staged keys, bin order
+  uint32_t* sref = stage + 2 * SCAT_SLOTS;                  // unpacked: staged references
+  uint32_t* spk = stage;                                    // packed: staged words, bin order
+  uint16_t* rank16 = reinterpret_cast<uint16_t*>(stage + SCAT_SLOTS);
+  uint16_t* sbin = rank16 + SCAT_SLOTS;                     // packed: bin of each staged word
   const uint32_t tid = threadIdx.x;
   const TileDesc tl = a.tiles[blockIdx.x];
   for (uint32_t b = tid; b < a.D1; b += SORT_BLOCK) cur[b] = hoff[tl.hbase + (size_t)b * tl.hstride];
@@ -223,7 +238,11 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __r
         const int w = w0 + wl;
         if (!live || w >= a.W) break;
         const int d = window_digit(s.v, w, a.c, carry);
-        if (d != 0) rank[tid * SCAT_WG + wl] = atomicAdd(&cnt[(bucket_key(a, j, w, d) - tl.kbase) >> a.l2], 1u);
+        if (d != 0) {
+          const uint32_t r = atomicAdd(&cnt[(bucket_key(a, j, w, d) - tl.kbase) >> a.l2], 1u);
+          if constexpr (PACKED) rank16[tid * SCAT_WG + wl] = (uint16_t)r;
+          else rank[tid * SCAT_WG + wl] = r;
+        }
       }
       __syncthreads();
       // exclusive scan of the counts: per-thread ranges, then a block scan of the partials
@@ -246,9 +265,16 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __r
           const uint32_t key = bucket_key(a, j, w, d) - tl.kbase;   // within the sort group
           uint32_t ref = j.ref_offset + i;
           if (!a.per_window) ref += (uint32_t)w * a.ref_stride;
-          const uint32_t p = offs[key >> a.l2] + rank[tid * SCAT_WG + wl];
-          skey[p] = key;
-          sref[p] = a.pack ? (ref | (d < 0 ? 1u << (a.pack - 1) : 0u)) : (ref | (d < 0 ? NEG_FLAG : 0u));
+          const uint32_t bin = key >> a.l2;
+          if constexpr (PACKED) {
+            const uint32_t p = offs[bin] + rank16[tid * SCAT_WG + wl];
+            spk[p] = ((key & mask2) << a.pack) | ref | (d < 0 ? 1u << (a.pack - 1) : 0u);
+            sbin[p] = (uint16_t)bin;
+          } else {
+            const uint32_t p = offs[bin] + rank[tid * SCAT_WG + wl];
+            skey[p] = key;
+            sref[p] = ref | (d < 0 ? NEG_FLAG : 0u);
+          }
         }
       }
       carry0 = carry;
@@ -256,12 +282,13 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __r
       // write out in LDS order: consecutive entries of a bin go to consecutive positions
       const uint32_t staged = offs[a.D1 - 1] + cnt[a.D1 - 1];
       for (uint32_t e = tid; e < staged; e += SORT_BLOCK) {
-        const uint32_t key = skey[e];
-        const uint32_t bin = key >> a.l2;
-        const uint32_t pos = cur[bin] + (e - offs[bin]);
-        if (a.pack) {
-          tmp_key[pos] = ((key & mask2) << a.pack) | sref[e];
+        if constexpr (PACKED) {
+          const uint32_t bin = sbin[e];
+          tmp_key[cur[bin] + (e - offs[bin])] = spk[e];
         } else {
+          const uint32_t key = skey[e];
+          const uint32_t bin = key >> a.l2;
+          const uint32_t pos = cur[bin] + (e - offs[bin]);
           tmp_key[pos] = key & mask2;
           tmp_ref[pos] = sref[e];
         }
@@ -979,9 +1006,11 @@ static int scatter_lds_attr() {
   static std::once_flag once;
   static int rc = MZK_OK;
   std::call_once(once, [] {
-    const int bytes = (int)((3 * SCAT_MAX_D1 + 3 * SCAT_SLOTS + SORT_BLOCK) * 4);
-    if (hipFuncSetAttribute((const void*)k_sort_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
-        hipSuccess) {
+    const int bytes = (int)(scatter_lds_words(SCAT_MAX_D1, false) * 4);
+    if (hipFuncSetAttribute((const void*)k_sort_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
+            hipSuccess ||
+        hipFuncSetAttribute((const void*)k_sort_scatter<true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
+            hipSuccess) {
       set_error("msm: cannot raise the scatter's LDS limit to %d bytes", bytes);
       rc = MZK_ERR_HIP;
     }
@@ -1265,11 +1294,11 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   uint32_t* chan_parts = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
   uint32_t* chan = chan_parts + nchan * parts * MZK_X28_WORDS;
 
-  // the scatter stages (3 D1 + 3 SCAT_SLOTS + SORT_BLOCK) words in LDS: above the 64 KiB a
-  // kernel gets without asking once D1 passes ~1300 bins
+  // the scatter's LDS (scatter_lds_words): above the 64 KiB a kernel gets without asking
+  // once D1 passes ~1300 bins (unpacked) or ~2700 (packed)
   MZK_REQUIRE(sa.D1 <= SCAT_MAX_D1, "msm: %u top-level sort bins exceed %u (too many bucket sets)", sa.D1,
               SCAT_MAX_D1);
-  const size_t scat_lds = (3 * (size_t)sa.D1 + 3 * SCAT_SLOTS + SORT_BLOCK) * 4;
+  const size_t scat_lds = scatter_lds_words(sa.D1, sa.pack != 0) * 4;
   MZK_TRY(scatter_lds_attr());
   // the device sequence (sort, accumulation, reduction) as one cached graph: every value
   // its launches take is in the key; the descriptors above are copied before it runs
@@ -1283,8 +1312,12 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
     MZK_LAUNCH(c, "k_scan_tiles", k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, hist, hoff, HN, tsums);
     MZK_LAUNCH(c, "k_scan_sums", k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, tsums, (int)tiles, hoff, HN);
     MZK_LAUNCH(c, "k_scan_add", k_scan_add, dim3(grid_for(HN, 256)), dim3(256), 0, hoff, HN, tsums);
-    MZK_LAUNCH(c, "k_sort_scatter", k_sort_scatter, dim3(sa.G), dim3(SORT_BLOCK), scat_lds, canon, sa, hoff, tmp_key,
-               tmp_ref);
+    if (sa.pack)
+      MZK_LAUNCH(c, "k_sort_scatter", k_sort_scatter<true>, dim3(sa.G), dim3(SORT_BLOCK), scat_lds, canon, sa, hoff,
+                 tmp_key, tmp_ref);
+    else
+      MZK_LAUNCH(c, "k_sort_scatter", k_sort_scatter<false>, dim3(sa.G), dim3(SORT_BLOCK), scat_lds, canon, sa, hoff,
+                 tmp_key, tmp_ref);
     MZK_LAUNCH(c, "k_sort_bins", k_sort_bins, dim3((unsigned)(ngroups * sa.D1)), dim3(SORT_BLOCK), (1u << sa.l2) * 4,
                hoff, sa, NB,
                tmp_key, tmp_ref, refs, off);

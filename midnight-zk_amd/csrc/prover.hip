@@ -89,38 +89,53 @@ struct Transcript {
   }
 };
 
-// G1Affine::to_compressed (ZCash encoding) of a Jacobian result
-static void compress(const uint64_t jac[18], uint8_t out[48]) {
-  uint64_t aff[12];
-  mzk_g1_jacobian_to_affine(jac, aff);
-  bool inf = true;
-  for (int i = 0; i < 12; i++) inf &= aff[i] == 0;
-  memset(out, 0, 48);
-  if (inf) {
-    out[0] = 0xc0;
-    return;
+// G1Affine::to_compressed (ZCash encoding) of `count` Jacobian results (18 words each),
+// with one field inversion for all of them: Montgomery's trick over their non-zero Z
+// (an inversion costs the host ~45 us, as much as a few hundred products, and each
+// phase's commitments are compressed before its challenge can be drawn)
+static void compress_many(const uint64_t* jac, size_t count, uint8_t* out) {
+  std::vector<host::Fp> pre(count);
+  host::Fp run = host::fp_one();
+  for (size_t i = 0; i < count; i++) {
+    host::Fp z;
+    memcpy(z.l, jac + 18 * i + 12, 48);
+    pre[i] = run;
+    if (!host::fp_is_zero(z)) run = host::fp_mul(run, z);
   }
-  host::Fp one = host::fp_zero();
-  one.l[0] = 1;
-  host::Fp x, y;
-  memcpy(x.l, aff, 48);
-  memcpy(y.l, aff + 6, 48);
-  x = host::fp_mul(x, one);
-  y = host::fp_mul(y, one);
-  for (int i = 0; i < 6; i++)
-    for (int b = 0; b < 8; b++) out[47 - (8 * i + b)] = (uint8_t)(x.l[i] >> (8 * b));
-  out[0] |= 0x80;
+  host::Fp inv = host::fp_inv(run);   // 1 / (product of the non-zero Z)
+  host::Fp plain_one = host::fp_zero();
+  plain_one.l[0] = 1;   // a product by 1 leaves Montgomery form
   // (p - 1) / 2
   static const uint64_t HALF[6] = {0xdcff7fffffffd555ull, 0x0f55ffff58a9ffffull, 0xb39869507b587b12ull,
                                    0xb23ba5c279c2895full, 0x258dd3db21a5d66bull, 0x0d0088f51cbff34dull};
-  bool larger = false;
-  for (int i = 5; i >= 0; i--) {
-    if (y.l[i] != HALF[i]) {
-      larger = y.l[i] > HALF[i];
-      break;
+  for (size_t i = count; i-- > 0;) {
+    uint8_t* o = out + 48 * i;
+    memset(o, 0, 48);
+    host::Fp X, Y, Z;
+    memcpy(X.l, jac + 18 * i, 48);
+    memcpy(Y.l, jac + 18 * i + 6, 48);
+    memcpy(Z.l, jac + 18 * i + 12, 48);
+    if (host::fp_is_zero(Z)) {
+      o[0] = 0xc0;   // the identity
+      continue;
     }
+    const host::Fp zi = host::fp_mul(inv, pre[i]);   // 1 / Z
+    inv = host::fp_mul(inv, Z);                      // 1 / (product of the earlier non-zero Z)
+    const host::Fp zi2 = host::fp_mul(zi, zi);
+    const host::Fp x = host::fp_mul(host::fp_mul(X, zi2), plain_one);
+    const host::Fp y = host::fp_mul(host::fp_mul(Y, host::fp_mul(zi2, zi)), plain_one);
+    for (int l = 0; l < 6; l++)
+      for (int b = 0; b < 8; b++) o[47 - (8 * l + b)] = (uint8_t)(x.l[l] >> (8 * b));
+    o[0] |= 0x80;
+    bool larger = false;
+    for (int l = 5; l >= 0; l--) {
+      if (y.l[l] != HALF[l]) {
+        larger = y.l[l] > HALF[l];
+        break;
+      }
+    }
+    if (larger) o[0] |= 0x20;
   }
-  if (larger) out[0] |= 0x20;
 }
 
 static Fr fr_pow_u64(const Fr& a, uint64_t e) { return host::fr_pow(a, e); }
@@ -430,12 +445,10 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
     for (size_t p = 0; p < np; p++)
       for (int j = 0; j < NADV; j++) jobs.push_back({B[p].adv[j], n});
     PV_TRY(msm_batch(c, key->g_lagrange, jobs, jac));
+    std::vector<uint8_t> cb(48 * jobs.size());
+    compress_many(jac.data(), jobs.size(), cb.data());
     for (size_t p = 0; p < np; p++)
-      for (int j = 0; j < NADV; j++) {
-        uint8_t cb[48];
-        compress(&jac[18 * (p * NADV + j)], cb);
-        T[p].write_point(cb);
-      }
+      for (int j = 0; j < NADV; j++) T[p].write_point(&cb[48 * (p * NADV + j)]);
   }
   mark(0);
   std::vector<Fr> beta(np), gamma(np), y(np);
@@ -535,12 +548,10 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
         }
       }
     PV_TRY(msm_batch(c, delta ? key->g_lagrange_delta : key->g_lagrange, jobs, jac));
+    std::vector<uint8_t> cb(48 * jobs.size());
+    compress_many(jac.data(), jobs.size(), cb.data());
     for (size_t p = 0; p < np; p++)
-      for (int s = 0; s < NSETS; s++) {
-        uint8_t cb[48];
-        compress(&jac[18 * (p * NSETS + s)], cb);
-        T[p].write_point(cb);
-      }
+      for (int s = 0; s < NSETS; s++) T[p].write_point(&cb[48 * (p * NSETS + s)]);
   }
   mark(1);
   for (size_t p = 0; p < np; p++) {
@@ -634,12 +645,10 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
     for (size_t p = 0; p < np; p++)
       for (int i = 0; i < QDEG; i++) jobs.push_back({B[p].limbs[i], n});
     PV_TRY(msm_batch(c, key->g, jobs, jac));
+    std::vector<uint8_t> cb(48 * jobs.size());
+    compress_many(jac.data(), jobs.size(), cb.data());
     for (size_t p = 0; p < np; p++)
-      for (int i = 0; i < QDEG; i++) {
-        uint8_t cb[48];
-        compress(&jac[18 * (p * QDEG + i)], cb);
-        T[p].write_point(cb);
-      }
+      for (int i = 0; i < QDEG; i++) T[p].write_point(&cb[48 * (p * QDEG + i)]);
   }
   mark(5);
 
@@ -801,11 +810,9 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
     std::vector<std::pair<const void*, size_t>> jobs;
     for (size_t p = 0; p < np; p++) jobs.push_back({B[p].f, n - 1});
     PV_TRY(msm_batch(c, key->g, jobs, jac));
-    for (size_t p = 0; p < np; p++) {
-      uint8_t cb[48];
-      compress(&jac[18 * p], cb);
-      T[p].write_point(cb);
-    }
+    std::vector<uint8_t> cb(48 * jobs.size());
+    compress_many(jac.data(), jobs.size(), cb.data());
+    for (size_t p = 0; p < np; p++) T[p].write_point(&cb[48 * p]);
   }
   {
     std::vector<std::pair<const void*, size_t>> polys;
@@ -852,11 +859,9 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
     std::vector<std::pair<const void*, size_t>> jobs;
     for (size_t p = 0; p < np; p++) jobs.push_back({B[p].pi, n - 1});
     PV_TRY(msm_batch(c, key->g, jobs, jac));
-    for (size_t p = 0; p < np; p++) {
-      uint8_t cb[48];
-      compress(&jac[18 * p], cb);
-      T[p].write_point(cb);
-    }
+    std::vector<uint8_t> cb(48 * jobs.size());
+    compress_many(jac.data(), jobs.size(), cb.data());
+    for (size_t p = 0; p < np; p++) T[p].write_point(&cb[48 * p]);
   }
   mark(7);
   for (size_t p = 0; p < np; p++) {
