@@ -118,6 +118,37 @@ MZK_HD29 fr29 fr29_sub_kr(const fr29& a, const fr29& b) {
   return o;
 }
 
+// Limb-wise sums with no carry propagation, for values that only feed a product or an
+// add / sub_kr (the NTT's first stage of a register pair):
+//  * add_lazy(a, b) = a + b limb by limb: normalised inputs give limbs <= 2^30 - 2,
+//    which mul takes (limbs < 2^30) and sub_kr<2> takes as its first operand (its limb
+//    sum a_i + 2 r_i - b_i + carry stays <= 2^31 - 1);
+//  * sub3r_lazy(a, b) = a + 3 r - b, 3 r written with its lower limbs raised by 2^29
+//    (and the next limb lowered by 1) so every limb difference is >= 0 for a normalised
+//    b < 2 r; limbs < 2^31, which mul takes when its other operand is normalised and
+//    below 2^29 per limb (a column then sums to < 9 2^60 + 9 2^58 < 2^64).
+namespace r29c {
+MZK_HD29 constexpr uint32_t R3RAISED(int i) {
+  constexpr uint32_t v[N] = {0x20000003, 0x3fffffe7, 0x3ec4ff3e, 0x31d811fe, 0x3880ffb2,
+                             0x240c0e71, 0x3f6266b5, 0x3f2f9b0d, 0x15bc8f4};
+  return v[i];
+}
+}  // namespace r29c
+
+MZK_HD29 fr29 fr29_add_lazy(const fr29& a, const fr29& b) {
+  fr29 o;
+#pragma unroll
+  for (int i = 0; i < r29c::N; i++) o.l[i] = a.l[i] + b.l[i];
+  return o;
+}
+
+MZK_HD29 fr29 fr29_sub3r_lazy(const fr29& a, const fr29& b) {
+  fr29 o;
+#pragma unroll
+  for (int i = 0; i < r29c::N; i++) o.l[i] = a.l[i] + r29c::R3RAISED(i) - b.l[i];
+  return o;
+}
+
 // blst fr (8 x 32-bit, canonical x 2^256) -> x 2^261: the value shifted left by 5 bits,
 // regrouped into 29-bit limbs (below 32 r)
 MZK_HD29 fr29 fr29_from_std(const uint32_t s[8]) {

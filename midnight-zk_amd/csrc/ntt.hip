@@ -149,8 +149,9 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block,
     z1 = arg29(a.zeta);
     z2 = arg29(a.zeta2);
   }
+  const uint32_t lc = 31u - __clz(a.C);   // C and L are powers of two: shifts, not divisions
   for (uint32_t e = threadIdx.x; e < CL; e += nt) {
-    const uint32_t c = e % a.C, j = e / a.C;
+    const uint32_t c = e & (a.C - 1u), j = e >> lc;
     const uint64_t b = b0 + c;
     const uint64_t gi = (b & a.in_mask) * a.in_lo + (b >> a.in_shift) * a.in_hi + (uint64_t)j * a.in_j;
     fr29 x;
@@ -185,7 +186,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block,
     for (; s < a.log_l; s++) {
       const uint32_t half = 1u << s;
       for (uint32_t u = threadIdx.x; u < a.C * halfL; u += nt) {
-        const uint32_t c = u / halfL, v = u % halfL;
+        const uint32_t c = u >> (a.log_l - 1), v = u & (halfL - 1u);
         const uint32_t pos = v & (half - 1);
         const uint32_t i = ((v >> s) << (s + 1)) + pos;
         uint32_t* pa = elem29(lds, c * L + i);
@@ -202,7 +203,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block,
   } else if (a.log_l & 1) {
     const uint32_t halfL = L >> 1;
     for (uint32_t u = threadIdx.x; u < a.C * halfL; u += nt) {
-      const uint32_t c = u / halfL, v = u % halfL;
+      const uint32_t c = u >> (a.log_l - 1), v = u & (halfL - 1u);
       uint32_t* pa = elem29(lds, c * L + 2 * v);
       uint32_t* pb = elem29(lds, c * L + 2 * v + 1);
       const fr29 x = load29(pa);
@@ -213,11 +214,18 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block,
     __syncthreads();
     s = 1;
   }
+  // Register pairs of stages.  The first stage's outputs are left unnormalised where the
+  // second stage can take them (fr29.hpp: add_lazy, sub3r_lazy): y0 and y2 as limb sums, y3
+  // as x2 + 3 r - t3; y1 feeds sub_kr<2> as its first operand and stays normalised.  A pair
+  // starting at stage 0 has twiddle 1 in its first stage and in half of its second
+  // (reduce_small where the twiddle is 1, uniformly across the wave); later pairs always
+  // multiply by their table entry, whose entry 0 is 1, so no lane takes another path.
+  // Values grow by at most 4 r per pair, as with stage-by-stage order.
   const uint32_t quarterL = L >> 2;
   for (; a.pairs && s + 1 < a.log_l; s += 2) {
     const uint32_t half = 1u << s;
     for (uint32_t u = threadIdx.x; u < a.C * quarterL; u += nt) {
-      const uint32_t c = u / quarterL, v = u % quarterL;
+      const uint32_t c = u >> (a.log_l - 2), v = u & (quarterL - 1u);
       const uint32_t pos = v & (half - 1);
       const uint32_t i = ((v >> s) << (s + 2)) + pos;
       uint32_t* p0 = elem29(lds, c * L + i);
@@ -225,22 +233,23 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block,
       uint32_t* p2 = elem29(lds, c * L + i + 2 * half);
       uint32_t* p3 = elem29(lds, c * L + i + 3 * half);
       const fr29 x0 = load29(p0), x1 = load29(p1), x2 = load29(p2), x3 = load29(p3);
-      // stage s: (x0, x1) and (x2, x3), both with w = omega_(2 half)^pos
-      fr29 t1, t3;
-      if (pos != 0) {
-        const fr29 w = load_rec29(a.tw + TW29 * (pos << (a.log_l - 1 - s)));
-        t1 = fr29_mul(x1, w);
-        t3 = fr29_mul(x3, w);
+      // stage s: (x0, x1) and (x2, x3), both with w = omega_(2 half)^pos; stage s + 1:
+      // (y0, y2) with omega_(4 half)^pos, (y1, y3) with omega_(4 half)^(pos + half)
+      fr29 y0, y1, t2, t4;
+      if (s == 0) {
+        const fr29 t1 = fr29_reduce_small(x1), t3 = fr29_reduce_small(x3);
+        y0 = fr29_add_lazy(x0, t1);
+        y1 = fr29_sub_kr<2>(x0, t1);
+        t2 = fr29_reduce_small(fr29_add(x2, t3));
+        t4 = fr29_mul(fr29_sub3r_lazy(x2, t3), load_rec29(a.tw + TW29 * (1u << (a.log_l - 2))));
       } else {
-        t1 = fr29_reduce_small(x1);
-        t3 = fr29_reduce_small(x3);
+        const fr29 w = load_rec29(a.tw + TW29 * (pos << (a.log_l - 1 - s)));
+        const fr29 t1 = fr29_mul(x1, w), t3 = fr29_mul(x3, w);
+        y0 = fr29_add_lazy(x0, t1);
+        y1 = fr29_sub_kr<2>(x0, t1);
+        t2 = fr29_mul(fr29_add_lazy(x2, t3), load_rec29(a.tw + TW29 * (pos << (a.log_l - 2 - s))));
+        t4 = fr29_mul(fr29_sub3r_lazy(x2, t3), load_rec29(a.tw + TW29 * ((pos + half) << (a.log_l - 2 - s))));
       }
-      const fr29 y0 = fr29_add(x0, t1), y1 = fr29_sub_kr<2>(x0, t1);
-      const fr29 y2 = fr29_add(x2, t3), y3 = fr29_sub_kr<2>(x2, t3);
-      // stage s + 1: (y0, y2) with omega_(4 half)^pos, (y1, y3) with omega_(4 half)^(pos + half)
-      const fr29 t2 = pos != 0 ? fr29_mul(y2, load_rec29(a.tw + TW29 * (pos << (a.log_l - 2 - s))))
-                               : fr29_reduce_small(y2);
-      const fr29 t4 = fr29_mul(y3, load_rec29(a.tw + TW29 * ((pos + half) << (a.log_l - 2 - s))));
       store29(p0, fr29_add(y0, t2));
       store29(p2, fr29_sub_kr<2>(y0, t2));
       store29(p1, fr29_add(y1, t4));
@@ -257,7 +266,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, uint32_t block,
     p2 = arg29(a.post[2]);
   }
   for (uint32_t e = threadIdx.x; e < CL; e += nt) {
-    const uint32_t c = e % a.C, k = e / a.C;
+    const uint32_t c = e & (a.C - 1u), k = e >> lc;
     const uint64_t b = b0 + c;
     fr29 x = load29(elem29(lds, c * L + k));
     if (a.do_twist) {

@@ -8,6 +8,10 @@
 //   op 4: to_std(sub_kr<2>(ra, mul(rb, ra)))   (the product is below 2 r: b < 2 r holds)
 //   op 5: 11 butterfly stages x <- x + w y, y <- x - w y + 2 r from reduced inputs,
 //         twiddle w = reduce_small(from_std(b)) fully reduced below r; outputs x, y
+//   op 6: one register pair of stages as the NTT runs it, the first stage's sums left
+//         lazy (add_lazy, sub3r_lazy), from inputs raised by 22 r (the largest values a
+//         stage pair sees): x = (a, b, b, a) + 22 r, w1 = b, w2 = a below r; outputs the
+//         four results (32 words)
 #include <stdio.h>
 #include <stdint.h>
 #include <vector>
@@ -37,7 +41,7 @@ int main() {
     for (unsigned i = 0; i < n; i++) {
       const uint32_t* x = &a[8 * i];
       const uint32_t* y = &b[8 * i];
-      uint32_t out[16] = {0};
+      uint32_t out[32] = {0};
       int words = 8;
       if (op == 0) {
         fr29_to_std(fr29_from_std(x), out);
@@ -49,6 +53,23 @@ int main() {
         const fr29 ra = fr29_reduce_small(fr29_from_std(x)), rb = fr29_reduce_small(fr29_from_std(y));
         const fr29 t = fr29_mul(rb, ra);
         fr29_to_std(op == 3 ? fr29_add(ra, t) : fr29_sub_kr<2>(ra, t), out);
+      } else if (op == 6) {
+        fr29 r22 = {{0}};   // 22 r, normalised
+        const fr29 rr = {{r29c::R(0), r29c::R(1), r29c::R(2), r29c::R(3), r29c::R(4), r29c::R(5), r29c::R(6),
+                          r29c::R(7), r29c::R(8)}};
+        for (int k = 0; k < 22; k++) r22 = fr29_add(r22, rr);
+        const fr29 A = fr29_add(fr29_reduce_small(fr29_from_std(x)), r22);
+        const fr29 B = fr29_add(fr29_reduce_small(fr29_from_std(y)), r22);
+        const fr29 w1 = below_r(y), w2 = below_r(x);
+        const fr29 t1 = fr29_mul(B, w1), t3 = fr29_mul(A, w1);
+        const fr29 y0 = fr29_add_lazy(A, t1), y1 = fr29_sub_kr<2>(A, t1);
+        const fr29 y2 = fr29_add_lazy(B, t3), y3 = fr29_sub3r_lazy(B, t3);
+        const fr29 t2 = fr29_mul(y2, w2), t4 = fr29_mul(y3, w2);
+        fr29_to_std(fr29_add(y0, t2), out);
+        fr29_to_std(fr29_sub_kr<2>(y0, t2), out + 8);
+        fr29_to_std(fr29_add(y1, t4), out + 16);
+        fr29_to_std(fr29_sub_kr<2>(y1, t4), out + 24);
+        words = 32;
       } else {
         fr29 X = fr29_reduce_small(fr29_from_std(x)), Y = fr29_reduce_small(fr29_from_std(y));
         const fr29 w = below_r(y);

@@ -71,3 +71,18 @@ def test_chained_butterflies(exe):
             t = y * Y * RINV % R
             X, Y = (X + t) % R, (X - t) % R
         assert (gx, gy) == (X, Y)
+
+
+def test_paired_stages_with_lazy_sums(exe):
+    """The NTT's register pair of stages with the first stage's sums left unnormalised
+    (limbs up to 2^31 into the second stage's products), from inputs raised by 22 r."""
+    rng = random.Random(31)
+    xs, ys = values(rng, 300), values(rng, 300)[::-1]
+    xs += [R - 1] * 4 + [(1 << 254) + i for i in range(4)]
+    ys += [R - 1, R - 2, 1, 0] + [R - 1 - i for i in range(4)]
+    for outs, a, b in zip(run(exe, 6, xs, ys, 32), xs, ys):
+        x0, x1, x2, x3 = a, b, b, a                 # std values x 2^256; twiddles b, a
+        t1, t3 = x1 * b * RINV % R, x3 * b * RINV % R
+        y0, y1, y2, y3 = x0 + t1, x0 - t1, x2 + t3, x2 - t3
+        t2, t4 = y2 * a * RINV % R, y3 * a * RINV % R
+        assert outs == [(y0 + t2) % R, (y0 - t2) % R, (y1 + t4) % R, (y1 - t4) % R]
