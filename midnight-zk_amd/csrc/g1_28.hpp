@@ -226,22 +226,13 @@ __device__ __forceinline__ f28 f28_quad_bcast(const f28& a) {
   return r;
 }
 
-// Lane q's operand by masks, not by ?: (the compiler turns a select of two loaded values
-// into a load from a selected address, which keeps the operands in scratch memory)
-__device__ __forceinline__ f28 f28_sel4(int q, const f28& a0, const f28& a1, const f28& a2, const f28& a3) {
-  const uint32_t m0 = 0u - (uint32_t)(q == 0), m1 = 0u - (uint32_t)(q == 1);
-  const uint32_t m2 = 0u - (uint32_t)(q == 2), m3 = 0u - (uint32_t)(q == 3);
+// a or b, limb by limb, by a per-lane condition: one v_cndmask per limb.  Both operands
+// are registers here; a select between two loaded values could turn into a load from a
+// selected address (the loads of Q below take their address by lane instead).
+__device__ __forceinline__ f28 f28_pick(bool c, const f28& a, const f28& b) {
   f28 r;
 #pragma unroll
-  for (int i = 0; i < f28c::N; i++) r.l[i] = (a0.l[i] & m0) | (a1.l[i] & m1) | (a2.l[i] & m2) | (a3.l[i] & m3);
-  return r;
-}
-
-__device__ __forceinline__ f28 f28_sel2(bool first, const f28& a, const f28& b) {
-  const uint32_t m = 0u - (uint32_t)first;
-  f28 r;
-#pragma unroll
-  for (int i = 0; i < f28c::N; i++) r.l[i] = (a.l[i] & m) | (b.l[i] & ~m);
+  for (int i = 0; i < f28c::N; i++) r.l[i] = c ? a.l[i] : b.l[i];
   return r;
 }
 
@@ -265,21 +256,23 @@ __device__ __forceinline__ g1x28 g1x28_add_quad(const g1x28 p, const uint32_t* _
   const bool q_inf = q_any == 0;                  // Q is the identity (ZZ literally zero)
   const bool p_inf = g1x28_is_inf(p);
   const int ql = (int)(threadIdx.x & 3u);
+  const bool o1 = (ql & 1) != 0, o2 = (ql & 2) != 0;   // lane bits: operands by two picks
   // step 1: U1 = X1 ZZ2, U2 = X2 ZZ1, S1 = Y1 ZZZ2, S2 = Y2 ZZZ1
   const f28 qa = load_f28(q + (ql == 1 ? 0 : 14));      // X2 (lane 1) or Y2 (lane 3)
   const f28 qb = load_f28(q + (ql == 0 ? 28 : 42));     // ZZ2 (lane 0) or ZZZ2 (lane 2)
-  f28 m = f28_mul(f28_sel4(ql, p.X, qa, p.Y, qa), f28_sel4(ql, qb, p.ZZ, qb, p.ZZZ));
+  f28 m = f28_mul(f28_pick(o1, qa, f28_pick(o2, p.Y, p.X)), f28_pick(o1, f28_pick(o2, p.ZZZ, p.ZZ), qb));
   const f28 U1 = f28_quad_bcast<0>(m), U2 = f28_quad_bcast<1>(m);
   const f28 S1 = f28_quad_bcast<2>(m), S2 = f28_quad_bcast<3>(m);
   const f28 P = f28_sub<4, 1>(U2, U1);            // < 6p
   const f28 R = f28_sub<4, 1>(S2, S1);            // < 6p
   // step 2: PP = P^2, R^2, ZZ1 ZZ2, ZZZ1 ZZZ2
   const f28 qc = load_f28(q + (ql == 2 ? 28 : 42));     // ZZ2 (lane 2) or ZZZ2 (lane 3)
-  m = f28_mul(f28_sel4(ql, P, R, p.ZZ, p.ZZZ), f28_sel4(ql, P, R, qc, qc));
+  const f28 PR = f28_pick(o1, R, P);
+  m = f28_mul(f28_pick(o2, f28_pick(o1, p.ZZZ, p.ZZ), PR), f28_pick(o2, qc, PR));
   const f28 PP = f28_quad_bcast<0>(m), RR = f28_quad_bcast<1>(m);
   const f28 ZZ12 = f28_quad_bcast<2>(m), ZZZ12 = f28_quad_bcast<3>(m);
   // step 3: PPP = P PP, Q = U1 PP, ZZ3 = ZZ1 ZZ2 PP (lane 3 repeats lane 2)
-  m = f28_mul(f28_sel4(ql, P, U1, ZZ12, ZZ12), PP);
+  m = f28_mul(f28_pick(o2, ZZ12, f28_pick(o1, U1, P)), PP);
   const f28 PPP = f28_quad_bcast<0>(m), Qv = f28_quad_bcast<1>(m);
   g1x28 r;
   r.ZZ = f28_quad_bcast<2>(m);
@@ -290,7 +283,7 @@ __device__ __forceinline__ g1x28 g1x28_add_quad(const g1x28 p, const uint32_t* _
   const f28 z = f28_zero();
   const f28 nPPP = f28_sub<4, 1>(z, PPP);
   const bool l0 = ql == 0;
-  m = f28_mul_sum(f28_sel2(l0, ZZZ12, R), f28_sel2(l0, PPP, T), f28_sel2(l0, z, S1), f28_sel2(l0, z, nPPP));
+  m = f28_mul_sum(f28_pick(l0, ZZZ12, R), f28_pick(l0, PPP, T), f28_pick(l0, z, S1), f28_pick(l0, z, nPPP));
   r.ZZZ = f28_quad_bcast<0>(m);
   r.Y = f28_quad_bcast<1>(m);
   if (f28_is_zero_mod_p<6>(P)) {   // no struct-valued ?: here: it would go through memory

@@ -543,6 +543,9 @@ __global__ void __launch_bounds__(64) k_msm_accum(const uint32_t* __restrict__ r
   }
   uint32_t cb = lo;
   uint32_t next = off[cb + 1];
+  // the boundary after next, loaded ahead: a lane that closes a bucket reads it from a
+  // register instead of waiting for a dependent load (off[NB] = M ends every search)
+  uint32_t next2 = off[std::min(cb + 2, NB)];
   g1x28 acc = g1x28_inf();
   uint32_t r_next = refs[e0];
   RawPoint<REC> raw;
@@ -558,10 +561,13 @@ __global__ void __launch_bounds__(64) k_msm_accum(const uint32_t* __restrict__ r
     if (e >= next) {
       store_x28_raw(partials + MZK_X28_WORDS * (size_t)(k + cb), acc);
       acc = g1x28_inf();
-      do {
+      cb++;
+      next = next2;
+      while (next <= e) {   // empty buckets (small sets)
         cb++;
         next = off[cb + 1];
-      } while (next <= e);
+      }
+      next2 = off[std::min(cb + 2, NB)];
     }
     if (live) acc = g1x28_madd<REC>(acc, x, y, (r & NEG_FLAG) != 0);
   }
