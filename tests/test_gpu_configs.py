@@ -95,6 +95,32 @@ def test_few_large_commits_against_2p20_srs(ctx):
 
 
 @pytest.mark.timeout(600)
+def test_shared_sets_against_2p23_srs(ctx):
+    """Twelve 2^16 commits at offsets into a resident 2^23 SRS (16 GiB of window tables):
+    their references need 27 bits, so the low key bits, the reference and its sign no
+    longer share one sort word and the scatter stages keys and references apart; each
+    commit against the closed form."""
+    n_srs, n, nj = 1 << 23, 1 << 16, 12
+    print(f"\n[{nj} commits of 2^16 against a resident 2^23 SRS]", flush=True)
+    tau = C.fr_from_seed(0xC6, 0, 1)[0]
+    tau_i = C.fr_rows_to_ints([tau])[0]
+    srs = ctx.srs_generate(tau, n_srs)
+    buf = empty_dev(nj * n, 4)
+    ctx.fr_from_seed_dev(98, 7, nj * n, buf)
+    ctx.synchronize()
+    offs = [0, n_srs - n] + [(i * 700001) % (n_srs - n) for i in range(1, nj - 1)]
+    jobs = [(buf[i * n:(i + 1) * n], n, off) for i, off in enumerate(offs)]
+    outs = srs.commit_batch(jobs)
+    assert ctx.msm_last_plan()["bucket_sets"] == nj
+    sc = from_dev(buf).reshape(-1, 4)
+    for i, ((_, _, off), out) in enumerate(zip(jobs, outs)):
+        s = C.fr_rows_to_ints([C.poly_eval(sc[i * n:(i + 1) * n], tau)])[0] * pow(tau_i, off, O.R_MOD) % O.R_MOD
+        expect = O.g1_affine_from_mont_limbs(C.g1_mul_generator(C.ints_to_fr_rows([s])[0]))
+        assert O.g1_jacobian_from_mont_limbs(out) == expect, (i, off)
+    srs.close()
+
+
+@pytest.mark.timeout(600)
 def test_many_shared_sets_commits(ctx):
     """200 commits of 2^15 terms in one batch: 200 shared bucket sets, whose sort bins
     need fewer, larger scalar tiles to keep the (bin, tile) histogram within the scan;
