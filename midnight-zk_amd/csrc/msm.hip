@@ -35,6 +35,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <functional>
@@ -1027,11 +1028,28 @@ static int scatter_lds_attr() {
 // Host workers for the per-job tails: up to 15 threads started once and kept for the
 // process (a batch's tails took ~0.1 ms more when each call started its own threads).
 // One batch runs at a time; a caller that finds the pool busy runs its tails itself.
+//
+// A sleeping worker takes ~50-100 us to wake, longer than a five-job tail takes on the
+// calling thread alone, so a caller about to wait for the device calls prepare(n): up to
+// SPIN_WORKERS workers wake then and poll for the batch (at most SPIN_NS) while the device
+// finishes.  run() waits only for the batch's jobs and for the workers that took it, never
+// for workers still asleep: the batch is withdrawn once the caller has drained it, and a
+// worker that wakes later finds nothing and sleeps again.
 class TailPool {
  public:
   static TailPool& get() {
     static TailPool* p = new TailPool();   // never destroyed: its workers wait until exit
     return *p;
+  }
+  void prepare(size_t n) {
+    if (n < 2 || workers_.empty() || forked_child()) return;
+    spin_until.store(now_ns() + SPIN_NS, std::memory_order_relaxed);
+    spin_slots.store((int)std::min<size_t>(n - 1, SPIN_WORKERS), std::memory_order_relaxed);
+    {
+      std::lock_guard<std::mutex> g(mu);
+      wake++;
+    }
+    cv.notify_all();
   }
   // fn(j) for j < n on the workers and the calling thread
   void run(size_t n, const std::function<void(size_t)>& fn) {
@@ -1044,18 +1062,29 @@ class TailPool {
       std::lock_guard<std::mutex> g(mu);
       task = &fn;
       total = n;
-      next.store(0);
-      pending = workers_.size();
-      gen++;
+      next.store(0, std::memory_order_relaxed);
+      completed.store(0, std::memory_order_relaxed);
+      holders = 0;
+      gen.fetch_add(1, std::memory_order_release);
     }
     cv.notify_all();
-    drain();
+    for (size_t j; (j = next.fetch_add(1)) < n;) {
+      fn(j);
+      completed.fetch_add(1, std::memory_order_acq_rel);
+    }
     std::unique_lock<std::mutex> g(mu);
-    done.wait(g, [&] { return pending == 0; });
-    task = nullptr;
+    task = nullptr;   // withdrawn: workers that wake from now on leave it alone
+    done.wait(g, [&] { return holders == 0 && completed.load(std::memory_order_acquire) == n; });
   }
 
  private:
+  static constexpr size_t SPIN_WORKERS = 4;
+  static constexpr uint64_t SPIN_NS = 5000000;   // 5 ms
+  static uint64_t now_ns() {
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+  }
   // a child of fork() has none of the parent's worker threads: it runs its tails itself
   static std::atomic<bool>& forked_flag() {
     static std::atomic<bool> f{false};
@@ -1069,32 +1098,55 @@ class TailPool {
     for (size_t t = 0; t < nt; t++) workers_.emplace_back([this] { loop(); });
     for (auto& w : workers_) w.detach();
   }
-  void drain() {
-    for (size_t j; (j = next.fetch_add(1)) < total;) (*task)(j);
-  }
   void loop() {
-    uint64_t seen = 0;
+    uint64_t seen = 0, seen_wake = 0;
     for (;;) {
+      bool spin = false;
       {
         std::unique_lock<std::mutex> g(mu);
-        cv.wait(g, [&] { return gen != seen; });
-        seen = gen;
+        cv.wait(g, [&] { return gen.load(std::memory_order_relaxed) != seen || wake != seen_wake; });
+        spin = gen.load(std::memory_order_relaxed) == seen;   // a prepare() and no batch yet
+        seen_wake = wake;
       }
-      drain();
+      if (spin) {
+        if (spin_slots.fetch_sub(1, std::memory_order_relaxed) <= 0) continue;   // enough pollers
+        const uint64_t until = spin_until.load(std::memory_order_relaxed);
+        while (gen.load(std::memory_order_acquire) == seen && now_ns() < until) __builtin_ia32_pause();
+        if (gen.load(std::memory_order_acquire) == seen) continue;   // no batch came: sleep again
+      }
+      const std::function<void(size_t)>* fn = nullptr;
+      size_t n = 0;
+      {
+        std::lock_guard<std::mutex> g(mu);
+        seen = gen.load(std::memory_order_relaxed);
+        if (task == nullptr) continue;   // already drained and withdrawn
+        fn = task;
+        n = total;
+        holders++;
+      }
+      for (size_t j; (j = next.fetch_add(1)) < n;) {
+        (*fn)(j);
+        completed.fetch_add(1, std::memory_order_acq_rel);
+      }
       std::lock_guard<std::mutex> g(mu);
-      if (--pending == 0) done.notify_one();
+      if (--holders == 0) done.notify_all();
     }
   }
   std::mutex run_mu, mu;
   std::condition_variable cv, done;
-  const std::function<void(size_t)>* task = nullptr;
-  size_t total = 0, pending = 0;
-  std::atomic<size_t> next{0};
-  uint64_t gen = 0;
+  const std::function<void(size_t)>* task = nullptr;   // the current batch, under mu
+  size_t total = 0;
+  int holders = 0;                  // workers that took the current batch, under mu
+  uint64_t wake = 0;                // prepare() requests, under mu
+  std::atomic<uint64_t> gen{0};     // batches started (written under mu)
+  std::atomic<size_t> next{0}, completed{0};
+  std::atomic<uint64_t> spin_until{0};
+  std::atomic<int> spin_slots{0};
   std::vector<std::thread> workers_;
 };
 
 static void parallel_jobs(size_t n, const std::function<void(size_t)>& fn) { TailPool::get().run(n, fn); }
+static void prepare_jobs(size_t n) { TailPool::get().prepare(n); }
 
 // Host Horner over one job's channel positions.  Set s (of sets_per_job) weighs
 // 2^(c s); inside it, sum_t A_t has weight 1 and U_j (bit j of t) 2^(log2 L + j).
@@ -1379,6 +1431,7 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
     uint32_t* tot = chan;
     const uint32_t* h = res;
     double h0 = host_ms();
+    prepare_jobs(nj);
     MZK_HIP(hipMemcpyAsync(res, tot, sets * 48 * 4, hipMemcpyDeviceToHost, c->stream));
     MZK_HIP(hipMemcpyAsync(res + sets * 48, off + NB, 4, hipMemcpyDeviceToHost, c->stream));
     MZK_HIP(hipStreamSynchronize(c->stream));
@@ -1400,6 +1453,7 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   }
   const uint32_t* h = res;
   double h0 = host_ms();
+  prepare_jobs(nj);
   MZK_HIP(hipMemcpyAsync(res, chan, nchan * 48 * 4, hipMemcpyDeviceToHost, c->stream));
   MZK_HIP(hipMemcpyAsync(res + nchan * 48, off + NB, 4, hipMemcpyDeviceToHost, c->stream));
   MZK_HIP(hipStreamSynchronize(c->stream));
