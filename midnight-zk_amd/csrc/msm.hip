@@ -1180,7 +1180,7 @@ static void job_tail_sets(const uint32_t* h, int windows, int c, uint64_t out[18
 }
 
 int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, const uint32_t* d_bases,
-                 uint32_t ref_stride, uint64_t* outs, bool records) {
+                 uint32_t ref_stride, uint64_t* outs, bool records, const std::function<int()>* on_enqueued) {
   const size_t nj = jobs.size();
   size_t ntot = 0;
   for (auto& j : jobs) ntot += j.n;
@@ -1426,6 +1426,7 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
     MZK_HIP(hipGetLastError());
     return MZK_OK;
   }));
+  if (on_enqueued) MZK_TRY((*on_enqueued)());
   if (small_sets) {
     // the window combination on the host
     uint32_t* tot = chan;

@@ -1,6 +1,7 @@
 // MSM planning and the shared device pipeline (msm.hip), used by the per-call
 // entry points and by the resident-SRS commit path.
 #pragma once
+#include <functional>
 #include <vector>
 
 #include "ctx.hpp"
@@ -29,8 +30,12 @@ struct MsmJob {
 // Runs all jobs through one pipeline (one sort, one accumulation, one reduction):
 // outs[18 j ..] receives job j's Jacobian result.  records: d_bases holds 128-byte
 // accumulation records (a resident SRS's shifted tables) instead of blst affine points.
+// on_enqueued (optional) runs once the device sequence is on the context stream and
+// before the host waits for it, with the context lock held: a caller can order other
+// streams after the MSM's kernels (an event recorded there fires when they end).
 int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, const uint32_t* d_bases,
-                 uint32_t ref_stride, uint64_t* outs, bool records = false);
+                 uint32_t ref_stride, uint64_t* outs, bool records = false,
+                 const std::function<int()>* on_enqueued = nullptr);
 
 // Runs the pipeline on device scalars (n x 8 u32 Montgomery) and device bases.
 // Reference r of scalar i in window w is ref_offset + i (+ w * ref_stride when

@@ -13,6 +13,7 @@
 // kernels.  Host work is the transcript (BLAKE2b), challenges and point encodings.
 #include <algorithm>
 #include <chrono>
+#include <functional>
 #include <vector>
 
 #include "blake2b.hpp"
@@ -203,8 +204,9 @@ double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// on_enqueued: see msm_run_jobs (runs with the context lock held, before the host waits)
 int msm_batch(mzk_ctx* c, const mzk_srs* srs, const std::vector<std::pair<const void*, size_t>>& jobs,
-              std::vector<uint64_t>& outs) {
+              std::vector<uint64_t>& outs, const std::function<int()>* on_enqueued = nullptr) {
   std::vector<mzk_msm_job> j(jobs.size());
   for (size_t i = 0; i < jobs.size(); i++) {
     j[i].d_coeffs = jobs[i].first;
@@ -212,7 +214,7 @@ int msm_batch(mzk_ctx* c, const mzk_srs* srs, const std::vector<std::pair<const 
     j[i].offset = 0;
   }
   outs.assign(18 * jobs.size(), 0);
-  return mzk_msm_srs_batch_dev(c, srs, j.data(), j.size(), outs.data());
+  return mzk::srs_commit_batch(c, srs, j.data(), j.size(), outs.data(), on_enqueued);
 }
 
 int eval_batch(mzk_ctx* c, const std::vector<std::pair<const void*, size_t>>& polys, const std::vector<Fr>& xs,
@@ -304,16 +306,15 @@ struct AuxStream {
   }
 };
 
-// The transforms of columns cols[] of every proof on the auxiliary stream, after the main
-// stream reaches this point: to coefficients (src -> polys), then to the extended coset
-// (polys -> cos); `done` is recorded after them.
-static int aux_transforms(mzk_ctx* c, AuxStream& ax, hipEvent_t ready, hipEvent_t done,
-                          const std::vector<std::pair<const void*, std::pair<uint32_t*, uint32_t*>>>& cols,
-                          uint32_t k) {
+// Columns to transform on the auxiliary stream: (source, (coefficients out, coset out)).
+typedef std::vector<std::pair<const void*, std::pair<uint32_t*, uint32_t*>>> TransformCols;
+
+// The transforms of columns cols[] of every proof on the auxiliary stream once `ready`
+// (already recorded on the context stream) has passed: to coefficients (src -> polys), then
+// to the extended coset (polys -> cos); `done` is recorded after them.
+static int aux_enqueue(AuxStream& ax, hipEvent_t ready, hipEvent_t done, const TransformCols& cols, uint32_t k) {
   {
-    std::lock_guard<std::mutex> g(c->mu);
-    MZK_HIP(hipSetDevice(c->device));
-    MZK_HIP(hipEventRecord(ready, c->stream));
+    std::lock_guard<std::mutex> g(ax.aux->mu);
     MZK_HIP(hipStreamWaitEvent(ax.aux->stream, ready, 0));
   }
   ax.used = true;
@@ -336,6 +337,18 @@ static int aux_transforms(mzk_ctx* c, AuxStream& ax, hipEvent_t ready, hipEvent_
   std::lock_guard<std::mutex> g(ax.aux->mu);
   MZK_HIP(hipEventRecord(done, ax.aux->stream));
   return MZK_OK;
+}
+
+// Each column to coefficients (n) and to the extended coset (4n) on the auxiliary stream,
+// after the work now on the context stream; `done` is recorded after them.
+static int aux_transforms(mzk_ctx* c, AuxStream& ax, hipEvent_t ready, hipEvent_t done, const TransformCols& cols,
+                          uint32_t k) {
+  {
+    std::lock_guard<std::mutex> g(c->mu);
+    MZK_HIP(hipSetDevice(c->device));
+    MZK_HIP(hipEventRecord(ready, c->stream));
+  }
+  return aux_enqueue(ax, ready, done, cols, k);
 }
 
 }  // namespace
@@ -425,26 +438,35 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
     }
   }
   // the advice columns' transforms (to coefficients, to the extended coset) depend on the
-  // advice alone: they run on the auxiliary stream while this stream commits the advice
-  // and builds the permutation products (they fill the commitments' sort and reduction
-  // phases, where most SIMDs wait); the numerator waits for them
+  // advice alone: they run on the auxiliary stream from the end of the advice commitments'
+  // kernels on, beside the host's tail of those commitments, the transcript and the
+  // permutation products (small, latency-bound kernels that leave most SIMDs idle); the
+  // numerator waits for them.  Started with the commitments instead, they competed with
+  // the accumulation for the same VALU slots and saved nothing.
   AuxStream ax;
+  TransformCols adv_cols;
   {
     if (!c->aux) PV_TRY(mzk_ctx_create(c->device, &c->aux));
     ax.aux = c->aux;
     MZK_HIP(hipSetDevice(c->device));
     for (hipEvent_t& e : ax.ev) MZK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    std::vector<std::pair<const void*, std::pair<uint32_t*, uint32_t*>>> cols;
     for (size_t p = 0; p < np; p++)
-      for (int j = 0; j < NADV; j++) cols.push_back({B[p].adv[j], {B[p].polys[j], B[p].cos[j]}});
-    PV_TRY(aux_transforms(c, ax, ax.ev[0], ax.ev[1], cols, k));
+      for (int j = 0; j < NADV; j++) adv_cols.push_back({B[p].adv[j], {B[p].polys[j], B[p].cos[j]}});
   }
   std::vector<uint64_t> jac;
   {
     std::vector<std::pair<const void*, size_t>> jobs;
     for (size_t p = 0; p < np; p++)
       for (int j = 0; j < NADV; j++) jobs.push_back({B[p].adv[j], n});
-    PV_TRY(msm_batch(c, key->g_lagrange, jobs, jac));
+    bool queued = false;
+    const std::function<int()> after_commit_kernels = [&]() -> int {
+      MZK_HIP(hipEventRecord(ax.ev[0], c->stream));   // the context lock is held here
+      MZK_TRY(aux_enqueue(ax, ax.ev[0], ax.ev[1], adv_cols, k));
+      queued = true;
+      return MZK_OK;
+    };
+    PV_TRY(msm_batch(c, key->g_lagrange, jobs, jac, &after_commit_kernels));
+    if (!queued) PV_TRY(aux_transforms(c, ax, ax.ev[0], ax.ev[1], adv_cols, k));
     std::vector<uint8_t> cb(48 * jobs.size());
     compress_many(jac.data(), jobs.size(), cb.data());
     for (size_t p = 0; p < np; p++)
@@ -523,9 +545,9 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
       }
     }
   }
-  {   // z's transforms likewise overlap its commitment (few non-zero deltas: mostly the
-      // MSM's sort and reduction phases)
-    std::vector<std::pair<const void*, std::pair<uint32_t*, uint32_t*>>> cols;
+  {   // z's transforms overlap its commitment (few non-zero deltas: mostly the MSM's sort
+      // and reduction phases)
+    TransformCols cols;
     for (size_t p = 0; p < np; p++)
       for (int s = 0; s < NSETS; s++) cols.push_back({B[p].z[s], {B[p].polys[NADV + s], B[p].cos[NADV + s]}});
     PV_TRY(aux_transforms(c, ax, ax.ev[2], ax.ev[3], cols, k));

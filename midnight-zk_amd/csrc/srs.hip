@@ -359,6 +359,39 @@ static int srs_alloc(mzk_ctx* c, size_t n, mzk_srs** out) {
 namespace mzk {
 int srs_alloc_bases(mzk_ctx* c, size_t n, mzk_srs** out) { return srs_alloc(c, n, out); }
 int srs_finish_bases(mzk_ctx* c, mzk_srs* s) { return srs_finish(c, s); }
+
+int srs_commit_batch(mzk_ctx* c, const mzk_srs* s, const mzk_msm_job* jobs, size_t njobs, uint64_t* outs,
+                     const std::function<int()>* on_enqueued) {
+  MZK_REQUIRE(c != nullptr && s != nullptr && (jobs != nullptr || njobs == 0) && (outs != nullptr || njobs == 0),
+              "mzk_msm_srs_batch_dev: null argument");
+  if (njobs == 0) return MZK_OK;
+  std::vector<MsmJob> v(njobs);
+  size_t total = 0;
+  for (size_t j = 0; j < njobs; j++) {
+    MZK_REQUIRE(jobs[j].offset <= s->n && jobs[j].n <= s->n - jobs[j].offset,
+                "mzk_msm_srs_batch_dev: job %zu (%zu bases from %zu) exceeds the SRS (%zu)", j, jobs[j].n,
+                jobs[j].offset, s->n);
+    MZK_REQUIRE(jobs[j].d_coeffs != nullptr || jobs[j].n == 0, "mzk_msm_srs_batch_dev: job %zu has no coefficients",
+                j);
+    v[j].d_coeffs = jobs[j].d_coeffs;
+    v[j].n = jobs[j].n;
+    v[j].ref_offset = (uint32_t)jobs[j].offset;
+    total += jobs[j].n;
+  }
+  std::lock_guard<std::mutex> g(c->mu);
+  MZK_HIP(hipSetDevice(c->device));
+  // the shared set costs 2^15 buckets of reduction per job: worth it once jobs average
+  // 2^15 terms; smaller jobs take per-window sets (small sets reduce in one workgroup each)
+  if (s->shifted && total / njobs >= SHARED_MIN_BATCH_AVG) {
+    MsmPlan p = msm_plan(total, 0, c->num_cus, s->shifted_c);
+    return msm_run_jobs(c, p, v, (const uint32_t*)s->shifted, (uint32_t)s->n, outs, true, on_enqueued);
+  }
+  MsmPlan p = msm_plan(total / njobs + 1, 1, c->num_cus, 0);
+  MsmPlan pt = msm_plan(total, 1, c->num_cus, p.c);   // K from the batch's total digits
+  if (s->shifted)   // window 0 of the shifted tables: the bases as records
+    return msm_run_jobs(c, pt, v, (const uint32_t*)s->shifted, 0, outs, true, on_enqueued);
+  return msm_run_jobs(c, pt, v, (const uint32_t*)s->bases, 0, outs, false, on_enqueued);
+}
 }  // namespace mzk
 
 extern "C" {
@@ -502,35 +535,7 @@ int mzk_msm_srs_dev(mzk_ctx* c, const mzk_srs* s, size_t offset, const void* d_c
 }
 
 int mzk_msm_srs_batch_dev(mzk_ctx* c, const mzk_srs* s, const mzk_msm_job* jobs, size_t njobs, uint64_t* outs) {
-  MZK_REQUIRE(c != nullptr && s != nullptr && (jobs != nullptr || njobs == 0) && (outs != nullptr || njobs == 0),
-              "mzk_msm_srs_batch_dev: null argument");
-  if (njobs == 0) return MZK_OK;
-  std::vector<MsmJob> v(njobs);
-  size_t total = 0;
-  for (size_t j = 0; j < njobs; j++) {
-    MZK_REQUIRE(jobs[j].offset <= s->n && jobs[j].n <= s->n - jobs[j].offset,
-                "mzk_msm_srs_batch_dev: job %zu (%zu bases from %zu) exceeds the SRS (%zu)", j, jobs[j].n,
-                jobs[j].offset, s->n);
-    MZK_REQUIRE(jobs[j].d_coeffs != nullptr || jobs[j].n == 0, "mzk_msm_srs_batch_dev: job %zu has no coefficients",
-                j);
-    v[j].d_coeffs = jobs[j].d_coeffs;
-    v[j].n = jobs[j].n;
-    v[j].ref_offset = (uint32_t)jobs[j].offset;
-    total += jobs[j].n;
-  }
-  std::lock_guard<std::mutex> g(c->mu);
-  MZK_HIP(hipSetDevice(c->device));
-  // the shared set costs 2^15 buckets of reduction per job: worth it once jobs average
-  // 2^15 terms; smaller jobs take per-window sets (small sets reduce in one workgroup each)
-  if (s->shifted && total / njobs >= SHARED_MIN_BATCH_AVG) {
-    MsmPlan p = msm_plan(total, 0, c->num_cus, s->shifted_c);
-    return msm_run_jobs(c, p, v, (const uint32_t*)s->shifted, (uint32_t)s->n, outs, true);
-  }
-  MsmPlan p = msm_plan(total / njobs + 1, 1, c->num_cus, 0);
-  MsmPlan pt = msm_plan(total, 1, c->num_cus, p.c);   // K from the batch's total digits
-  if (s->shifted)   // window 0 of the shifted tables: the bases as records
-    return msm_run_jobs(c, pt, v, (const uint32_t*)s->shifted, 0, outs, true);
-  return msm_run_jobs(c, pt, v, (const uint32_t*)s->bases, 0, outs);
+  return mzk::srs_commit_batch(c, s, jobs, njobs, outs, nullptr);
 }
 
 }  // extern "C"

@@ -1,5 +1,7 @@
 // The resident-SRS object behind the opaque mzk_srs handle.
 #pragma once
+#include <functional>
+
 #include "ctx.hpp"
 
 #define MZK_SRS_REC_WORDS 32
@@ -22,4 +24,8 @@ int batch_normalize(mzk_ctx* c, const uint32_t* d_xyzz, size_t n, uint32_t* d_af
 // device set): a new SRS of n bases, and its shifted tables once the bases are written.
 int srs_alloc_bases(mzk_ctx* c, size_t n, mzk_srs** out);
 int srs_finish_bases(mzk_ctx* c, mzk_srs* s);
+// mzk_msm_srs_batch_dev with the msm_run_jobs hook: on_enqueued runs once the batch's
+// device sequence is enqueued, before the host waits for it.
+int srs_commit_batch(mzk_ctx* c, const mzk_srs* s, const mzk_msm_job* jobs, size_t njobs, uint64_t* outs,
+                     const std::function<int()>* on_enqueued);
 }  // namespace mzk
