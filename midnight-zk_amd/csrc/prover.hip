@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <functional>
+#include <mutex>
 #include <vector>
 
 #include "blake2b.hpp"
@@ -140,6 +141,37 @@ static void compress_many(const uint64_t* jac, size_t count, uint8_t* out) {
 }
 
 static Fr fr_pow_u64(const Fr& a, uint64_t e) { return host::fr_pow(a, e); }
+
+// The field constants of a proof at 2^k rows, which depend on k alone: computed once per
+// k and process (five inversions and their powers cost the host ~80 us per proof, spent
+// where the device waits for the next phase's launches).
+struct KConsts {
+  Fr omega, omega_inv, delta, omega_last;
+  Fr te[4];   // 1 / (X^n - 1) on the four cosets zeta w_ext^i of the extended domain
+};
+static const KConsts& k_consts(uint32_t k, uint32_t blinding_rows) {
+  static std::mutex mu;
+  static std::vector<std::pair<uint64_t, KConsts*>> cache;   // (k, blinding rows) -> constants, kept
+  const uint64_t key = ((uint64_t)k << 32) | blinding_rows;
+  std::lock_guard<std::mutex> g(mu);
+  for (auto& e : cache)
+    if (e.first == key) return *e.second;
+  KConsts* kc = new KConsts();
+  const Fr one = host::fr_one();
+  const uint64_t n = (uint64_t)1 << k;
+  kc->omega = host::fr_root_of_unity(k);
+  kc->omega_inv = host::fr_inv(kc->omega);
+  kc->delta = fr_pow_u64(host::fr_from_u64(7), 1ull << 32);   // fq.rs:501-509
+  kc->omega_last = fr_pow_u64(kc->omega_inv, blinding_rows + 1);
+  const Fr wext = host::fr_root_of_unity(k + 2), zeta = host::fr_const(host::kFrZeta);
+  Fr cur = zeta;
+  for (int i = 0; i < 4; i++) {
+    kc->te[i] = host::fr_inv(host::fr_sub(fr_pow_u64(cur, n), one));
+    cur = host::fr_mul(cur, wext);
+  }
+  cache.push_back({key, kc});
+  return *kc;
+}
 
 }  // namespace prover
 }  // namespace mzk
@@ -424,8 +456,8 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
   hipStream_t st = c->stream;
   std::vector<Transcript> T(np);
   const Fr one = host::fr_one();
-  const Fr omega = host::fr_root_of_unity(k), omega_inv = host::fr_inv(omega);
-  const Fr delta = fr_pow_u64(host::fr_from_u64(7), 1ull << 32);   // fq.rs:501-509
+  const KConsts& kc = k_consts(k, BF);
+  const Fr omega = kc.omega, delta = kc.delta;
 
   // ---- VK hash, advice with blinding rows, advice commitments (prover.rs:101-108, 541-631)
   for (size_t p = 0; p < np; p++) {
@@ -590,15 +622,7 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
   }
   mark(2);
   mark(3);
-  std::vector<Fr> te(4);
-  {
-    const Fr wext = host::fr_root_of_unity(k + 2), zeta = host::fr_const(host::kFrZeta);
-    Fr cur = zeta;
-    for (int i = 0; i < 4; i++) {
-      te[i] = host::fr_inv(host::fr_sub(fr_pow_u64(cur, n), one));
-      cur = host::fr_mul(cur, wext);
-    }
-  }
+  const Fr* te = kc.te;
   for (size_t p = 0; p < np; p++) {
     mzk_plonk_numerator A;
     memset(&A, 0, sizeof A);
@@ -676,7 +700,7 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
 
   // ---- evaluations (prover.rs:798-869, permutation/prover.rs:266-340)
   std::vector<Fr> x(np), xn1(np), xl(np);
-  const Fr omega_last = fr_pow_u64(omega_inv, BF + 1);
+  const Fr omega_last = kc.omega_last;
   {
     std::vector<std::pair<const void*, size_t>> polys;
     std::vector<Fr> pts;
