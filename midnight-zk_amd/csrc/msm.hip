@@ -1285,8 +1285,12 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   const size_t nchunks = (M_max + K - 1) / K;
   // reduction geometry: segments of L buckets, T per set; channels A + bits of t.
   // One MSM alone: a thread per bucket keeps chains short.  Many sets: segments of
-  // 4 cut the channel work, which is then parallel across sets.
-  const uint32_t L = std::min<uint32_t>(sets > 1 ? 4 : 1, p.B);
+  // 4 cut the channel work, which is then parallel across sets; with segments enough for
+  // ~3 waves per SIMD at 8 buckets (48 shared sets and more), the reduction is bound by
+  // throughput rather than by its dependent chains and segments of 8 halve the segment
+  // tree's work (96 uneven commits -1.3 %; five sets were 4 % slower that way).
+  const bool many_sets = (size_t)sets * (p.B / 8) >= (size_t)3 * 64 * 1024;
+  const uint32_t L = std::min<uint32_t>(sets > 1 ? (many_sets ? 8 : 4) : 1, p.B);
   const uint32_t T = p.B / L;
   const uint32_t tbits = (uint32_t)ceil_log2(T);
   const uint32_t nch = 1 + tbits;
