@@ -21,9 +21,9 @@
 //  7. k_chan_sum      channels: sum_t A_t, and U_j = sum of the R_t whose index t has
 //                     bit j set, so sum_b (b+1) S_b = sum_t A_t + L sum_j 2^j U_j --
 //                     plain sums in any order, short dependency chains
-//     (one set of 2^t buckets instead: k_chan_tree merges the partials of 128
+//     (up to 4 sets of 2^t buckets instead: k_chan_tree merges the partials of 128
 //     buckets per block and reduces them as a binary tree whose nodes carry their
-//     per-bit sums, k_chan_top sums the block records into the same channels)
+//     per-bit sums, k_chan_top sums each set's block records into the same channels)
 //  8. host tail       Horner over the channel bit positions (one host core runs a
 //                     dependent chain of point operations ~20x faster than one wave).
 //
@@ -753,6 +753,7 @@ static constexpr uint32_t CT_LEVELS = 7;
 static constexpr uint32_t CT_LEAVES = 1u << CT_LEVELS;   // buckets per block, a quad each
 static constexpr uint32_t CT_REC = CT_LEVELS + 1;         // points per block record: P, A_0..A_6
 static constexpr uint32_t CT_TOP_QUADS = 128;
+static constexpr size_t TREE_SETS = 4;   // most sets reduced by the block tree
 
 // Slot of A_i of node m at the current level: A_i exists for 64 >> i nodes when created.
 __device__ __forceinline__ uint32_t ct_acc_slot(uint32_t i, uint32_t m) {
@@ -805,16 +806,20 @@ __global__ void __launch_bounds__(4 * CT_LEAVES) k_chan_tree(const uint32_t* __r
   }
 }
 
-// Channel h of nblk block records (nblk = 2^kbits): h = 0 sums every P_k; h = 1 + i
+// Channel h (blockIdx.x) of set s (blockIdx.y), from the set's nblk block records
+// (nblk = 2^kbits, set s's at in[CT_REC nblk s ...]): h = 0 sums every P_k; h = 1 + i
 // (i < CT_LEVELS) sums every A_i; h = 1 + CT_LEVELS + j sums the P_k whose k has bit j
 // set.  A quad per CT_TOP_QUADS-th record (the first of its terms loaded, not added to the
 // identity: each addition step is a serial step of the kernel), then a tree over the quads
-// in LDS; writes the blst-layout XYZZ (canonical 12 x 32-bit limbs) of channel h to out[48 h].
+// in LDS; writes the blst-layout XYZZ (canonical 12 x 32-bit limbs) of the channel to
+// out[48 (s gridDim.x + h)].
 __global__ void __launch_bounds__(4 * CT_TOP_QUADS) k_chan_top(const uint32_t* __restrict__ in, uint32_t nblk,
                                                               uint32_t* __restrict__ out) {
   extern __shared__ uint32_t lds[];
   const uint32_t qd = threadIdx.x >> 2, ql = threadIdx.x & 3u;
   const uint32_t h = blockIdx.x;
+  in += (size_t)MZK_X28_WORDS * CT_REC * nblk * blockIdx.y;
+  out += (size_t)48 * gridDim.x * blockIdx.y;
   const uint32_t slot = (h >= 1 && h <= CT_LEVELS) ? h : 0;
   const uint32_t bit = h > CT_LEVELS ? h - 1 - CT_LEVELS : 0;
   const bool all = h <= CT_LEVELS;
@@ -1289,17 +1294,21 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   // ~3 waves per SIMD at 8 buckets (48 shared sets and more), the reduction is bound by
   // throughput rather than by its dependent chains and segments of 8 halve the segment
   // tree's work (96 uneven commits -1.3 %; five sets were 4 % slower that way).
+  // Up to TREE_SETS sets of 2^t buckets (t >= 8) take the block tree of a single set, a
+  // quad of lanes per bucket: two to four sets in segments leave the SIMDs mostly idle
+  // through the segments' dependent additions (2^15-bucket sets, reduction kernels: two
+  // sets 0.51 -> 0.30 ms, four 0.49 -> 0.46 ms; from five sets on the tree's extra
+  // additions cost more than the segments' chains: five 0.43 -> 0.47 ms, eight 0.58 -> 0.64).
+  const bool small_sets = p.per_window && p.B <= 256;
   const bool many_sets = (size_t)sets * (p.B / 8) >= (size_t)3 * 64 * 1024;
-  const uint32_t L = std::min<uint32_t>(sets > 1 ? (many_sets ? 8 : 4) : 1, p.B);
+  const bool tree = sets <= TREE_SETS && !small_sets && (p.B & (p.B - 1)) == 0 && p.B >= 2 * CT_LEAVES;
+  const uint32_t L = std::min<uint32_t>(tree || sets == 1 ? 1 : (many_sets ? 8 : 4), p.B);
   const uint32_t T = p.B / L;
   const uint32_t tbits = (uint32_t)ceil_log2(T);
   const uint32_t nch = 1 + tbits;
   const uint32_t parts = std::max<uint32_t>(1, T / 2048);
   const size_t nchan = sets * nch;
   const uint32_t lbits = (uint32_t)ceil_log2(L);
-  // one set of 2^t buckets, t >= 8: channels by the block tree (k_chan_tree, k_chan_top)
-  const bool tree = sets == 1 && L == 1 && !(p.per_window && p.B <= 256) && T == NB && T >= 2 * CT_LEAVES &&
-                    (T & (T - 1)) == 0;
   // segments of L > 1 buckets, 2^7 <= T <= 2^14 per set: channels by the segment tree
   // (k_seg_tree, k_seg_top), its records written over the chunk partials once merged
   const bool segtree = L > 1 && !(p.per_window && p.B <= 256) && (T & (T - 1)) == 0 && T >= 2 * ST_LEAVES &&
@@ -1350,7 +1359,6 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   uint32_t* segA = partials + (nchunks + NB) * MZK_X28_WORDS;
   uint32_t* segR = segA + nseg * MZK_X28_WORDS;
   // scratch D: channel partials and channel sums (small sets: one sum per set)
-  const bool small_sets = p.per_window && p.B <= 256;
   size_t d_words = nchan * parts * MZK_X28_WORDS + std::max(nchan, sets) * 48 + 64;
   MZK_TRY(c->scratch[SCR_MSM_D].reserve(d_words * 4));
   uint32_t* chan_parts = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
@@ -1402,11 +1410,11 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
                    256 * MZK_X28_WORDS * 4, off, NB, K, p.B, (uint32_t)sets, partials, chan);
       }
     } else if (tree) {
-      // a single set of 2^t buckets: the merge and the tree by blocks, then the channels
+      // sets of 2^t buckets: the merge and the tree by blocks, then the channels of each set
       MZK_LAUNCH(c, "k_chan_tree", k_chan_tree, dim3(NB / CT_LEAVES), dim3(4 * CT_LEAVES),
                  (2 * CT_LEAVES - 1) * MZK_X28_WORDS * 4, off, K, partials, segA);
-      MZK_LAUNCH(c, "k_chan_top", k_chan_top, dim3((unsigned)nch), dim3(4 * CT_TOP_QUADS),
-                 CT_TOP_QUADS * MZK_X28_WORDS * 4, segA, NB / CT_LEAVES, chan);
+      MZK_LAUNCH(c, "k_chan_top", k_chan_top, dim3((unsigned)nch, (unsigned)sets), dim3(4 * CT_TOP_QUADS),
+                 CT_TOP_QUADS * MZK_X28_WORDS * 4, segA, T / CT_LEAVES, chan);
     } else {
       if (L == 1)   // a single set: a quad per bucket
         MZK_LAUNCH(c, "k_bucket_seg", k_bucket_merge_quad, dim3(grid_for((size_t)NB * 4, 256)), dim3(256), 0, off, NB,

@@ -48,6 +48,32 @@ def test_msm_batch_matches_per_call(ctx, shifted):
     srs.close()
 
 
+@pytest.mark.parametrize("nj", [2, 3, 4, 5])
+def test_few_shared_sets_against_oracle(ctx, nj):
+    """Two to four shared bucket sets are reduced by the block tree, a set per block range
+    and its channels by set; five by segments of four buckets. Each commit against the C
+    restatement of msm_best and against its single commit."""
+    tau = C.fr_from_seed(4, 0, 1)[0]
+    n_srs = 1 << 16
+    bases = C.srs_bases(tau, n_srs)
+    srs = ctx.srs_upload(bases)
+    rng = random.Random(nj)
+    jobs, host = [], []
+    for i in range(nj):
+        n = rng.randrange(1 << 15, (1 << 15) + 6000)   # shared sets once jobs average 2^15 terms
+        off = rng.randrange(0, n_srs - n + 1)
+        sc = C.fr_from_seed(200 + 10 * nj + i, 1, n)
+        host.append((sc, off))
+        jobs.append((to_dev(sc), n, off))
+    outs = srs.commit_batch(jobs)
+    assert ctx.msm_last_plan()["bucket_sets"] == nj
+    for (sc, off), (d, n, _), out in zip(host, jobs, outs):
+        expect = aff(C.msm_best(sc, bases[off:off + n]))
+        assert aff(out) == expect, (nj, n, off)
+        assert aff(srs.commit(d, n, offset=off)) == expect
+    srs.close()
+
+
 def test_msm_batch_rejects_out_of_range_job(ctx):
     tau = C.fr_from_seed(3, 0, 1)[0]
     srs = ctx.srs_upload(C.srs_bases(tau, 100))
