@@ -1008,8 +1008,13 @@ MsmPlan msm_plan(size_t n, int per_window, int num_cus, int c_override) {
   size_t M = (size_t)p.W * n;
   const uint32_t tpc = per_window ? (t.tpc_window ? t.tpc_window : 512) : (t.tpc_shared ? t.tpc_shared : 512);
   size_t target_threads = (size_t)std::max(num_cus, 1) * tpc;
-  size_t K = M / target_threads;
-  p.K = (uint32_t)std::max<size_t>(2, std::min<size_t>(256, K));
+  // Chunks of K digits in whole rounds of target_threads: the fewest rounds that keep
+  // K <= 256, then K rounded up.  Rounded down, K left a few more chunks than threads
+  // and a second, nearly empty round (2^18 - 1 terms: K = 31, 135 300 chunks for
+  // 131 072 threads, accumulation 0.67 -> 0.87 ms).
+  const size_t rounds = std::max<size_t>(1, (M + 256 * target_threads - 1) / (256 * target_threads));
+  const size_t K = (M + rounds * target_threads - 1) / (rounds * target_threads);
+  p.K = (uint32_t)std::max<size_t>(2, K);
   return p;
 }
 
