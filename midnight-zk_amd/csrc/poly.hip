@@ -73,45 +73,125 @@ __global__ void k_vec_scalar(int op, const uint32_t* __restrict__ a, FrArg s, ui
   store<FrP>(out + 8 * i, r);
 }
 
-static constexpr int INV_CHUNK = 32;
+static constexpr int SCAN_T_INV = 256;   // threads per batch-inversion workgroup
 
-// Batch inversion as a 32-ary product tree: the up pass turns each chunk of 32 values
-// into its product (zeros skipped) and the exclusive prefix products inside the chunk;
-// the inverse of the root is taken on the host (a Fermat inversion on one GPU lane is
-// ~400 dependent products); the down pass turns each chunk's inverse product back
-// into the inverses of its values (Montgomery's trick, walked from the chunk's top).
-__global__ void k_binv_up(const uint32_t* __restrict__ x, size_t m, uint32_t* __restrict__ pre,
-                          uint32_t* __restrict__ prod) {
-  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t lo = c * INV_CHUNK;
-  if (lo >= m) return;
-  const size_t hi = lo + INV_CHUNK < m ? lo + INV_CHUNK : m;
-  fr_t acc = fr_t::one();
-  for (size_t i = lo; i < hi; i++) {
-    store<FrP>(pre + 8 * i, acc);
-    const fr_t v = load<FrP>(x + 8 * i);
-    if (!v.is_zero()) acc = mul(acc, v);
+// Batch inversion (ff BatchInvert): zeros stay zero, every other value is replaced by its
+// inverse.  The inverse of x_i is T^-1 times the product of all the other non-zero values
+// (T the product of all of them), and any grouping of "the others" will do: a workgroup
+// tile of INV_TILE values, thread t holding the tile's values t, t + 256, ... (loads
+// coalesce).  The others of value (t, k) are: the other tiles (one product per tile from
+// the tile totals' prefix and suffix), the tile's other threads (the threads' products'
+// exclusive prefix and suffix in the workgroup) and the thread's other values (Montgomery's
+// trick over its INV_E values).  Three launches and one inversion of T on the host (a
+// Fermat inversion on one GPU lane is ~400 dependent products); no dependent chain is
+// longer than a few tens of products.
+static constexpr int INV_E = 4;   // values per thread (at 8 the two arrays below leave registers for scratch)
+static constexpr uint32_t INV_TILE = SCAN_T_INV * INV_E;          // values per tile
+
+// Workgroup-wide exclusive prefix and suffix products of one value per thread (zeros
+// are the caller's to replace): pre = product over threads < t, suf = over threads > t.
+__device__ void block_pre_suf(fr_t v, uint32_t* lp, uint32_t* ls, fr_t& pre, fr_t& suf, fr_t& total) {
+  const int t = threadIdx.x;
+  fr_t a = v, b = v;   // inclusive prefix, inclusive suffix
+  store<FrP>(lp + 8 * t, a);
+  store<FrP>(ls + 8 * t, b);
+  __syncthreads();
+  for (int d = 1; d < SCAN_T_INV; d <<= 1) {
+    const fr_t ya = t >= d ? load<FrP>(lp + 8 * (t - d)) : fr_t::one();
+    const fr_t yb = t + d < SCAN_T_INV ? load<FrP>(ls + 8 * (t + d)) : fr_t::one();
+    __syncthreads();
+    a = mul(a, ya);
+    b = mul(b, yb);
+    store<FrP>(lp + 8 * t, a);
+    store<FrP>(ls + 8 * t, b);
+    __syncthreads();
   }
-  store<FrP>(prod + 8 * c, acc);
+  pre = t > 0 ? load<FrP>(lp + 8 * (t - 1)) : fr_t::one();
+  suf = t + 1 < SCAN_T_INV ? load<FrP>(ls + 8 * (t + 1)) : fr_t::one();
+  total = load<FrP>(lp + 8 * (SCAN_T_INV - 1));
+  __syncthreads();
 }
 
-// inv[i] = x[i]^-1 (0 for 0) from the chunk inverses cinv (root_inv when cinv is null:
-// the level holds one chunk).  inv may be x: each value is read before it is replaced.
-__global__ void k_binv_down(const uint32_t* x, size_t m, const uint32_t* __restrict__ pre,
-                            const uint32_t* __restrict__ cinv, FrArg root_inv, uint32_t* inv) {
-  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t lo = c * INV_CHUNK;
-  if (lo >= m) return;
-  const size_t hi = lo + INV_CHUNK < m ? lo + INV_CHUNK : m;
-  fr_t acc = cinv ? load<FrP>(cinv + 8 * c) : arg_fr(root_inv);
-  for (size_t i = hi; i-- > lo;) {
-    const fr_t v = load<FrP>(x + 8 * i);
-    if (v.is_zero()) {
-      store<FrP>(inv + 8 * i, fr_t::zero());
-      continue;
+// Value k of thread t in tile `tile`: index tile INV_TILE + k 256 + t.
+__device__ __forceinline__ size_t inv_index(uint32_t tile, int k) {
+  return (size_t)tile * INV_TILE + (size_t)k * SCAN_T_INV + threadIdx.x;
+}
+
+// Pass 1: the product of each tile's non-zero values.
+__global__ void __launch_bounds__(SCAN_T_INV) k_binv_tiles(const uint32_t* __restrict__ x, size_t n,
+                                                            uint32_t* __restrict__ tot) {
+  __shared__ uint32_t lds[8 * SCAN_T_INV];
+  fr_t p = fr_t::one();
+#pragma unroll
+  for (int k = 0; k < INV_E; k++) {
+    const size_t i = inv_index(blockIdx.x, k);
+    if (i < n) {
+      const fr_t v = load<FrP>(x + 8 * i);
+      if (!v.is_zero()) p = mul(p, v);
     }
-    store<FrP>(inv + 8 * i, mul(acc, load<FrP>(pre + 8 * i)));
-    acc = mul(acc, v);
+  }
+  store<FrP>(lds + 8 * threadIdx.x, p);
+  __syncthreads();
+  for (int d = SCAN_T_INV / 2; d >= 1; d >>= 1) {
+    if ((int)threadIdx.x < d) {
+      p = mul(p, load<FrP>(lds + 8 * (threadIdx.x + d)));
+      store<FrP>(lds + 8 * threadIdx.x, p);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) store<FrP>(tot + 8 * blockIdx.x, p);
+}
+
+// Pass 2, one workgroup: others[j] = product of the totals of every tile but j, and
+// others[m] = T, the product of all m totals.  Thread t holds tiles t, t + 256, ...; its
+// tiles' exclusive products are walked in others[] (forward products, then the backward
+// trick), the other threads' by block_pre_suf.
+__global__ void __launch_bounds__(SCAN_T_INV) k_binv_totals(const uint32_t* __restrict__ tot, size_t m,
+                                                             uint32_t* __restrict__ others) {
+  __shared__ uint32_t lp[8 * SCAN_T_INV], ls[8 * SCAN_T_INV];
+  fr_t p = fr_t::one();
+  for (size_t j = threadIdx.x; j < m; j += SCAN_T_INV) {
+    store<FrP>(others + 8 * j, p);   // product of this thread's earlier tiles
+    p = mul(p, load<FrP>(tot + 8 * j));
+  }
+  fr_t pre, suf, total;
+  block_pre_suf(p, lp, ls, pre, suf, total);
+  if (threadIdx.x < m) {
+    fr_t b = mul(pre, suf);   // the other threads' tiles
+    const size_t last = threadIdx.x + ((m - 1 - threadIdx.x) / SCAN_T_INV) * SCAN_T_INV;
+    for (size_t j = last;; j -= SCAN_T_INV) {   // this thread's tiles, last to first
+      store<FrP>(others + 8 * j, mul(load<FrP>(others + 8 * j), b));
+      b = mul(b, load<FrP>(tot + 8 * j));
+      if (j < SCAN_T_INV) break;
+    }
+  }
+  if (threadIdx.x == 0) store<FrP>(others + 8 * m, total);
+}
+
+// Pass 3: inv[i] = x[i]^-1 (0 for 0) with tinv = T^-1.  inv may be x: a thread reads its
+// values before it writes them, and no other thread touches them.
+__global__ void __launch_bounds__(SCAN_T_INV) k_binv_apply(const uint32_t* x, size_t n,
+                                                            const uint32_t* __restrict__ others, FrArg tinv,
+                                                            uint32_t* inv) {
+  __shared__ uint32_t lp[8 * SCAN_T_INV], ls[8 * SCAN_T_INV];
+  fr_t v[INV_E], fw[INV_E];
+  fr_t p = fr_t::one();
+#pragma unroll
+  for (int k = 0; k < INV_E; k++) {
+    const size_t i = inv_index(blockIdx.x, k);
+    v[k] = i < n ? load<FrP>(x + 8 * i) : fr_t::zero();
+    fw[k] = p;   // product of this thread's values before k
+    if (!v[k].is_zero()) p = mul(p, v[k]);
+  }
+  fr_t pre, suf, total;
+  block_pre_suf(p, lp, ls, pre, suf, total);
+  fr_t b = mul(mul(arg_fr(tinv), load<FrP>(others + 8 * blockIdx.x)), mul(pre, suf));
+#pragma unroll
+  for (int k = INV_E - 1; k >= 0; k--) {
+    const size_t i = inv_index(blockIdx.x, k);
+    const bool z = v[k].is_zero();
+    if (i < n) store<FrP>(inv + 8 * i, z ? fr_t::zero() : mul(fw[k], b));
+    if (!z) b = mul(b, v[k]);
   }
 }
 
@@ -649,43 +729,21 @@ int mzk_fr_batch_invert(mzk_ctx* c, void* a, size_t n) {
   MZK_REQUIRE(a != nullptr, "mzk_fr_batch_invert: null data");
   MZK_TRY(vec_ready(c));
   std::lock_guard<std::mutex> g(c->mu);
-  // levels: sizes m_0 = n, m_(l+1) = ceil(m_l / 32), down to one value
-  std::vector<size_t> sizes = {n};
-  while (sizes.back() > 1) sizes.push_back((sizes.back() + INV_CHUNK - 1) / INV_CHUNK);
-  size_t total = 0;
-  for (size_t l = 0; l < sizes.size(); l++) total += sizes[l];
-  // scratch: per level a prefix buffer (m_l) and the level's values (m_l, from level 1)
-  MZK_TRY(c->scratch[SCR_MSM_D].reserve(2 * total * 32 + 256));
-  uint32_t* base = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
-  std::vector<uint32_t*> val(sizes.size()), pre(sizes.size());
-  size_t o = 0;
-  for (size_t l = 0; l < sizes.size(); l++) {
-    val[l] = l == 0 ? (uint32_t*)a : base + 8 * o;
-    o += sizes[l];
-    pre[l] = base + 8 * o;
-    o += sizes[l];
-  }
-  for (size_t l = 0; l + 1 < sizes.size(); l++)
-    MZK_LAUNCH(c, "k_binv_up", k_binv_up, dim3(grid_for(sizes[l + 1], 64)), dim3(64), 0, val[l], sizes[l], pre[l],
-               val[l + 1]);
-  // the root (one value) is inverted on the host and enters the top down pass by value
+  const size_t m = (n + INV_TILE - 1) / INV_TILE;   // tiles
+  MZK_REQUIRE(m < (1ull << 31), "mzk_fr_batch_invert: n too large");
+  // scratch: tile totals (m), the other tiles' products (m) and T
+  MZK_TRY(c->scratch[SCR_MSM_D].reserve((2 * m + 1) * 32 + 256));
+  uint32_t* tot = (uint32_t*)c->scratch[SCR_MSM_D].ptr;
+  uint32_t* others = tot + 8 * m;
+  MZK_LAUNCH(c, "k_binv_tiles", k_binv_tiles, dim3((unsigned)m), dim3(SCAN_T_INV), 0, (const uint32_t*)a, n, tot);
+  MZK_LAUNCH(c, "k_binv_totals", k_binv_totals, dim3(1), dim3(SCAN_T_INV), 0, tot, m, others);
   host::Fr root;
-  MZK_HIP(hipMemcpyAsync(root.l, val.back(), 32, hipMemcpyDeviceToHost, c->stream));
+  MZK_HIP(hipMemcpyAsync(root.l, others + 8 * m, 32, hipMemcpyDeviceToHost, c->stream));
   MZK_HIP(hipStreamSynchronize(c->stream));
-  const bool zero = (root.l[0] | root.l[1] | root.l[2] | root.l[3]) == 0;
+  const bool zero = (root.l[0] | root.l[1] | root.l[2] | root.l[3]) == 0;   // not reached: zeros are skipped
   const host::Fr rinv = zero ? host::fr_zero() : host::fr_inv(root);
-  if (sizes.size() == 1) {   // n == 1: the value itself
-    MZK_HIP(hipMemcpyAsync(a, rinv.l, 32, hipMemcpyHostToDevice, c->stream));
-    MZK_HIP(hipStreamSynchronize(c->stream));
-    return MZK_OK;
-  }
-  // level l's chunk inverses are the inverted values of level l + 1, written in place
-  const uint32_t* up_inv = nullptr;
-  for (size_t l = sizes.size() - 1; l-- > 0;) {
-    MZK_LAUNCH(c, "k_binv_down", k_binv_down, dim3(grid_for(sizes[l + 1], 64)), dim3(64), 0, val[l], sizes[l],
-               pre[l], up_inv, to_arg(rinv), val[l]);
-    up_inv = val[l];
-  }
+  MZK_LAUNCH(c, "k_binv_apply", k_binv_apply, dim3((unsigned)m), dim3(SCAN_T_INV), 0, (const uint32_t*)a, n,
+             others, to_arg(rinv), (uint32_t*)a);
   MZK_HIP(hipGetLastError());
   return MZK_OK;
 }

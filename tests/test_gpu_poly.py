@@ -57,7 +57,10 @@ def test_vec_ops_small_values_by_integers(be):
     assert C.fr_rows_to_ints(host(be, be.mul(da, db))) == [x * y % R for x, y in zip(a, b)]
 
 
-@pytest.mark.parametrize("n", [1, 5, 32, 33, 1000, 65536 + 7, (1 << 20) + 1])
+# tiles of 1024 values (256 threads x 4): one partial tile, tile edges, and more tiles than
+# the totals' workgroup has threads (256 x 1024 = 262144 values)
+@pytest.mark.parametrize("n", [1, 5, 32, 33, 1000, 1023, 1024, 1025, 65536 + 7, 262144 + 1024 * 3 + 1,
+                               (1 << 20) + 1])
 def test_batch_invert(be, n):
     a = C.fr_from_seed(4, n, n)
     a[::7] = 0                       # zeros stay zero (BatchInvert skips them)
@@ -66,6 +69,13 @@ def test_batch_invert(be, n):
     if n <= 1000:
         ints = C.fr_rows_to_ints(a)
         assert C.fr_rows_to_ints(got) == [pow(v, -1, O.R_MOD) if v else 0 for v in ints]
+
+
+def test_batch_invert_all_zero_and_single(be):
+    z = np.zeros((3000, 4), dtype=np.uint64)
+    assert not host(be, be.batch_invert(to_dev(z))).any()
+    one = C.fr_from_seed(9, 1, 1)
+    assert C.fr_rows_to_ints(host(be, be.batch_invert(to_dev(one)))) == [pow(C.fr_rows_to_ints(one)[0], -1, O.R_MOD)]
 
 
 @pytest.mark.parametrize("n", [1, 2, 255, 256, 4095, 4096, 4097, 8192 * 3 + 5, (1 << 18) + 1, (1 << 20)])
