@@ -108,8 +108,9 @@ MZK_HD bool f28_is_literal_zero(const f28& a) {
 }
 
 // Montgomery product a b 2^-392 mod p by product scanning (Koc, Acar, Kaliski 1996):
-// column k accumulates a_j b_(k-j) in one 64-bit word and m_j p_(k-j) in a second one
-// (two independent v_mad_u64_u32 chains), merged before m_k is taken.
+// column k accumulates a_j b_(k-j) and m_j p_(k-j) in one 64-bit word, a single chain of
+// v_mad_u64_u32 with no merge (the column's final sum, which bounds every partial one,
+// is what the header's bounds keep below 2^64).
 MZK_HD f28 f28_mul(const f28& a, const f28& b) {
   using namespace f28c;
   uint32_t m[N];
@@ -117,27 +118,23 @@ MZK_HD f28 f28_mul(const f28& a, const f28& b) {
   uint64_t acc = 0;
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    uint64_t red = 0;
 #pragma unroll
     for (int j = 0; j < k; j++) {
       acc += (uint64_t)a.l[j] * b.l[k - j];
-      red += (uint64_t)m[j] * P(k - j);
+      acc += (uint64_t)m[j] * P(k - j);
     }
     acc += (uint64_t)a.l[k] * b.l[0];
-    acc += red;
     m[k] = ((uint32_t)acc * PINV) & M28;
     acc += (uint64_t)m[k] * P(0);
     acc >>= 28;
   }
 #pragma unroll
   for (int k = N; k < 2 * N - 1; k++) {
-    uint64_t red = 0;
 #pragma unroll
     for (int j = k - N + 1; j < N; j++) {
       acc += (uint64_t)a.l[j] * b.l[k - j];
-      red += (uint64_t)m[j] * P(k - j);
+      acc += (uint64_t)m[j] * P(k - j);
     }
-    acc += red;
     r.l[k - N] = (uint32_t)acc & M28;
     acc >>= 28;
   }
@@ -146,7 +143,7 @@ MZK_HD f28 f28_mul(const f28& a, const f28& b) {
 }
 
 // (a b + c d) 2^-392 mod p with one reduction: the two products share the column
-// accumulators.  Needs a b + c d < 2520 p^2 and column sums below 2^64.
+// accumulator.  Needs a b + c d < 2520 p^2 and column sums below 2^64.
 MZK_HD f28 f28_mul_sum(const f28& a, const f28& b, const f28& c, const f28& d) {
   using namespace f28c;
   uint32_t m[N];
@@ -154,30 +151,26 @@ MZK_HD f28 f28_mul_sum(const f28& a, const f28& b, const f28& c, const f28& d) {
   uint64_t acc = 0;
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    uint64_t red = 0;
 #pragma unroll
     for (int j = 0; j < k; j++) {
       acc += (uint64_t)a.l[j] * b.l[k - j];
-      red += (uint64_t)c.l[j] * d.l[k - j];
-      red += (uint64_t)m[j] * P(k - j);
+      acc += (uint64_t)c.l[j] * d.l[k - j];
+      acc += (uint64_t)m[j] * P(k - j);
     }
     acc += (uint64_t)a.l[k] * b.l[0];
-    red += (uint64_t)c.l[k] * d.l[0];
-    acc += red;
+    acc += (uint64_t)c.l[k] * d.l[0];
     m[k] = ((uint32_t)acc * PINV) & M28;
     acc += (uint64_t)m[k] * P(0);
     acc >>= 28;
   }
 #pragma unroll
   for (int k = N; k < 2 * N - 1; k++) {
-    uint64_t red = 0;
 #pragma unroll
     for (int j = k - N + 1; j < N; j++) {
       acc += (uint64_t)a.l[j] * b.l[k - j];
-      red += (uint64_t)c.l[j] * d.l[k - j];
-      red += (uint64_t)m[j] * P(k - j);
+      acc += (uint64_t)c.l[j] * d.l[k - j];
+      acc += (uint64_t)m[j] * P(k - j);
     }
-    acc += red;
     r.l[k - N] = (uint32_t)acc & M28;
     acc >>= 28;
   }
@@ -185,37 +178,34 @@ MZK_HD f28 f28_mul_sum(const f28& a, const f28& b, const f28& c, const f28& d) {
   return r;
 }
 
-// a^2: the cross products a_i a_j (i < j) once, doubled, plus the squares; the
-// reduction terms in a second accumulator as in f28_mul
+// a^2: each cross product a_i a_j (i < j) once, as a_i (2 a_j), plus the squares, and the
+// reduction terms, in one accumulator chain as in f28_mul (the column sums are those of
+// f28_mul(a, a))
 MZK_HD f28 f28_sqr(const f28& a) {
   using namespace f28c;
-  uint32_t m[N];
+  uint32_t m[N], d[N];
   f28 r;
+#pragma unroll
+  for (int i = 0; i < N; i++) d[i] = a.l[i] << 1;
   uint64_t acc = 0;
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    uint64_t cross = 0, red = 0;
 #pragma unroll
-    for (int j = 0; 2 * j < k; j++) cross += (uint64_t)a.l[j] * a.l[k - j];
+    for (int j = 0; 2 * j < k; j++) acc += (uint64_t)a.l[j] * d[k - j];
 #pragma unroll
-    for (int j = 0; j < k; j++) red += (uint64_t)m[j] * P(k - j);
-    acc += cross << 1;
+    for (int j = 0; j < k; j++) acc += (uint64_t)m[j] * P(k - j);
     if ((k & 1) == 0) acc += (uint64_t)a.l[k / 2] * a.l[k / 2];
-    acc += red;
     m[k] = ((uint32_t)acc * PINV) & M28;
     acc += (uint64_t)m[k] * P(0);
     acc >>= 28;
   }
 #pragma unroll
   for (int k = N; k < 2 * N - 1; k++) {
-    uint64_t cross = 0, red = 0;
 #pragma unroll
-    for (int j = k - N + 1; 2 * j < k; j++) cross += (uint64_t)a.l[j] * a.l[k - j];
+    for (int j = k - N + 1; 2 * j < k; j++) acc += (uint64_t)a.l[j] * d[k - j];
 #pragma unroll
-    for (int j = k - N + 1; j < N; j++) red += (uint64_t)m[j] * P(k - j);
-    acc += cross << 1;
+    for (int j = k - N + 1; j < N; j++) acc += (uint64_t)m[j] * P(k - j);
     if ((k & 1) == 0) acc += (uint64_t)a.l[k / 2] * a.l[k / 2];
-    acc += red;
     r.l[k - N] = (uint32_t)acc & M28;
     acc >>= 28;
   }
