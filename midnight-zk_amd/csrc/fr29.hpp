@@ -56,30 +56,26 @@ MZK_HD29 fr29 fr29_mul(const fr29& a, const fr29& b) {
   using namespace r29c;
   uint32_t m[N];
   fr29 o;
-  uint64_t acc = 0;
+  uint64_t acc = 0;   // one chain per column: products and reduction terms alike
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    uint64_t red = 0;
 #pragma unroll
     for (int j = 0; j < k; j++) {
       acc += (uint64_t)a.l[j] * b.l[k - j];
-      red += (uint64_t)m[j] * R(k - j);
+      acc += (uint64_t)m[j] * R(k - j);
     }
     acc += (uint64_t)a.l[k] * b.l[0];
-    acc += red;
     m[k] = (0u - (uint32_t)acc) & M29;   // -acc / r mod 2^29 (r = 1 mod 2^29)
     acc += m[k];
     acc >>= 29;
   }
 #pragma unroll
   for (int k = N; k < 2 * N - 1; k++) {
-    uint64_t red = 0;
 #pragma unroll
     for (int j = k - N + 1; j < N; j++) {
       acc += (uint64_t)a.l[j] * b.l[k - j];
-      red += (uint64_t)m[j] * R(k - j);
+      acc += (uint64_t)m[j] * R(k - j);
     }
-    acc += red;
     o.l[k - N] = (uint32_t)acc & M29;
     acc >>= 29;
   }
