@@ -19,6 +19,7 @@
 
 #include "blake2b.hpp"
 #include "ctx.hpp"
+#include "field.hpp"
 #include "host_ec.hpp"
 #include "srs.hpp"
 
@@ -209,6 +210,56 @@ __global__ void k_perm_z_finish(uint32_t* __restrict__ z, const uint32_t* __rest
     d4[1] = make_uint4(one.v[4], one.v[5], one.v[6], one.v[7]);
   }
   if (t < bf) copy_elem(z + 8 * (n - bf + t), blind + 8 * t);
+}
+
+// The advice columns with their blinding rows, one launch for all columns of a proof:
+// row i < u of column j from the caller's column, rows u..n-1 from the draws
+// (draws[j (bf + 1) + i - u]).  Ten copies enqueued one by one left the device idle
+// ~0.15 ms at the start of every proof.
+struct AdviceRows {
+  const uint32_t* src[NADV];
+  uint32_t* dst[NADV];
+};
+__global__ void k_advice_rows(AdviceRows a, const uint32_t* __restrict__ draws, uint64_t n, uint64_t u,
+                              uint32_t bf) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t j = blockIdx.y;
+  const uint32_t* s = i < u ? a.src[j] + 8 * i : draws + 8 * ((uint64_t)j * (bf + 1) + (i - u));
+  copy_elem(a.dst[j] + 8 * i, s);
+}
+
+// The quotient's limbs (plonk/prover.rs:743-785): limb i holds h's coefficients
+// i (n - 1) .. i (n - 1) + n - 2, then blinding value i (0 for the last limb); the first
+// coefficient of limb i > 0 less blinding value i - 1.  One launch for all limbs of a
+// proof, in place of eleven copies, fills and one-element subtractions.
+struct QuotientLimbs {
+  uint32_t* dst[QDEG];
+};
+__global__ void k_quotient_limbs(const uint32_t* __restrict__ h, QuotientLimbs q, const uint32_t* __restrict__ blind,
+                                 uint64_t n) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const uint32_t i = blockIdx.y;
+  fr_t v;
+  if (r + 1 < n) {
+    v = load<FrP>(h + 8 * ((uint64_t)i * (n - 1) + r));
+    if (i > 0 && r == 0) v = sub(v, load<FrP>(blind + 8 * (i - 1)));
+  } else {
+    v = i + 1 < QDEG ? load<FrP>(blind + 8 * i) : fr_t::zero();
+  }
+  store<FrP>(q.dst[i] + 8 * r, v);
+}
+
+// z of set s > 0 continues from the earlier sets: z[i] *= last[0] ... last[s - 1], the
+// factor read on the device (no host round trip for the last values).
+__global__ void k_perm_z_continue(uint32_t* __restrict__ z, uint64_t n, const uint32_t* __restrict__ last,
+                                  uint32_t s) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  fr_t f = load<FrP>(last);
+  for (uint32_t t = 1; t < s; t++) f = mul(f, load<FrP>(last + 8 * t));
+  store<FrP>(z + 8 * i, mul(load<FrP>(z + 8 * i), f));
 }
 
 struct ProofBufs {
@@ -463,11 +514,15 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
   for (size_t p = 0; p < np; p++) {
     T[p].common_fr(fr_of(key->transcript_repr));
     PV_TRY(mzk_fr_from_seed_dev(c, rng_seeds[p], RNG_STREAM, 0, NDRAWS, B[p].draws));
+    AdviceRows ar;
     for (int j = 0; j < NADV; j++) {
-      MZK_HIP(hipMemcpyAsync(B[p].adv[j], advice[p * NADV + j], u * 32, hipMemcpyDeviceToDevice, st));
-      MZK_HIP(hipMemcpyAsync(B[p].adv[j] + 8 * u, B[p].draws + 8 * (size_t)(j * (BF + 1)), (BF + 1) * 32,
-                             hipMemcpyDeviceToDevice, st));
+      ar.src[j] = (const uint32_t*)advice[p * NADV + j];
+      ar.dst[j] = B[p].adv[j];
     }
+    std::lock_guard<std::mutex> g(c->mu);
+    hipLaunchKernelGGL(k_advice_rows, dim3((unsigned)((n + 255) / 256), NADV), dim3(256), 0, st, ar, B[p].draws,
+                       (uint64_t)n, (uint64_t)u, (uint32_t)BF);
+    MZK_HIP(hipGetLastError());
   }
   // the advice columns' transforms (to coefficients, to the extended coset) depend on the
   // advice alone: they run on the auxiliary stream from the end of the advice commitments'
@@ -562,20 +617,14 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
                            (uint64_t)u, one_w, lasts + 8 * js);
         MZK_HIP(hipGetLastError());
       }
-    std::vector<Fr> last(J);
-    {
-      std::lock_guard<std::mutex> g(c->mu);
-      MZK_HIP(hipMemcpyAsync(last.data(), lasts, J * 32, hipMemcpyDeviceToHost, st));
-      MZK_HIP(hipStreamSynchronize(st));
-    }
     // set s > 0 continues from the earlier sets' last values: scale by their product
-    for (size_t p = 0; p < np; p++) {
-      Fr corr = one;
-      for (int s = 0; s < NSETS; s++) {
-        if (s > 0) PV_TRY(vscal(c, 0, B[p].z[s], corr, B[p].z[s], n));
-        corr = host::fr_mul(corr, last[p * NSETS + s]);
+    for (size_t p = 0; p < np; p++)
+      for (int s = 1; s < NSETS; s++) {
+        std::lock_guard<std::mutex> g(c->mu);
+        hipLaunchKernelGGL(k_perm_z_continue, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, B[p].z[s],
+                           (uint64_t)n, lasts + 8 * p * NSETS, (uint32_t)s);
+        MZK_HIP(hipGetLastError());
       }
-    }
   }
   {   // z's transforms overlap its commitment (few non-zero deltas: mostly the MSM's sort
       // and reduction phases)
@@ -677,14 +726,12 @@ int mzk_plonk_prove(mzk_ctx* c, const mzk_plonk_key* key, size_t np, const void*
   for (size_t p = 0; p < np; p++) {
     ProofBufs& b = B[p];
     const uint32_t* blind = b.draws + 8 * (size_t)(NADV * (BF + 1) + NSETS * BF);
-    for (int i = 0; i < QDEG; i++) {
-      MZK_HIP(hipMemcpyAsync(b.limbs[i], b.h + 8 * (size_t)i * (n - 1), (n - 1) * 32, hipMemcpyDeviceToDevice, st));
-      if (i < QDEG - 1)
-        MZK_HIP(hipMemcpyAsync(b.limbs[i] + 8 * (n - 1), blind + 8 * i, 32, hipMemcpyDeviceToDevice, st));
-      else
-        MZK_HIP(hipMemsetAsync(b.limbs[i] + 8 * (n - 1), 0, 32, st));
-      if (i > 0) PV_TRY(mzk_fr_vec_op(c, 1, b.limbs[i], blind + 8 * (i - 1), b.limbs[i], 1));
-    }
+    QuotientLimbs q;
+    for (int i = 0; i < QDEG; i++) q.dst[i] = b.limbs[i];
+    std::lock_guard<std::mutex> g(c->mu);
+    hipLaunchKernelGGL(k_quotient_limbs, dim3((unsigned)((n + 255) / 256), QDEG), dim3(256), 0, st, b.h, q, blind,
+                       (uint64_t)n);
+    MZK_HIP(hipGetLastError());
   }
   {
     std::vector<std::pair<const void*, size_t>> jobs;
