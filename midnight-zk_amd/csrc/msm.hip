@@ -302,6 +302,7 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __r
 }
 
 static constexpr int BINS_UNROLL = 8;
+static constexpr size_t BINS_LDS_CU = 160u << 10;   // LDS per CU
 
 // 4. per top-bit bin of a sort group (workgroup g D1 + bin): counting sort on the low
 //    bits; writes bucket offsets off[] (off[NB] = M) and the references in bucket order.
@@ -1024,7 +1025,9 @@ static int scatter_lds_attr() {
   static int rc = MZK_OK;
   std::call_once(once, [] {
     const int bytes = (int)(scatter_lds_words(SCAT_MAX_D1, false) * 4);
-    if (hipFuncSetAttribute((const void*)k_sort_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
+    if (hipFuncSetAttribute((const void*)k_sort_bins, hipFuncAttributeMaxDynamicSharedMemorySize, 128 << 10) !=
+            hipSuccess ||
+        hipFuncSetAttribute((const void*)k_sort_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
             hipSuccess ||
         hipFuncSetAttribute((const void*)k_sort_scatter<true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
             hipSuccess) {
@@ -1374,6 +1377,15 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   MZK_REQUIRE(sa.D1 <= SCAT_MAX_D1, "msm: %u top-level sort bins exceed %u (too many bucket sets)", sa.D1,
               SCAT_MAX_D1);
   const size_t scat_lds = scatter_lds_words(sa.D1, sa.pack != 0) * 4;
+  // k_sort_bins places each reference with a 4-byte store at its bucket's cursor: a
+  // workgroup keeps 2^l2 partly written lines open, and once an XCD's resident bins hold
+  // more than its L2 can merge, the partial lines go back to HBM (fifteen 2^18 commits,
+  // 1024 bins of 512 buckets, 4 bins per CU: 0.675 ms; 1 per CU: 0.42 ms).  Extra LDS caps
+  // the resident bins at ~512 / 2^l2 per CU (one XCD's 32 CUs then hold ~2 MB of open lines).
+  const uint32_t bins_per_cu = std::max<uint32_t>(1, 512u >> sa.l2);
+  const size_t bins_lds = bins_per_cu < 8 && ngroups * sa.D1 > (size_t)c->num_cus * bins_per_cu
+                              ? std::max<size_t>((1u << sa.l2) * 4, BINS_LDS_CU / (2 * bins_per_cu + 1) * 2)
+                              : (1u << sa.l2) * 4;
   MZK_TRY(scatter_lds_attr());
   // the device sequence (sort, accumulation, reduction) as one cached graph: every value
   // its launches take is in the key; the descriptors above are copied before it runs
@@ -1393,7 +1405,7 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
     else
       MZK_LAUNCH(c, "k_sort_scatter", k_sort_scatter<false>, dim3(sa.G), dim3(SORT_BLOCK), scat_lds, canon, sa, hoff,
                  tmp_key, tmp_ref);
-    MZK_LAUNCH(c, "k_sort_bins", k_sort_bins, dim3((unsigned)(ngroups * sa.D1)), dim3(SORT_BLOCK), (1u << sa.l2) * 4,
+    MZK_LAUNCH(c, "k_sort_bins", k_sort_bins, dim3((unsigned)(ngroups * sa.D1)), dim3(SORT_BLOCK), bins_lds,
                hoff, sa, NB,
                tmp_key, tmp_ref, refs, off);
     if (records)
