@@ -576,6 +576,70 @@ __global__ void __launch_bounds__(64) k_msm_accum(const uint32_t* __restrict__ r
   store_x28_raw(partials + MZK_X28_WORDS * (size_t)(k + cb), acc);
 }
 
+// A bucket's chunk partials are the slots k + b for its chunks k0..k1 (a chunk's thread
+// writes the partial of each bucket it touches there).  Skewed scalars (a boolean or
+// small-valued column, repeated values) put most digits in a few buckets, whose partials
+// a reduction kernel would merge as one dependent chain of additions (a boolean column
+// of 2^18 terms: ~4000 partials, ~60 ms).  A bucket spanning more than LONG_CHUNKS chunks
+// is therefore merged by k_long_merge first, by a whole wave, into its first slot, and the
+// reductions read only that slot (bucket_chunks).
+static constexpr uint32_t LONG_CHUNKS = 64;
+
+__device__ __forceinline__ bool bucket_chunks(const uint32_t* __restrict__ off, uint32_t b, uint32_t K, uint32_t& k0,
+                                              uint32_t& k1) {
+  const uint32_t lo = off[b], hi = off[b + 1];
+  if (hi <= lo) return false;
+  k0 = lo / K;
+  k1 = (hi - 1) / K;
+  if (k1 - k0 >= LONG_CHUNKS) k1 = k0;   // merged by k_long_merge
+  return true;
+}
+
+// A butterfly step of a wave-wide point sum: lane l gets lane (l ^ d)'s point.
+__device__ __forceinline__ g1x28 shfl_xor_x28(const g1x28& a, int d) {
+  g1x28 r;
+#pragma unroll
+  for (int i = 0; i < f28c::N; i++) {
+    r.X.l[i] = (uint32_t)__shfl_xor((int)a.X.l[i], d);
+    r.Y.l[i] = (uint32_t)__shfl_xor((int)a.Y.l[i], d);
+    r.ZZ.l[i] = (uint32_t)__shfl_xor((int)a.ZZ.l[i], d);
+    r.ZZZ.l[i] = (uint32_t)__shfl_xor((int)a.ZZZ.l[i], d);
+  }
+  return r;
+}
+
+// A workgroup of four waves per 64 buckets: every wave reads the 64 buckets' spans (lane
+// i: bucket 64 blockIdx + i) and takes every fourth long bucket; its lanes sum the
+// bucket's partials strided by 64 (each lane's first slot is its own), then a butterfly
+// over the lanes by shuffles (no LDS, no barrier: each wave runs alone) leaves the
+// bucket's sum in its first slot.  With no long bucket a workgroup reads 65 offsets and
+// leaves.
+__global__ void __launch_bounds__(256) k_long_merge(const uint32_t* __restrict__ off, uint32_t NB, uint32_t K,
+                                                    uint32_t* __restrict__ partials) {
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+  const uint32_t b_lane = blockIdx.x * 64 + lane;
+  uint32_t k0 = 0, n = 0;
+  if (b_lane < NB) {
+    const uint32_t lo = off[b_lane], hi = off[b_lane + 1];
+    if (hi > lo) {
+      k0 = lo / K;
+      n = (hi - 1) / K - k0 + 1;
+    }
+  }
+  uint64_t todo = __ballot(n > LONG_CHUNKS);
+  for (uint32_t r = 0; todo; r++) {   // uniform in the wave: the ballot is the wave's
+    const uint32_t i = (uint32_t)__ffsll((long long)todo) - 1;
+    todo &= todo - 1;
+    if ((r & 3u) != wv) continue;
+    const uint32_t first = (uint32_t)__shfl((int)k0, (int)i) + blockIdx.x * 64 + i;   // slot k0 + b
+    const uint32_t cnt = (uint32_t)__shfl((int)n, (int)i);                             // > 64 slots
+    g1x28 x = load_x28(partials + MZK_X28_WORDS * (size_t)(first + lane));
+    for (uint32_t k = lane + 64; k < cnt; k += 64) x = g1x28_add(x, load_x28(partials + MZK_X28_WORDS * (size_t)(first + k)));
+    for (int d = 32; d >= 1; d >>= 1) x = g1x28_add(x, shfl_xor_x28(x, d));
+    if (lane == 0) store_x28_raw(partials + MZK_X28_WORDS * (size_t)first, x);
+  }
+}
+
 // Segment t of L buckets of one set (b = t L + i): S_b = sum of the chunk partials
 // of bucket b, then running sums from the top: R_t = sum_i S_b and
 // A_t = sum_i (i + 1) S_b.  The set's weighted sum is then
@@ -588,11 +652,9 @@ __global__ void __launch_bounds__(64) k_bucket_seg(const uint32_t* __restrict__ 
   g1x28 run = g1x28_inf(), acc = g1x28_inf();
   for (uint32_t i = L; i-- > 0;) {
     const uint32_t b = t * L + i;
-    const uint32_t lo = off[b], hi = off[b + 1];
-    if (hi > lo) {
-      const uint32_t k0 = lo / K, k1 = (hi - 1) / K;
+    uint32_t k0, k1;
+    if (bucket_chunks(off, b, K, k0, k1))
       for (uint32_t k = k0; k <= k1; k++) run = g1x28_add(run, load_x28(partials + MZK_X28_WORDS * (size_t)(k + b)));
-    }
     acc = g1x28_add(acc, run);
   }
   store_x28_raw(A + MZK_X28_WORDS * (size_t)t, acc);
@@ -616,11 +678,9 @@ __global__ void __launch_bounds__(256) k_set_reduce(const uint32_t* __restrict__
   g1x28 x = g1x28_inf();
   if (live) {
     const uint32_t b = set * B + i;
-    const uint32_t lo = off[b], hi = off[b + 1];
-    if (hi > lo) {
-      const uint32_t k0 = lo / K, k1 = (hi - 1) / K;
+    uint32_t k0, k1;
+    if (bucket_chunks(off, b, K, k0, k1))
       for (uint32_t k = k0; k <= k1; k++) x = g1x28_add(x, load_x28(partials + MZK_X28_WORDS * (size_t)(k + b)));
-    }
   }
   uint32_t* mine = lds + MZK_X28_WORDS * t;
   store_x28_raw(mine, x);
@@ -668,9 +728,8 @@ __global__ void __launch_bounds__(512) k_set_reduce_quad(const uint32_t* __restr
   g1x28 x = g1x28_inf();
   if (live) {
     const uint32_t b = set * B + i;
-    const uint32_t lo = off[b], hi = off[b + 1];
-    if (hi > lo) {   // the first partial loaded, the rest added (an addition to the identity costs a full one here)
-      const uint32_t k0 = lo / K, k1 = (hi - 1) / K;
+    uint32_t k0, k1;
+    if (bucket_chunks(off, b, K, k0, k1)) {   // the first partial loaded, the rest added (an addition to the identity costs a full one here)
       x = load_x28(partials + MZK_X28_WORDS * (size_t)(k0 + b));
       for (uint32_t k = k0 + 1; k <= k1; k++) x = g1x28_add_quad(x, partials + MZK_X28_WORDS * (size_t)(k + b));
     }
@@ -711,10 +770,9 @@ __global__ void __launch_bounds__(256) k_bucket_merge_quad(const uint32_t* __res
                                                            uint32_t* __restrict__ A, uint32_t* __restrict__ R) {
   const uint32_t b = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
   if (b >= NB) return;   // whole quads leave together (NB quads, 4 lanes each)
-  const uint32_t lo = off[b], hi = off[b + 1];
   g1x28 x = g1x28_inf();
-  if (hi > lo) {
-    const uint32_t k0 = lo / K, k1 = (hi - 1) / K;
+  uint32_t k0, k1;
+  if (bucket_chunks(off, b, K, k0, k1)) {
     x = load_x28(partials + MZK_X28_WORDS * (size_t)(k0 + b));
     for (uint32_t k = k0 + 1; k <= k1; k++) x = g1x28_add_quad(x, partials + MZK_X28_WORDS * (size_t)(k + b));
   }
@@ -772,10 +830,9 @@ __global__ void __launch_bounds__(4 * CT_LEAVES) k_chan_tree(const uint32_t* __r
   const uint32_t t = threadIdx.x, qd = t >> 2, ql = t & 3u;
   {
     const uint32_t b = blockIdx.x * CT_LEAVES + qd;
-    const uint32_t lo = off[b], hi = off[b + 1];
     g1x28 x = g1x28_inf();
-    if (hi > lo) {
-      const uint32_t k0 = lo / K, k1 = (hi - 1) / K;
+    uint32_t k0, k1;
+    if (bucket_chunks(off, b, K, k0, k1)) {
       x = load_x28(partials + MZK_X28_WORDS * (size_t)(k0 + b));
       for (uint32_t k = k0 + 1; k <= k1; k++) x = g1x28_add_quad(x, partials + MZK_X28_WORDS * (size_t)(k + b));
     }
@@ -1414,6 +1471,7 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
     else
       MZK_LAUNCH(c, "k_msm_accum", k_msm_accum<false>, dim3(grid_for(nchunks, 64)), dim3(64), 0, refs, off, NB, K,
                  d_bases, partials);
+    MZK_LAUNCH(c, "k_long_merge", k_long_merge, dim3(grid_for(NB, 64)), dim3(256), 0, off, NB, K, partials);
     if (small_sets) {
       // per-set weighted sums in LDS (one per set, into chan); a quad of lanes per bucket
       // when the sets are too few to fill the SIMDs one lane per bucket

@@ -1,0 +1,105 @@
+"""MSMs of skewed scalars (GPU): boolean, small-valued and constant columns put most window
+digits into a few buckets, whose chunk partials the reductions then merge from one slot
+after the long-bucket merge.  Every result against the C restatement of msm_best; each
+scalar kind through the single shared set, a batch of three shared sets (block tree), a
+batch of six (segments) and the per-window sets of host bases."""
+import random
+
+import numpy as np
+import pytest
+
+from oracle import bls12_381 as O
+from oracle import cpu_ref as C
+from tests.helpers import mzk, to_dev
+
+pytestmark = pytest.mark.gpu
+
+N_SRS = 1 << 16
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    C.set_threads(16)
+    c = mzk.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def srs(ctx):
+    tau = C.fr_from_seed(77, 0, 1)[0]
+    bases = C.srs_bases(tau, N_SRS)
+    s = ctx.srs_upload(bases)
+    yield bases, s
+    s.close()
+
+
+def aff(j):
+    return O.g1_jacobian_from_mont_limbs(j)
+
+
+def scalars(kind, n, seed):
+    rng = random.Random(seed)
+    if kind == "bool":
+        vals = [rng.randrange(2) for _ in range(n)]
+    elif kind == "nibble":
+        vals = [rng.randrange(16) for _ in range(n)]
+    elif kind == "const":
+        vals = [5] * n
+    else:   # one huge run of a repeated value among random ones
+        vals = [rng.randrange(O.R_MOD) if i % 8 == 0 else 3 for i in range(n)]
+    return C.ints_to_fr_rows(vals)
+
+
+KINDS = ["bool", "nibble", "const", "runs"]
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_single_shared_set(ctx, srs, kind):
+    bases, s = srs
+    n = 1 << 16
+    sc = scalars(kind, n, 1)
+    assert aff(s.commit(to_dev(sc), n)) == aff(C.msm_best(sc, bases[:n]))
+    assert ctx.msm_last_plan()["bucket_sets"] == 1
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("nj", [3, 6])
+def test_batch_shared_sets(ctx, srs, kind, nj):
+    bases, s = srs
+    n = (1 << 15) + 3
+    rng = random.Random(nj)
+    host, jobs = [], []
+    for i in range(nj):
+        sc = scalars(kind, n, 10 * nj + i)
+        off = rng.randrange(0, N_SRS - n + 1)
+        host.append((sc, off))
+        jobs.append((to_dev(sc), n, off))
+    outs = s.commit_batch(jobs)
+    assert ctx.msm_last_plan()["bucket_sets"] == nj
+    for (sc, off), out in zip(host, outs):
+        assert aff(out) == aff(C.msm_best(sc, bases[off:off + n])), (kind, nj, off)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_per_window_sets(ctx, srs, kind):
+    bases, _ = srs
+    n = 1 << 14
+    sc = scalars(kind, n, 3)
+    out = ctx.msm_best(sc, bases[:n])
+    assert ctx.msm_last_plan()["bucket_sets"] > 1
+    assert aff(out) == aff(C.msm_best(sc, bases[:n]))
+
+
+def test_long_bucket_threshold_edges(ctx, srs):
+    """Buckets spanning just under, at and just over the long-bucket threshold: a column
+    of 0/1 whose ones number 64, 65 and 66 chunks' worth of references."""
+    bases, s = srs
+    n = 1 << 16
+    s.commit(to_dev(scalars("bool", n, 9)), n)
+    K = ctx.msm_last_plan()["refs_per_thread"]
+    for chunks in (63, 64, 65, 66):
+        ones = min(n, chunks * K)
+        vals = [1] * ones + [0] * (n - ones)
+        sc = C.ints_to_fr_rows(vals)
+        assert aff(s.commit(to_dev(sc), n)) == aff(C.msm_best(sc, bases[:n])), chunks
