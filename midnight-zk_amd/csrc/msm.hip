@@ -123,6 +123,9 @@ struct SortArgs {
   // > 0: key low bits and reference share one word between scatter and bins, as
   // (low key << pack) | reference index | sign << (pack - 1); 0: two arrays
   uint32_t pack;
+  // refs per accumulation chunk, written by k_sort_bins to off[NB + 1] once the digit count
+  // M is known: ceil(M / kchunks) within [kmin, kmax]
+  uint32_t kchunks, kmin, kmax;
 };
 
 // Bucket key of digit d of window w of a job.
@@ -349,7 +352,11 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_bins(const uint32_t* __rest
       c2[d] = run;
       run += v;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) off[NB] = hoff[(size_t)a.D1 * a.G];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      const uint32_t M = hoff[(size_t)a.D1 * a.G];
+      off[NB] = M;
+      off[NB + 1] = min(a.kmax, max(a.kmin, (uint32_t)(((uint64_t)M + a.kchunks - 1) / a.kchunks)));
+    }
   }
   __syncthreads();
   for (uint32_t d = threadIdx.x; d < R; d += SORT_BLOCK) {
@@ -528,10 +535,10 @@ __device__ __forceinline__ bool decode_point(const RawPoint<REC>& o, f28& x, f28
 
 template <bool REC>
 __global__ void __launch_bounds__(64) k_msm_accum(const uint32_t* __restrict__ refs,
-                                                  const uint32_t* __restrict__ off, uint32_t NB, uint32_t K,
+                                                  const uint32_t* __restrict__ off, uint32_t NB,
                                                   const uint32_t* __restrict__ bases,
                                                   uint32_t* __restrict__ partials) {
-  const uint32_t M = off[NB];
+  const uint32_t M = off[NB], K = off[NB + 1];
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t e0_64 = (uint64_t)k * K;
   if (e0_64 >= M) return;
@@ -614,8 +621,9 @@ __device__ __forceinline__ g1x28 shfl_xor_x28(const g1x28& a, int d) {
 // over the lanes by shuffles (no LDS, no barrier: each wave runs alone) leaves the
 // bucket's sum in its first slot.  With no long bucket a workgroup reads 65 offsets and
 // leaves.
-__global__ void __launch_bounds__(256) k_long_merge(const uint32_t* __restrict__ off, uint32_t NB, uint32_t K,
+__global__ void __launch_bounds__(256) k_long_merge(const uint32_t* __restrict__ off, uint32_t NB,
                                                     uint32_t* __restrict__ partials) {
+  const uint32_t K = off[NB + 1];
   const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
   const uint32_t b_lane = blockIdx.x * 64 + lane;
   uint32_t k0 = 0, n = 0;
@@ -644,9 +652,10 @@ __global__ void __launch_bounds__(256) k_long_merge(const uint32_t* __restrict__
 // of bucket b, then running sums from the top: R_t = sum_i S_b and
 // A_t = sum_i (i + 1) S_b.  The set's weighted sum is then
 //   sum_b (b + 1) S_b = sum_t A_t + L sum_t t R_t.
-__global__ void __launch_bounds__(64) k_bucket_seg(const uint32_t* __restrict__ off, uint32_t NB, uint32_t K,
+__global__ void __launch_bounds__(64) k_bucket_seg(const uint32_t* __restrict__ off, uint32_t NB,
                                                    uint32_t L, const uint32_t* __restrict__ partials,
                                                    uint32_t* __restrict__ A, uint32_t* __restrict__ R) {
+  const uint32_t K = off[NB + 1];
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if ((size_t)t * L >= NB) return;
   g1x28 run = g1x28_inf(), acc = g1x28_inf();
@@ -666,10 +675,11 @@ __global__ void __launch_bounds__(64) k_bucket_seg(const uint32_t* __restrict__ 
 // partials into S_i, a suffix scan in LDS gives U_i = sum_(i' >= i) S_i', and a tree sum
 // gives sum_i U_i = sum_i (i + 1) S_i, the set's weighted bucket sum, written in the
 // blst layout for the host's window combination.
-__global__ void __launch_bounds__(256) k_set_reduce(const uint32_t* __restrict__ off, uint32_t NB, uint32_t K,
+__global__ void __launch_bounds__(256) k_set_reduce(const uint32_t* __restrict__ off, uint32_t NB,
                                                     uint32_t B, uint32_t nsets,
                                                     const uint32_t* __restrict__ partials,
                                                     uint32_t* __restrict__ out) {
+  const uint32_t K = off[NB + 1];
   extern __shared__ uint32_t lds[];
   const uint32_t t = threadIdx.x;
   const uint32_t i = t % B;
@@ -716,10 +726,11 @@ __global__ void __launch_bounds__(256) k_set_reduce(const uint32_t* __restrict__
 // latency is the kernel's time.  128 quads per workgroup, 128 / B whole sets (B <= 128).
 static constexpr uint32_t QUAD_SETS_MAX_BUCKETS = 16384;   // sets x B at most: 1 wave per SIMD as quads
 
-__global__ void __launch_bounds__(512) k_set_reduce_quad(const uint32_t* __restrict__ off, uint32_t NB, uint32_t K,
+__global__ void __launch_bounds__(512) k_set_reduce_quad(const uint32_t* __restrict__ off, uint32_t NB,
                                                          uint32_t B, uint32_t nsets,
                                                          const uint32_t* __restrict__ partials,
                                                          uint32_t* __restrict__ out) {
+  const uint32_t K = off[NB + 1];
   extern __shared__ uint32_t lds[];   // two buffers of 128 points: the scan reads one, writes the other
   const uint32_t t = threadIdx.x, qd = t >> 2, ql = t & 3u;
   const uint32_t i = qd % B;
@@ -765,9 +776,10 @@ __global__ void __launch_bounds__(512) k_set_reduce_quad(const uint32_t* __restr
 // k_bucket_seg for segments of one bucket (a single set): a quad per bucket merges its
 // chunk partials with quad-lane additions; A_t = R_t = S_b.  2^15 quads put two waves on
 // each SIMD where one lane per bucket left half of them idle.
-__global__ void __launch_bounds__(256) k_bucket_merge_quad(const uint32_t* __restrict__ off, uint32_t NB, uint32_t K,
+__global__ void __launch_bounds__(256) k_bucket_merge_quad(const uint32_t* __restrict__ off, uint32_t NB,
                                                            const uint32_t* __restrict__ partials,
                                                            uint32_t* __restrict__ A, uint32_t* __restrict__ R) {
+  const uint32_t K = off[NB + 1];
   const uint32_t b = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
   if (b >= NB) return;   // whole quads leave together (NB quads, 4 lanes each)
   g1x28 x = g1x28_inf();
@@ -823,11 +835,12 @@ __device__ __forceinline__ uint32_t ct_acc_slot(uint32_t i, uint32_t m) {
 // (as k_bucket_merge_quad), then the tree; writes out[CT_REC k + 0] = P, out[CT_REC k + 1 + i]
 // = A_i (radix-2^28 XYZZ, MZK_X28_WORDS words each).  LDS: P of the current level (in place,
 // CT_LEAVES slots) and the A_i (CT_LEAVES - 1 slots), (2 CT_LEAVES - 1) points.
-__global__ void __launch_bounds__(4 * CT_LEAVES) k_chan_tree(const uint32_t* __restrict__ off, uint32_t K,
+__global__ void __launch_bounds__(4 * CT_LEAVES) k_chan_tree(const uint32_t* __restrict__ off, uint32_t NB,
                                                             const uint32_t* __restrict__ partials,
                                                             uint32_t* __restrict__ out) {
   extern __shared__ uint32_t lds[];
   const uint32_t t = threadIdx.x, qd = t >> 2, ql = t & 3u;
+  const uint32_t K = off[NB + 1];
   {
     const uint32_t b = blockIdx.x * CT_LEAVES + qd;
     g1x28 x = g1x28_inf();
@@ -1353,6 +1366,14 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
     K = (uint32_t)std::min<size_t>(256, std::max<size_t>(K, (nmax + 4 * (size_t)p.B - 1) / (4 * (size_t)p.B)));
   }
   const size_t nchunks = (M_max + K - 1) / K;
+  // The chunk size the kernels use comes from the digits the sort found: shared sets take
+  // ceil(M / nchunks) refs per thread (at least 2), so sparse scalars (zeros give no
+  // digits: a delta-encoded column, a mostly-zero advice column) still spread their
+  // digits over every thread instead of leaving a few threads K serial additions each at
+  // a small fraction of the occupancy; per-window sets keep K (their chains' bound).
+  sa.kchunks = (uint32_t)nchunks;
+  sa.kmax = K;
+  sa.kmin = p.per_window ? K : std::min<uint32_t>(2, K);
   // reduction geometry: segments of L buckets, T per set; channels A + bits of t.
   // One MSM alone: a thread per bucket keeps chains short.  Many sets: segments of
   // 4 cut the channel work, which is then parallel across sets; with segments enough for
@@ -1379,10 +1400,10 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   const bool segtree = L > 1 && !(p.per_window && p.B <= 256) && (T & (T - 1)) == 0 && T >= 2 * ST_LEAVES &&
                        T <= 256 * ST_LEAVES;
 
-  // scratch A: descriptors | canonical scalars | hist | hoff (HN+1) | tile sums | off (NB+1)
+  // scratch A: descriptors | canonical scalars | hist | hoff (HN+1) | tile sums | off (NB+1) | K
   const size_t desc_words =
       (nj * sizeof(JobDesc) + td.size() * sizeof(TileDesc) + ngroups * sizeof(GroupDesc) + 31) / 4 + 8;
-  size_t a_words = desc_words + 8 * ntot + HN + (HN + 1) + tiles + ((size_t)NB + 1) + 64;
+  size_t a_words = desc_words + 8 * ntot + HN + (HN + 1) + tiles + ((size_t)NB + 2) + 64;   // off: NB + 1 and K
   MZK_TRY(c->scratch[SCR_MSM_A].reserve(a_words * 4));
   uint32_t* base = (uint32_t*)c->scratch[SCR_MSM_A].ptr;
   JobDesc* d_jobs = (JobDesc*)base;
@@ -1466,36 +1487,36 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
                hoff, sa, NB,
                tmp_key, tmp_ref, refs, off);
     if (records)
-      MZK_LAUNCH(c, "k_msm_accum", k_msm_accum<true>, dim3(grid_for(nchunks, 64)), dim3(64), 0, refs, off, NB, K,
+      MZK_LAUNCH(c, "k_msm_accum", k_msm_accum<true>, dim3(grid_for(nchunks, 64)), dim3(64), 0, refs, off, NB,
                  d_bases, partials);
     else
-      MZK_LAUNCH(c, "k_msm_accum", k_msm_accum<false>, dim3(grid_for(nchunks, 64)), dim3(64), 0, refs, off, NB, K,
+      MZK_LAUNCH(c, "k_msm_accum", k_msm_accum<false>, dim3(grid_for(nchunks, 64)), dim3(64), 0, refs, off, NB,
                  d_bases, partials);
-    MZK_LAUNCH(c, "k_long_merge", k_long_merge, dim3(grid_for(NB, 64)), dim3(256), 0, off, NB, K, partials);
+    MZK_LAUNCH(c, "k_long_merge", k_long_merge, dim3(grid_for(NB, 64)), dim3(256), 0, off, NB, partials);
     if (small_sets) {
       // per-set weighted sums in LDS (one per set, into chan); a quad of lanes per bucket
       // when the sets are too few to fill the SIMDs one lane per bucket
       if (p.B <= 128 && sets * p.B <= QUAD_SETS_MAX_BUCKETS) {
         const uint32_t spb = 128 / p.B;
         MZK_LAUNCH(c, "k_set_reduce", k_set_reduce_quad, dim3((unsigned)((sets + spb - 1) / spb)), dim3(512),
-                   2 * 128 * MZK_X28_WORDS * 4, off, NB, K, p.B, (uint32_t)sets, partials, chan);
+                   2 * 128 * MZK_X28_WORDS * 4, off, NB, p.B, (uint32_t)sets, partials, chan);
       } else {
         const uint32_t spb = 256 / p.B;
         MZK_LAUNCH(c, "k_set_reduce", k_set_reduce, dim3((unsigned)((sets + spb - 1) / spb)), dim3(256),
-                   256 * MZK_X28_WORDS * 4, off, NB, K, p.B, (uint32_t)sets, partials, chan);
+                   256 * MZK_X28_WORDS * 4, off, NB, p.B, (uint32_t)sets, partials, chan);
       }
     } else if (tree) {
       // sets of 2^t buckets: the merge and the tree by blocks, then the channels of each set
       MZK_LAUNCH(c, "k_chan_tree", k_chan_tree, dim3(NB / CT_LEAVES), dim3(4 * CT_LEAVES),
-                 (2 * CT_LEAVES - 1) * MZK_X28_WORDS * 4, off, K, partials, segA);
+                 (2 * CT_LEAVES - 1) * MZK_X28_WORDS * 4, off, NB, partials, segA);
       MZK_LAUNCH(c, "k_chan_top", k_chan_top, dim3((unsigned)nch, (unsigned)sets), dim3(4 * CT_TOP_QUADS),
                  CT_TOP_QUADS * MZK_X28_WORDS * 4, segA, T / CT_LEAVES, chan);
     } else {
       if (L == 1)   // a single set: a quad per bucket
         MZK_LAUNCH(c, "k_bucket_seg", k_bucket_merge_quad, dim3(grid_for((size_t)NB * 4, 256)), dim3(256), 0, off, NB,
-                   K, partials, segA, segR);
+                   partials, segA, segR);
       else
-        MZK_LAUNCH(c, "k_bucket_seg", k_bucket_seg, dim3(grid_for(nseg, 64)), dim3(64), 0, off, NB, K, L, partials,
+        MZK_LAUNCH(c, "k_bucket_seg", k_bucket_seg, dim3(grid_for(nseg, 64)), dim3(64), 0, off, NB, L, partials,
                    segA, segR);
       if (segtree) {   // the segments' channels by the block tree: records into the chunk partials' space
         const uint32_t nrec = T / ST_LEAVES, top_threads = std::max<uint32_t>(64, nrec);
@@ -1521,15 +1542,15 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
     double h0 = host_ms();
     prepare_jobs(nj);
     MZK_HIP(hipMemcpyAsync(res, tot, sets * 48 * 4, hipMemcpyDeviceToHost, c->stream));
-    MZK_HIP(hipMemcpyAsync(res + sets * 48, off + NB, 4, hipMemcpyDeviceToHost, c->stream));
+    MZK_HIP(hipMemcpyAsync(res + sets * 48, off + NB, 8, hipMemcpyDeviceToHost, c->stream));
     MZK_HIP(hipStreamSynchronize(c->stream));
-    const uint32_t M = res[sets * 48];
+    const uint32_t M = res[sets * 48], Keff = res[sets * 48 + 1];
     double h1 = host_ms();
     c->prof.host("host_wait", h1 - h0);
     c->last_msm[0] = (uint64_t)p.c;
     c->last_msm[1] = (uint64_t)p.W;
     c->last_msm[2] = (uint64_t)sets;
-    c->last_msm[3] = K;
+    c->last_msm[3] = Keff;
     c->last_msm[4] = M;
     c->last_msm[5] = ntot;
     parallel_jobs(nj, [&](size_t j) {
@@ -1543,15 +1564,15 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   double h0 = host_ms();
   prepare_jobs(nj);
   MZK_HIP(hipMemcpyAsync(res, chan, nchan * 48 * 4, hipMemcpyDeviceToHost, c->stream));
-  MZK_HIP(hipMemcpyAsync(res + nchan * 48, off + NB, 4, hipMemcpyDeviceToHost, c->stream));
+  MZK_HIP(hipMemcpyAsync(res + nchan * 48, off + NB, 8, hipMemcpyDeviceToHost, c->stream));
   MZK_HIP(hipStreamSynchronize(c->stream));
-  const uint32_t M = res[nchan * 48];
+  const uint32_t M = res[nchan * 48], Keff = res[nchan * 48 + 1];
   double h1 = host_ms();
   c->prof.host("host_wait", h1 - h0);
   c->last_msm[0] = (uint64_t)p.c;
   c->last_msm[1] = (uint64_t)p.W;
   c->last_msm[2] = (uint64_t)sets;
-  c->last_msm[3] = K;
+  c->last_msm[3] = Keff;
   c->last_msm[4] = M;
   c->last_msm[5] = ntot;
   const size_t per_job = (size_t)spj * nch * 48;

@@ -92,14 +92,33 @@ def test_per_window_sets(ctx, srs, kind):
 
 
 def test_long_bucket_threshold_edges(ctx, srs):
-    """Buckets spanning just under, at and just over the long-bucket threshold: a column
-    of 0/1 whose ones number 64, 65 and 66 chunks' worth of references."""
+    """Buckets spanning just under, at and just over the long-bucket threshold (64 chunks):
+    a column of ones and zeros, all of whose digits fall in one bucket.  The chunk size
+    comes from the digits found (ceil(M / chunks), at least 2): with a few hundred ones it
+    is 2, so 2 c ones span c chunks."""
     bases, s = srs
     n = 1 << 16
-    s.commit(to_dev(scalars("bool", n, 9)), n)
-    K = ctx.msm_last_plan()["refs_per_thread"]
     for chunks in (63, 64, 65, 66):
-        ones = min(n, chunks * K)
+        ones = 2 * chunks
         vals = [1] * ones + [0] * (n - ones)
         sc = C.ints_to_fr_rows(vals)
         assert aff(s.commit(to_dev(sc), n)) == aff(C.msm_best(sc, bases[:n])), chunks
+        plan = ctx.msm_last_plan()
+        assert plan["refs_per_thread"] == 2 and plan["digits"] == ones, plan
+
+
+@pytest.mark.parametrize("density", [0.001, 0.05, 0.3])
+def test_sparse_columns(ctx, srs, density):
+    """Mostly-zero columns: the chunk size follows the digits found, not the column's
+    length; single commit and a batch of three against msm_best."""
+    bases, s = srs
+    n = 1 << 16
+    rng = random.Random(int(density * 1000))
+    cols = []
+    for i in range(3):
+        vals = [rng.randrange(1, O.R_MOD) if rng.random() < density else 0 for _ in range(n)]
+        cols.append(C.ints_to_fr_rows(vals))
+    assert aff(s.commit(to_dev(cols[0]), n)) == aff(C.msm_best(cols[0], bases[:n]))
+    outs = s.commit_batch([(to_dev(c), n, 0) for c in cols])
+    for c, out in zip(cols, outs):
+        assert aff(out) == aff(C.msm_best(c, bases[:n]))
