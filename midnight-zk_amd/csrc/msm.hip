@@ -312,7 +312,8 @@ static constexpr size_t BINS_LDS_CU = 160u << 10;   // LDS per CU
 __global__ void __launch_bounds__(SORT_BLOCK) k_sort_bins(const uint32_t* __restrict__ hoff, SortArgs a, uint32_t NB,
                                                           const uint32_t* __restrict__ tmp_key,
                                                           const uint32_t* __restrict__ tmp_ref,
-                                                          uint32_t* __restrict__ refs, uint32_t* __restrict__ off) {
+                                                          uint32_t* __restrict__ refs, uint32_t* __restrict__ off,
+                                                          uint32_t* __restrict__ bmax) {
   extern __shared__ uint32_t c2[];   // 2^l2 counters, then scan scratch
   const uint32_t g = blockIdx.x / a.D1, bin = blockIdx.x % a.D1;
   const GroupDesc gr = a.groups[g];
@@ -343,8 +344,11 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_bins(const uint32_t* __rest
     __shared__ uint32_t wtot[SORT_BLOCK / 64];
     const uint32_t per = (R + SORT_BLOCK - 1) / SORT_BLOCK;
     const uint32_t d0 = threadIdx.x * per, d1 = d0 + per < R ? d0 + per : R;
-    uint32_t part = 0;
-    for (uint32_t d = d0; d < d1; d++) part += c2[d];
+    uint32_t part = 0, big = 0;
+    for (uint32_t d = d0; d < d1; d++) {
+      part += c2[d];
+      big = max(big, c2[d]);
+    }
     uint32_t all;
     uint32_t run = lo + block_excl_scan(part, wtot, &all);
     for (uint32_t d = d0; d < d1; d++) {
@@ -352,11 +356,17 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_bins(const uint32_t* __rest
       c2[d] = run;
       run += v;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      const uint32_t M = hoff[(size_t)a.D1 * a.G];
-      off[NB] = M;
-      off[NB + 1] = min(a.kmax, max(a.kmin, (uint32_t)(((uint64_t)M + a.kchunks - 1) / a.kchunks)));
+    // the bin's largest bucket, for k_chunk_size
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) big = max(big, (uint32_t)__shfl_xor((int)big, d));
+    if ((threadIdx.x & 63u) == 0) wtot[threadIdx.x >> 6] = big;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t m = 0;
+      for (int w = 0; w < SORT_BLOCK / 64; w++) m = max(m, wtot[w]);
+      bmax[blockIdx.x] = m;
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) off[NB] = hoff[(size_t)a.D1 * a.G];
   }
   __syncthreads();
   for (uint32_t d = threadIdx.x; d < R; d += SORT_BLOCK) {
@@ -399,6 +409,31 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_bins(const uint32_t* __rest
         }
       }
     }
+  }
+}
+
+// The refs per accumulation chunk, from the sort's results (off[NB] = M digits, bmax[] the
+// bins' largest buckets): ceil(M / kchunks) spreads the digits over every thread (sparse
+// scalars give few digits), but a bucket of D digits then spans D / K chunks whose
+// partials k_long_merge sums by 64 lanes; on the few waves such inputs leave, the
+// accumulation's K mixed additions (~9 us each on a lone wave) and the merge's D / (64 K)
+// one-lane additions (~23 us each) balance near K = sqrt(D) / 4.  K = the larger of the
+// two, within [kmin, kmax]; written to off[NB + 1], where every later kernel reads it.
+__global__ void __launch_bounds__(256) k_chunk_size(uint32_t* __restrict__ off, uint32_t NB,
+                                                    const uint32_t* __restrict__ bmax, uint32_t nbins, SortArgs a) {
+  __shared__ uint32_t wm[4];
+  uint32_t big = 0;
+  for (uint32_t i = threadIdx.x; i < nbins; i += 256) big = max(big, bmax[i]);
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) big = max(big, (uint32_t)__shfl_xor((int)big, d));
+  if ((threadIdx.x & 63u) == 0) wm[threadIdx.x >> 6] = big;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t D = max(max(wm[0], wm[1]), max(wm[2], wm[3]));
+    const uint32_t M = off[NB];
+    const uint32_t spread = (uint32_t)(((uint64_t)M + a.kchunks - 1) / a.kchunks);
+    const uint32_t skew = (uint32_t)ceilf(sqrtf((float)D) * 0.25f);
+    off[NB + 1] = min(a.kmax, max(a.kmin, max(spread, skew)));
   }
 }
 
@@ -1366,11 +1401,12 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
     K = (uint32_t)std::min<size_t>(256, std::max<size_t>(K, (nmax + 4 * (size_t)p.B - 1) / (4 * (size_t)p.B)));
   }
   const size_t nchunks = (M_max + K - 1) / K;
-  // The chunk size the kernels use comes from the digits the sort found: shared sets take
-  // ceil(M / nchunks) refs per thread (at least 2), so sparse scalars (zeros give no
-  // digits: a delta-encoded column, a mostly-zero advice column) still spread their
-  // digits over every thread instead of leaving a few threads K serial additions each at
-  // a small fraction of the occupancy; per-window sets keep K (their chains' bound).
+  // The chunk size the kernels use comes from the digits the sort found (k_chunk_size):
+  // for shared sets ceil(M / nchunks) refs per thread, at least 2 and at least sqrt(D) / 4
+  // for the largest bucket's D digits, so sparse scalars (zeros give no digits: a
+  // delta-encoded column, a mostly-zero advice column) still spread their digits over
+  // every thread while skewed ones (a boolean column) keep their long buckets' partials
+  // few; per-window sets keep K (their chains' bound).
   sa.kchunks = (uint32_t)nchunks;
   sa.kmax = K;
   sa.kmin = p.per_window ? K : std::min<uint32_t>(2, K);
@@ -1400,10 +1436,12 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   const bool segtree = L > 1 && !(p.per_window && p.B <= 256) && (T & (T - 1)) == 0 && T >= 2 * ST_LEAVES &&
                        T <= 256 * ST_LEAVES;
 
-  // scratch A: descriptors | canonical scalars | hist | hoff (HN+1) | tile sums | off (NB+1) | K
+  // scratch A: descriptors | canonical scalars | hist | hoff (HN+1) | tile sums | off (NB+1) | K |
+  // the bins' largest buckets
   const size_t desc_words =
       (nj * sizeof(JobDesc) + td.size() * sizeof(TileDesc) + ngroups * sizeof(GroupDesc) + 31) / 4 + 8;
-  size_t a_words = desc_words + 8 * ntot + HN + (HN + 1) + tiles + ((size_t)NB + 2) + 64;   // off: NB + 1 and K
+  // off: NB + 1 offsets, K, then the bins' largest buckets (8-word aligned)
+  size_t a_words = desc_words + 8 * ntot + HN + (HN + 1) + tiles + ((size_t)NB + 8) + ngroups * sa.D1 + 64;
   MZK_TRY(c->scratch[SCR_MSM_A].reserve(a_words * 4));
   uint32_t* base = (uint32_t*)c->scratch[SCR_MSM_A].ptr;
   JobDesc* d_jobs = (JobDesc*)base;
@@ -1413,6 +1451,7 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
   uint32_t* hoff = hist + HN;
   uint32_t* tsums = hoff + HN + 1;
   uint32_t* off = tsums + tiles;
+  uint32_t* bmax = off + (((size_t)NB + 2 + 7) & ~(size_t)7);
   // descriptors through pinned memory: [jobs | pad | tiles] as laid out in scratch A, one
   // asynchronous copy; the channel sums come back through the same buffer after it
   const size_t tiles_at = (size_t)((char*)d_tiles - (char*)d_jobs);
@@ -1485,7 +1524,8 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
                  tmp_key, tmp_ref);
     MZK_LAUNCH(c, "k_sort_bins", k_sort_bins, dim3((unsigned)(ngroups * sa.D1)), dim3(SORT_BLOCK), bins_lds,
                hoff, sa, NB,
-               tmp_key, tmp_ref, refs, off);
+               tmp_key, tmp_ref, refs, off, bmax);
+    MZK_LAUNCH(c, "k_chunk_size", k_chunk_size, dim3(1), dim3(256), 0, off, NB, bmax, (uint32_t)(ngroups * sa.D1), sa);
     if (records)
       MZK_LAUNCH(c, "k_msm_accum", k_msm_accum<true>, dim3(grid_for(nchunks, 64)), dim3(64), 0, refs, off, NB,
                  d_bases, partials);

@@ -3,6 +3,7 @@ digits into a few buckets, whose chunk partials the reductions then merge from o
 after the long-bucket merge.  Every result against the C restatement of msm_best; each
 scalar kind through the single shared set, a batch of three shared sets (block tree), a
 batch of six (segments) and the per-window sets of host bases."""
+import math
 import random
 
 import numpy as np
@@ -94,17 +95,19 @@ def test_per_window_sets(ctx, srs, kind):
 def test_long_bucket_threshold_edges(ctx, srs):
     """Buckets spanning just under, at and just over the long-bucket threshold (64 chunks):
     a column of ones and zeros, all of whose digits fall in one bucket.  The chunk size
-    comes from the digits found (ceil(M / chunks), at least 2): with a few hundred ones it
-    is 2, so 2 c ones span c chunks."""
+    comes from the digits found (ceil(M / chunks), at least 2 and at least sqrt(D) / 4 for
+    the largest bucket's D digits), so k c ones at the k the rule gives span c chunks."""
     bases, s = srs
     n = 1 << 16
     for chunks in (63, 64, 65, 66):
-        ones = 2 * chunks
+        # the chunk size k the rule gives k chunks' worth of ones: 4, 4, 5, 5
+        k = next(k for k in range(2, 16) if max(2, math.ceil(math.sqrt(k * chunks) / 4)) == k)
+        ones = k * chunks
         vals = [1] * ones + [0] * (n - ones)
         sc = C.ints_to_fr_rows(vals)
         assert aff(s.commit(to_dev(sc), n)) == aff(C.msm_best(sc, bases[:n])), chunks
         plan = ctx.msm_last_plan()
-        assert plan["refs_per_thread"] == 2 and plan["digits"] == ones, plan
+        assert plan["refs_per_thread"] == k and plan["digits"] == ones, plan
 
 
 @pytest.mark.parametrize("density", [0.001, 0.05, 0.3])
