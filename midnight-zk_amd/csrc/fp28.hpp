@@ -260,7 +260,8 @@ MZK_HD bool f28_is_zero_mod_p(const f28& a) {
   // the low digit of k p is k p_0 mod 2^28 (p_0 odd): k = lo p_0^-1 mod 2^28
   const uint32_t lo = a.l[0] & f28c::M28;
   const uint32_t kk = (lo * f28c::P0INV) & f28c::M28;
-  if (kk - 1u >= (uint32_t)(KMAX - 1)) return false;
+  // (the likely exit: the slow path below is placed out of the callers' loops)
+  if (__builtin_expect(kk - 1u >= (uint32_t)(KMAX - 1), 1)) return false;
   const f28 n = f28_norm(a);
   // unrolled: the digits of k p are then constants (a runtime k or i would index the
   // constant tables in scratch memory)

@@ -95,7 +95,7 @@ MZK_HD g1x28 g1x28_madd(const g1x28& p, const f28& x, const f28& y, bool neg) {
   const f28 P = f28_sub<16, 1>(U2, p.X);
   // R = +-S2 - Y: S2 + 8p - Y < 10p, or 16p - S2 - Y < 16p (lower limbs raised by 2 2^28)
   const f28 R = neg ? f28_neg_sum<16, 2>(S2, p.Y) : f28_sub<8, 1>(S2, p.Y);
-  if (f28_is_zero_mod_p<18>(P)) {
+  if (__builtin_expect(f28_is_zero_mod_p<18>(P), 0)) {
     if (f28_is_zero_mod_p<16>(R)) {
       if constexpr (REDUCED) {
         const g1x28 q = g1x28_from_reduced(x, y, neg);

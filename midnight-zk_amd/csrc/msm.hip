@@ -607,8 +607,18 @@ __global__ void __launch_bounds__(64) k_msm_accum(const uint32_t* __restrict__ r
       acc = g1x28_inf();
       cb++;
       next = next2;
-      while (next <= e) {   // empty buckets (small sets)
+      uint32_t steps = 0;
+      while (next <= e && ++steps < 16) {   // a few empty buckets (small sets)
         cb++;
+        next = off[cb + 1];
+      }
+      if (__builtin_expect(next <= e, 0)) {   // a long empty run (skewed scalars): bisection
+        uint32_t lo = cb, hi = NB;           // off[lo] <= e < off[hi] (off[NB] = M > e)
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (off[mid] <= e) lo = mid; else hi = mid;
+        }
+        cb = lo;
         next = off[cb + 1];
       }
       next2 = off[std::min(cb + 2, NB)];
