@@ -661,13 +661,15 @@ __device__ __forceinline__ g1x28 shfl_xor_x28(const g1x28& a, int d) {
 }
 
 // A workgroup of four waves per 64 buckets: every wave reads the 64 buckets' spans (lane
-// i: bucket 64 blockIdx + i) and takes every fourth long bucket; its lanes sum the
-// bucket's partials strided by 64 (each lane's first slot is its own), then a butterfly
-// over the lanes by shuffles (no LDS, no barrier: each wave runs alone) leaves the
-// bucket's sum in its first slot.  With no long bucket a workgroup reads 65 offsets and
-// leaves.
+// i: bucket 64 blockIdx + i).  Four or more long buckets: each wave takes every fourth
+// one, its lanes sum the bucket's partials strided by 64 (each lane's first slot is its
+// own) and a butterfly over the lanes by shuffles (no barrier: each wave runs alone) leaves
+// the sum in the bucket's first slot.  Fewer (one hot bucket is the common skewed case):
+// the four waves share each bucket, lanes strided by 256, and add their four sums through
+// LDS.  With no long bucket a workgroup reads 65 offsets and leaves.
 __global__ void __launch_bounds__(256) k_long_merge(const uint32_t* __restrict__ off, uint32_t NB,
                                                     uint32_t* __restrict__ partials) {
+  __shared__ uint32_t lds[4 * MZK_X28_WORDS];
   const uint32_t K = off[NB + 1];
   const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
   const uint32_t b_lane = blockIdx.x * 64 + lane;
@@ -679,17 +681,29 @@ __global__ void __launch_bounds__(256) k_long_merge(const uint32_t* __restrict__
       n = (hi - 1) / K - k0 + 1;
     }
   }
-  uint64_t todo = __ballot(n > LONG_CHUNKS);
-  for (uint32_t r = 0; todo; r++) {   // uniform in the wave: the ballot is the wave's
+  uint64_t todo = __ballot(n > LONG_CHUNKS);   // the same in every wave of the workgroup
+  const bool share = __popcll(todo) < 4;
+  for (uint32_t r = 0; todo; r++) {
     const uint32_t i = (uint32_t)__ffsll((long long)todo) - 1;
     todo &= todo - 1;
-    if ((r & 3u) != wv) continue;
+    if (!share && (r & 3u) != wv) continue;
     const uint32_t first = (uint32_t)__shfl((int)k0, (int)i) + blockIdx.x * 64 + i;   // slot k0 + b
     const uint32_t cnt = (uint32_t)__shfl((int)n, (int)i);                             // > 64 slots
-    g1x28 x = load_x28(partials + MZK_X28_WORDS * (size_t)(first + lane));
-    for (uint32_t k = lane + 64; k < cnt; k += 64) x = g1x28_add(x, load_x28(partials + MZK_X28_WORDS * (size_t)(first + k)));
+    const uint32_t me = share ? threadIdx.x : lane, stride = share ? 256u : 64u;
+    g1x28 x = g1x28_inf();
+    for (uint32_t k = me; k < cnt; k += stride) x = g1x28_add(x, load_x28(partials + MZK_X28_WORDS * (size_t)(first + k)));
     for (int d = 32; d >= 1; d >>= 1) x = g1x28_add(x, shfl_xor_x28(x, d));
-    if (lane == 0) store_x28_raw(partials + MZK_X28_WORDS * (size_t)first, x);
+    if (share) {   // uniform: every wave of the workgroup is here
+      if (lane == 0) store_x28_raw(lds + MZK_X28_WORDS * wv, x);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; w++) x = g1x28_add(x, load_x28(lds + MZK_X28_WORDS * w));
+        store_x28_raw(partials + MZK_X28_WORDS * (size_t)first, x);
+      }
+      __syncthreads();   // the LDS is reused by the next bucket
+    } else if (lane == 0) {
+      store_x28_raw(partials + MZK_X28_WORDS * (size_t)first, x);
+    }
   }
 }
 
