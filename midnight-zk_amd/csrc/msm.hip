@@ -304,7 +304,7 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __r
   }
 }
 
-static constexpr int BINS_UNROLL = 8;
+static constexpr int BINS_UNROLL = 16;
 static constexpr size_t BINS_LDS_CU = 160u << 10;   // LDS per CU
 
 // 4. per top-bit bin of a sort group (workgroup g D1 + bin): counting sort on the low
