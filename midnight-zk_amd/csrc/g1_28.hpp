@@ -286,7 +286,11 @@ __device__ __forceinline__ g1x28 g1x28_add_quad(const g1x28 p, const uint32_t* _
   m = f28_mul_sum(f28_pick(l0, ZZZ12, R), f28_pick(l0, PPP, T), f28_pick(l0, z, S1), f28_pick(l0, z, nPPP));
   r.ZZZ = f28_quad_bcast<0>(m);
   r.Y = f28_quad_bcast<1>(m);
-  if (f28_is_zero_mod_p<6>(P)) {   // no struct-valued ?: here: it would go through memory
+  // An identity operand also gives P = R = 0; the selects below replace the result then,
+  // so only two finite points take the doubling, which would otherwise run on the whole
+  // wave whenever one quad adds an identity.
+  const bool equal_x = !p_inf && !q_inf && f28_is_zero_mod_p<6>(P);
+  if (__builtin_expect(equal_x, 0)) {   // no struct-valued ?: here: it would go through memory
     if (f28_is_zero_mod_p<6>(R)) r = g1x28_dbl(p);
     else r = g1x28_inf();
   }
