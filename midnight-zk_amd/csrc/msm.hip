@@ -980,29 +980,32 @@ __global__ void __launch_bounds__(4 * CT_TOP_QUADS) k_chan_top(const uint32_t* _
 }
 
 // The same tree for sets of segments (L > 1, T = 2^t segments per set, t >= 7): leaves are
-// the segments' (R_t, A_t), a lane each, ST_LEAVES per block; a node also carries Asum, the
-// sum of its A_t (channel 0).  Level j has (ST_LEAVES / 2 >> j)(j + 2) one-lane additions.
+// the segments' (R_t, A_t), ST_LEAVES per block; a node also carries Asum, the sum of its
+// A_t (channel 0).  Level j has (ST_LEAVES / 2 >> j)(j + 2) additions, a quad each.
 // The records (P, Asum, A_0..A_5) of a set's T / ST_LEAVES blocks go to k_seg_top.
 static constexpr uint32_t ST_LEVELS = 6;
 static constexpr uint32_t ST_LEAVES = 1u << ST_LEVELS;
 static constexpr uint32_t ST_REC = ST_LEVELS + 2;
+static constexpr uint32_t ST_TOP_QUADS = 128;   // most quads of a k_seg_top block
 
 __device__ __forceinline__ uint32_t st_acc_slot(uint32_t i, uint32_t m) {
   return 2 * ST_LEAVES + (ST_LEAVES - (ST_LEAVES >> i)) + m;
 }
 
 // LDS: P (ST_LEAVES slots, in place), Asum (ST_LEAVES), the A_i (ST_LEAVES - 1).
-__global__ void __launch_bounds__(ST_LEAVES) k_seg_tree(const uint32_t* __restrict__ A,
-                                                       const uint32_t* __restrict__ R,
-                                                       uint32_t* __restrict__ out) {
+__global__ void __launch_bounds__(4 * ST_LEAVES) k_seg_tree(const uint32_t* __restrict__ A,
+                                                           const uint32_t* __restrict__ R,
+                                                           uint32_t* __restrict__ out) {
   extern __shared__ uint32_t lds[];
-  const uint32_t t = threadIdx.x;
-  const size_t leaf = (size_t)blockIdx.x * ST_LEAVES + t;
-  for (uint32_t e = 0; e < MZK_X28_WORDS; e += 4) {
-    *reinterpret_cast<uint4*>(lds + MZK_X28_WORDS * t + e) =
-        *reinterpret_cast<const uint4*>(R + MZK_X28_WORDS * leaf + e);
-    *reinterpret_cast<uint4*>(lds + MZK_X28_WORDS * (ST_LEAVES + t) + e) =
-        *reinterpret_cast<const uint4*>(A + MZK_X28_WORDS * leaf + e);
+  const uint32_t t = threadIdx.x, qd = t >> 2, ql = t & 3u;
+  const size_t leaf0 = (size_t)blockIdx.x * ST_LEAVES;
+  constexpr uint32_t V = MZK_X28_WORDS / 4;   // uint4 per point
+  for (uint32_t w = t; w < ST_LEAVES * V; w += blockDim.x) {
+    const uint32_t leaf = w / V, e = 4 * (w % V);
+    *reinterpret_cast<uint4*>(lds + MZK_X28_WORDS * leaf + e) =
+        *reinterpret_cast<const uint4*>(R + MZK_X28_WORDS * (leaf0 + leaf) + e);
+    *reinterpret_cast<uint4*>(lds + MZK_X28_WORDS * (ST_LEAVES + leaf) + e) =
+        *reinterpret_cast<const uint4*>(A + MZK_X28_WORDS * (leaf0 + leaf) + e);
   }
   __syncthreads();
   for (uint32_t j = 0; j < ST_LEVELS; j++) {
@@ -1011,14 +1014,16 @@ __global__ void __launch_bounds__(ST_LEAVES) k_seg_tree(const uint32_t* __restri
       const uint32_t m = w / MZK_X28_WORDS, e = w % MZK_X28_WORDS;
       lds[MZK_X28_WORDS * st_acc_slot(j, m) + e] = lds[MZK_X28_WORDS * (2 * m + 1) + e];
     }
-    // task (m, u): u = 0 adds the children's P, u = 1 their Asum, u = i + 2 their A_i (i < j)
-    const bool act = t < parents * per;
-    const uint32_t m = t / per, u = t % per;
+    // task (m, u) of quad qd: u = 0 adds the children's P, u = 1 their Asum, u = i + 2
+    // their A_i (i < j); the A_j slots written above are not read this level
+    const bool act = qd < parents * per;
+    const uint32_t m = qd / per, u = qd % per;
     const uint32_t s0 = u == 0 ? 2 * m : u == 1 ? ST_LEAVES + 2 * m : st_acc_slot(u - 2, 2 * m);
     g1x28 x = g1x28_inf();
-    if (act) x = g1x28_add(load_x28(lds + MZK_X28_WORDS * s0), load_x28(lds + MZK_X28_WORDS * (s0 + 1)));
+    if (act) x = g1x28_add_quad(load_x28(lds + MZK_X28_WORDS * s0), lds + MZK_X28_WORDS * (s0 + 1));
     __syncthreads();
-    if (act) store_x28_raw(lds + MZK_X28_WORDS * (u == 0 ? m : u == 1 ? ST_LEAVES + m : st_acc_slot(u - 2, m)), x);
+    if (act && ql == 0)
+      store_x28_raw(lds + MZK_X28_WORDS * (u == 0 ? m : u == 1 ? ST_LEAVES + m : st_acc_slot(u - 2, m)), x);
     __syncthreads();
   }
   for (uint32_t w = t; w < ST_REC * MZK_X28_WORDS; w += blockDim.x) {
@@ -1030,19 +1035,36 @@ __global__ void __launch_bounds__(ST_LEAVES) k_seg_tree(const uint32_t* __restri
 
 // Channel h of set s from its nrec block records (block (s nch + h)): h = 0 sums the Asum,
 // h = 1 + i (i < ST_LEVELS) the A_i, h = 1 + ST_LEVELS + j the P of the records whose
-// index has bit j set.  A lane per record (blockDim.x >= nrec), a tree sum, then the
-// blst-layout XYZZ of the channel at out[48 (s nch + h)].
-__global__ void __launch_bounds__(256) k_seg_top(const uint32_t* __restrict__ in, uint32_t nrec, uint32_t nch,
-                                                 uint32_t* __restrict__ out) {
+// index has bit j set.  nq = blockDim.x / 4 quads (a power of two): quad q takes records
+// q, q + nq, ... (the first of them loaded, the rest added), then a tree over the quads in
+// LDS; writes the blst-layout XYZZ of the channel at out[48 (s nch + h)].
+__global__ void __launch_bounds__(4 * ST_TOP_QUADS) k_seg_top(const uint32_t* __restrict__ in, uint32_t nrec,
+                                                             uint32_t nch, uint32_t* __restrict__ out) {
   extern __shared__ uint32_t lds[];
-  const uint32_t k = threadIdx.x;
+  const uint32_t qd = threadIdx.x >> 2, ql = threadIdx.x & 3u, nq = blockDim.x >> 2;
   const uint32_t s = blockIdx.x / nch, h = blockIdx.x % nch;
   const uint32_t slot = h == 0 ? 1 : h <= ST_LEVELS ? h + 1 : 0;
+  const bool all = h <= ST_LEVELS;
+  const uint32_t bit = all ? 0 : h - 1 - ST_LEVELS;
+  const uint32_t* rec = in + MZK_X28_WORDS * ((size_t)s * nrec * ST_REC + slot);
+  uint32_t k = qd;
+  while (k < nrec && !all && !((k >> bit) & 1u)) k += nq;
   g1x28 x = g1x28_inf();
-  if (k < nrec && (h <= ST_LEVELS || ((k >> (h - 1 - ST_LEVELS)) & 1u)))
-    x = load_x28(in + MZK_X28_WORDS * (((size_t)s * nrec + k) * ST_REC + slot));
-  x = block_sum(x, lds);
-  if (k == 0) {
+  if (k < nrec) {
+    x = load_x28(rec + MZK_X28_WORDS * (size_t)k * ST_REC);
+    for (k += nq; k < nrec; k += nq)
+      if (all || ((k >> bit) & 1u)) x = g1x28_add_quad(x, rec + MZK_X28_WORDS * (size_t)k * ST_REC);
+  }
+  if (ql == 0) store_x28_raw(lds + MZK_X28_WORDS * qd, x);
+  __syncthreads();
+  for (uint32_t d = nq / 2; d >= 1; d >>= 1) {   // reads slots >= d, writes < d
+    if (qd < d) {
+      x = g1x28_add_quad(x, lds + MZK_X28_WORDS * (qd + d));
+      if (ql == 0) store_x28_raw(lds + MZK_X28_WORDS * qd, x);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
     uint32_t o[48];
     g1x28_to_std(x, o);
     uint4* qv = reinterpret_cast<uint4*>(out + 48 * (size_t)blockIdx.x);
@@ -1583,11 +1605,13 @@ int msm_run_jobs(mzk_ctx* c, const MsmPlan& p, const std::vector<MsmJob>& jobs, 
         MZK_LAUNCH(c, "k_bucket_seg", k_bucket_seg, dim3(grid_for(nseg, 64)), dim3(64), 0, off, NB, L, partials,
                    segA, segR);
       if (segtree) {   // the segments' channels by the block tree: records into the chunk partials' space
-        const uint32_t nrec = T / ST_LEAVES, top_threads = std::max<uint32_t>(64, nrec);
-        MZK_LAUNCH(c, "k_seg_tree", k_seg_tree, dim3((unsigned)(nseg / ST_LEAVES)), dim3(ST_LEAVES),
+        // nrec = T / ST_LEAVES is a power of two >= 2; k_seg_top's quads: min(nrec, 128), at least 16
+        const uint32_t nrec = T / ST_LEAVES;
+        const uint32_t top_quads = std::min<uint32_t>(ST_TOP_QUADS, std::max<uint32_t>(16, nrec));
+        MZK_LAUNCH(c, "k_seg_tree", k_seg_tree, dim3((unsigned)(nseg / ST_LEAVES)), dim3(4 * ST_LEAVES),
                    (3 * ST_LEAVES - 1) * MZK_X28_WORDS * 4, segA, segR, partials);
-        MZK_LAUNCH(c, "k_seg_top", k_seg_top, dim3((unsigned)nchan), dim3(top_threads),
-                   top_threads * MZK_X28_WORDS * 4, partials, nrec, nch, chan);
+        MZK_LAUNCH(c, "k_seg_top", k_seg_top, dim3((unsigned)nchan), dim3(4 * top_quads),
+                   top_quads * MZK_X28_WORDS * 4, partials, nrec, nch, chan);
       } else {
         MZK_LAUNCH(c, "k_chan_sum", k_chan_sum, dim3((unsigned)(nchan * parts)), dim3(CHAN_BLOCK),
                    CHAN_BLOCK * MZK_X28_WORDS * 4, segA, segR, T, nch, parts, chan_parts);
