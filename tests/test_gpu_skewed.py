@@ -125,3 +125,27 @@ def test_sparse_columns(ctx, srs, density):
     outs = s.commit_batch([(to_dev(c), n, 0) for c in cols])
     for c, out in zip(cols, outs):
         assert aff(out) == aff(C.msm_best(c, bases[:n]))
+
+
+@pytest.mark.parametrize("ones,pattern", [(120, "same"), (120, "alternate"), (1 << 16, "same")])
+def test_repeated_points_in_one_bucket(ctx, ones, pattern):
+    """A column of ones over repeated bases: 120 ones take 40 chunks of 3, whose partials
+    are equal points (the quad addition's doubling) or, over alternating P and -P, opposite
+    points (its cancellation to the identity); every other bucket is empty, so the channel
+    trees add identity operands throughout.  2^16 ones of one point: the long-bucket merge
+    of equal partials.  Against msm_best on the same bases."""
+    tau = C.fr_from_seed(78, 0, 1)[0]
+    n = 1 << 16
+    bs = C.srs_bases(tau, n)
+    p = bs[5].copy()
+    neg = np.array(O.g1_affine_mont_limbs(O.g1_neg(O.g1_affine_from_mont_limbs(p))), dtype=np.uint64)
+    for i in range(ones):
+        bs[i] = neg if pattern == "alternate" and i % 2 else p
+    s = ctx.srs_upload(bs)
+    sc = C.ints_to_fr_rows([1] * ones + [0] * (n - ones))
+    got = aff(s.commit(to_dev(sc), n))
+    plan = ctx.msm_last_plan()
+    s.close()
+    assert got == aff(C.msm_best(sc, bs)), (ones, pattern, plan)
+    if ones == 120:
+        assert plan["refs_per_thread"] == 3 and plan["digits"] == ones, plan
